@@ -1,0 +1,62 @@
+"""Dense GPU path (reference IDs 0-4, gpu_exact_dense.cu) against the oracle.
+
+Tolerances: FP64 results must agree with the exact permanent to 1e-10
+relative (or to the stated error bound when the sum is ill-conditioned),
+0/1 matrices with permanent < 2^53 must round to the exact integer."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("n", list(range(1, 21)))
+def test_dense_random_vs_exact(n):
+    a = P.synthetic_matrix(n, 0.6, seed=100 + n)
+    ex = float(O.exact_permanent(a))
+    r = P.dense(a, 4)
+    assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_bound), (n, r.value, ex)
+    assert abs(r.value - ex) <= r.err_bound + 1e-300
+
+
+@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4])
+def test_dense_all_ids_same_value(algo):
+    a = P.synthetic_matrix(14, 0.5, seed=7)
+    ex = float(O.exact_permanent(a))
+    v = P.dropin_dense(a, algo)
+    assert rel(v, ex) < 1e-10
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 20, 24])
+def test_binary_rounds_to_exact_integer(n):
+    a = P.synthetic_matrix(n, 0.5, seed=300 + n, binary=True)
+    ex = O.exact_permanent(a)
+    assert ex.denominator == 1
+    r = P.dense(a.astype(np.int32), 1)
+    assert round(r.value) == ex.numerator
+    assert abs(r.value - ex.numerator) < 0.5
+
+
+@pytest.mark.parametrize("n", [5, 9, 12])
+def test_closed_forms(n):
+    J = np.ones((n, n))
+    assert P.dense(J).value == pytest.approx(math.factorial(n), rel=1e-13)
+    I = np.eye(n)
+    assert P.dense(I).value == pytest.approx(1.0, rel=1e-13)
+    D = J - I  # derangements
+    d = round(math.factorial(n) * sum((-1) ** k / math.factorial(k) for k in range(n + 1)))
+    assert P.dense(D).value == pytest.approx(d, rel=1e-13)
+
+
+def test_grid_graph_tilings():
+    for m, k in [(4, 4), (4, 6), (6, 6)]:
+        G = O.grid_graph_matrix(m, k)
+        assert round(P.dense(G.astype(np.int32)).value) == O.domino_tilings(m, k)
