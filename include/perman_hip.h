@@ -39,7 +39,9 @@ extern "C" {
 /* Result of one permanent.  value = scale * (sum_hi + sum_lo) where
  * scale = 4(n&1)-2 and sum is the double-double Ryser sum; abs_sum is
  * |scale| * sum of |terms| (the condition of the alternating sum) and
- * err_bound the library's stated error for `value`.  mode: 0 = FP64 products
+ * err_bound the library's stated worst-case error for `value` (every rounding
+ * aligned); err_est the expected error when roundings are independent,
+ * 3 (n + sqrt(n)) u abs_sum / sqrt(2^(n-1)).  mode: 0 = FP64 products
  * (the fast kernel), 1 = double-double products (used for integer-valued
  * matrices whose FP64 result could not be guaranteed to round to the exact
  * integer). */
@@ -55,6 +57,7 @@ typedef struct perm_result {
   int status;
   int mode;
   int integral;         /* 1 if every entry is an integer (X is then exact) */
+  double err_est;       /* statistical error estimate (independent roundings) */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */
@@ -140,6 +143,11 @@ int perm_plan_dense(int dtype, const void* mat, int nov, perm_plan** plan);
 int perm_plan_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                      const void* cvals, int nov, perm_plan** plan);
 int perm_plan_run(perm_plan* plan, perm_result* out);
+/* Run one kernel mode explicitly: 0 FP64 (production), 1 double-double
+ * products and sums, 2 double-double X as well (no X drift; verification
+ * mode, several times slower).  perm_plan_run picks 0, then 1 when an
+ * integer matrix needs it to round to the exact integer. */
+int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out);
 void perm_plan_destroy(perm_plan* plan);
 
 /* ---- CPU path of `perman -c` (parallel_perman64, algo.h:663; and

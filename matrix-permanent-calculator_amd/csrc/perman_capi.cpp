@@ -253,8 +253,8 @@ int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows
 }
 
 // Launch one pass (FP64 or double-double products) and read the result.
-int run_pass(perm_plan* P, bool accurate, perm::Partial& r, float& ms) {
-  P->L.accurate = accurate ? 1 : 0;
+int run_pass(perm_plan* P, int mode, perm::Partial& r, float& ms) {
+  P->L.mode = mode;
   HIP_TRY(hipEventRecord(P->ev0, P->stream));
   HIP_TRY(perm::launch_dense(P->L, P->stream));
   HIP_TRY(hipEventRecord(P->ev1, P->stream));
@@ -264,7 +264,7 @@ int run_pass(perm_plan* P, bool accurate, perm::Partial& r, float& ms) {
   return PERM_OK;
 }
 
-void fill_result(const perm_plan* P, const perm::Partial& r, bool accurate, perm_result* out) {
+void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_result* out) {
   const int n = P->n;
   const double scale = (double)(4 * (n & 1) - 2);
   out->n = n;
@@ -273,27 +273,32 @@ void fill_result(const perm_plan* P, const perm::Partial& r, bool accurate, perm
   out->value = scale * (r.hi + r.lo);
   out->abs_sum = std::fabs(scale) * r.abs_sum;
   out->err_bound = perm::error_statement(out->abs_sum, n, P->L.walk_bits, P->unroll_bits,
-                                         P->integral, accurate);
+                                         P->integral, mode);
   out->subsets = std::ldexp(1.0, n - 1);
+  out->err_est = perm::error_estimate(out->abs_sum, n, mode);
   out->status = PERM_OK;
-  out->mode = accurate ? 1 : 0;
+  out->mode = mode;
   out->integral = P->integral ? 1 : 0;
 }
 
-// FP64 pass; for an integer matrix whose result is below 2^53 but whose
-// stated error could exceed 1/4, a second pass with double-double products
-// makes the value round to the exact integer.
-int plan_run_impl(perm_plan* P, perm_result* out) {
+// mode < 0 (automatic): FP64 pass; for an integer matrix whose result is
+// below 2^53 but whose stated error could reach 1/4, a second pass with
+// double-double products makes the value round to the exact integer.
+// mode 0/1/2 runs exactly that kernel mode.
+int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
   if (!P || !out) return fail(PERM_ERR_ARG, "null pointer");
+  if (mode > 2) return fail(PERM_ERR_ARG, "unknown mode");
   HIP_TRY(hipSetDevice(P->device));
   perm::Partial r;
   float ms = 0.f;
-  if (int s = run_pass(P, false, r, ms)) return s;
-  fill_result(P, r, false, out);
+  const int first = mode < 0 ? 0 : mode;
+  if (int s = run_pass(P, first, r, ms)) return s;
+  fill_result(P, r, first, out);
   double total_ms = ms;
-  if (P->integral && std::fabs(out->value) < 9007199254740992.0 && out->err_bound >= 0.25) {
-    if (int s = run_pass(P, true, r, ms)) return s;
-    fill_result(P, r, true, out);
+  if (mode < 0 && P->integral && std::fabs(out->value) < 9007199254740992.0 &&
+      out->err_bound >= 0.25) {
+    if (int s = run_pass(P, 1, r, ms)) return s;
+    fill_result(P, r, 1, out);
     total_ms += ms;
   }
   out->kernel_ms = total_ms;
@@ -328,7 +333,7 @@ int run_oneshot(int n, const std::vector<double>& cs, const std::vector<double>&
   }
   bind_plan_memory(&P, C.base, cs.size());
   if (int s = upload_plan(&P, cs, x0)) return s;
-  return plan_run_impl(&P, out);
+  return plan_run_impl(&P, -1, out);
 }
 
 double nan_result() { return std::numeric_limits<double>::quiet_NaN(); }
@@ -378,7 +383,10 @@ int perm_plan_sparse(int dtype, const void* mat, const int* cptrs, const int* ro
   (*plan)->integral = integral;
   return PERM_OK;
 }
-int perm_plan_run(perm_plan* plan, perm_result* out) { return plan_run_impl(plan, out); }
+int perm_plan_run(perm_plan* plan, perm_result* out) { return plan_run_impl(plan, -1, out); }
+int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out) {
+  return plan_run_impl(plan, mode, out);
+}
 void perm_plan_destroy(perm_plan* P) {
   if (!P || P->borrowed) return;
   (void)hipSetDevice(P->device);

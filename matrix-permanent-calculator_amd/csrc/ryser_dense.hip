@@ -46,27 +46,46 @@ __device__ __forceinline__ void load_col(double (&cv)[N], ColPtr base, int blk) 
   for (int j = 0; j < N; ++j) cv[j] = c[j];
 }
 
-// Per-lane accumulator.  FAST: plain double `p` folded into double-double
-// every 32 groups.  ACC: the signed sum is kept as an unevaluated pair
-// (p, plo) updated with TwoSum on every term.
+// Kernel modes.
+//  kFast: FP64 X, FP64 products (the production kernel).
+//  kDDProd: FP64 X, double-double products and lane sums -- exact products
+//    for integer matrices (whose X is exact), used to round 0/1 results to
+//    the exact integer.
+//  kDDAll: X itself kept as a double-double (every column addition through
+//    TwoSum), double-double products and sums: no X drift; a verification
+//    mode for large n where no CPU oracle finishes.
+enum Mode : int { kFast = 0, kDDProd = 1, kDDAll = 2 };
+
+#ifndef PERM_FAST_WAVES_PER_EU
+#define PERM_FAST_WAVES_PER_EU 4
+#endif
+// Minimum waves per SIMD the fast kernel is compiled for (caps its VGPRs at
+// 512/waves): enough waves to cover the scalar-load latency of the column
+// prefetch.
+constexpr int kFastWavesPerEU = PERM_FAST_WAVES_PER_EU;
+
+// Per-lane accumulator.  kFast: plain double `p` folded into double-double
+// every 32 groups.  Other modes: the signed sum is kept as an unevaluated
+// pair (p, plo) updated with TwoSum on every term.
 struct LaneAcc {
   double p = 0.0, plo = 0.0, a = 0.0;
 };
-
 // p += sign * prod(x);  a += |prod(x)|.
-// FAST: the product of x[0..N-2] runs as four interleaved multiply chains
+// kFast: the product of x[0..N-2] runs as four interleaved multiply chains
 // (independent work for the scheduler), N-2 multiplies; the last factor and
 // the signed accumulation are one FMA.
-// ACC: the product is a double-double (TwoProd through FMA: hi*x exact as
-// hi' + e), two interleaved chains joined by a double-double multiply; the
-// term is added to (p, plo) with TwoSum.  For integer inputs x is exact and
-// each term carries a relative error of a few 2^-104.
-template <int N, bool NEG, bool ACC>
-__device__ __forceinline__ void accumulate_term(const double (&x)[N], LaneAcc& acc) {
+// kDDProd / kDDAll: the product is a double-double (TwoProd through FMA:
+// hi*x = hi' + e exactly), two interleaved chains joined by a double-double
+// multiply; the term enters (p, plo) through TwoSum.  In kDDAll each factor
+// is the pair (x[j], xl[j]).
+template <int N, bool NEG, int MODE>
+__device__ __forceinline__ void accumulate_term(const double (&x)[N], const double (&xl)[N],
+                                                LaneAcc& acc) {
   if constexpr (N == 1) {
-    acc.p += NEG ? -x[0] : x[0];
-    acc.a += __builtin_fabs(x[0]);
-  } else if constexpr (!ACC) {
+    const double v = MODE == kDDAll ? x[0] + xl[0] : x[0];
+    acc.p += NEG ? -v : v;
+    acc.a += __builtin_fabs(v);
+  } else if constexpr (MODE == kFast) {
     constexpr int M = N - 1;  // factors before the last one
     double c[4];
 #pragma unroll
@@ -81,15 +100,20 @@ __device__ __forceinline__ void accumulate_term(const double (&x)[N], LaneAcc& a
     const double last = x[N - 1];
     acc.p = __builtin_fma(NEG ? -pp : pp, last, acc.p);
     acc.a = __builtin_fma(__builtin_fabs(pp), __builtin_fabs(last), acc.a);
+    // Pin the |term| accumulation to this step: left alone, the compiler
+    // defers it to the end of the group and keeps 2 registers per step live.
+    asm volatile("" : "+v"(acc.a));
   } else {
     // chain 0 over even j, chain 1 over odd j
-    double h0 = x[0], l0 = 0.0, h1 = x[1], l1 = 0.0;
+    double h0 = x[0], l0 = MODE == kDDAll ? xl[0] : 0.0;
+    double h1 = x[1], l1 = MODE == kDDAll ? xl[1] : 0.0;
 #pragma unroll
     for (int j = 2; j < N; ++j) {
       double& h = (j & 1) ? h1 : h0;
       double& l = (j & 1) ? l1 : l0;
       const double ph = h * x[j];
-      const double e = __builtin_fma(h, x[j], -ph);
+      double e = __builtin_fma(h, x[j], -ph);
+      if constexpr (MODE == kDDAll) e = __builtin_fma(h, xl[j], e);
       l = __builtin_fma(l, x[j], e);
       h = ph;
     }
@@ -111,18 +135,34 @@ __device__ __forceinline__ void accumulate_term(const double (&x)[N], LaneAcc& a
   }
 }
 
+// x += column.  kDDAll: (x, xl) += c through TwoSum, the rounding error of
+// every addition is carried in xl (exact, never renormalised).
+template <int N, int MODE>
+__device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], const double (&cv)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if constexpr (MODE == kDDAll) {
+      const double s = x[j] + cv[j];
+      const double bb = s - x[j];
+      xl[j] += (x[j] - (s - bb)) + (cv[j] - bb);
+      x[j] = s;
+    } else {
+      x[j] += cv[j];
+    }
+  }
+}
+
 // Step v of group g (step index t = g*2^R + v).  Phase A adds the column
 // that was prefetched into SGPRs by the previous step; phase B issues the
 // scalar loads of the next step's column and, while they are in flight,
 // forms the product and accumulates the term.  sched_barrier keeps the two
 // phases apart so the loads always overlap a product.
-template <int N, int R, bool ACC, int V>
-__device__ __forceinline__ void group_steps(double (&x)[N], double (&cv)[N], ColPtr cols,
-                                            uint32_t g, uint32_t groups, int zero_blk,
-                                            LaneAcc& acc) {
+template <int N, int R, int MODE, int V>
+__device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], double (&cv)[N],
+                                            ColPtr cols, uint32_t g, uint32_t groups,
+                                            int zero_blk, LaneAcc& acc) {
   if constexpr (V < (1 << R)) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) x[j] += cv[j];
+    add_column<N, MODE>(x, xl, cv);
     __builtin_amdgcn_sched_barrier(0);
     int blk;
     if constexpr (V + 1 < (1 << R)) {
@@ -143,16 +183,17 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&cv)[N], Col
       blk = gn < groups ? 2 * (R + c) + neg : zero_blk;
     }
     load_col<N>(cv, cols, blk);
-    accumulate_term<N, (V & 1) != 0, ACC>(x, acc);
+    accumulate_term<N, (V & 1) != 0, MODE>(x, xl, acc);
     __builtin_amdgcn_sched_barrier(0);
-    group_steps<N, R, ACC, V + 1>(x, cv, cols, g, groups, zero_blk, acc);
+    group_steps<N, R, MODE, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc);
   }
 }
 
 // colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
 // Gray columns k < n-1, block 2(n-1) = zeros.
-template <int N, int R, bool ACC>
-__global__ __launch_bounds__(256) void ryser_dense_kernel(const double* __restrict__ colpm,
+template <int N, int R, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
+void ryser_dense_kernel(const double* __restrict__ colpm,
                                                           const double* __restrict__ x0,
                                                           Partial* __restrict__ partials,
                                                           uint64_t lanes, int n_walk_bits,
@@ -162,19 +203,25 @@ __global__ __launch_bounds__(256) void ryser_dense_kernel(const double* __restri
   const ColPtr cols = (ColPtr)colpm;
   const int zero_blk = 2 * (N - 1);
 
-  double x[N];
+  double x[N], xl[N];
 #pragma unroll
-  for (int j = 0; j < N; ++j) x[j] = x0[j];
+  for (int j = 0; j < N; ++j) {
+    x[j] = x0[j];
+    xl[j] = 0.0;
+  }
 
-  // Closed-form start: add the high columns selected by the lane id.
+  // Closed-form start: add the high columns selected by the lane id
+  // (adding sel*c with sel in {0, 1} is exact selection, no branch).
   int parity = 0;
   for (int b = 0; b < n_high_bits; ++b) {
     ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
     const int bit = (int)((lane >> b) & 1);
     parity ^= bit;
     const double sel = bit ? 1.0 : 0.0;
+    double cv[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) x[j] = __builtin_fma(sel, c[j], x[j]);
+    for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
+    add_column<N, MODE>(x, xl, cv);
   }
 
   DD s = {0.0, 0.0};
@@ -183,16 +230,16 @@ __global__ __launch_bounds__(256) void ryser_dense_kernel(const double* __restri
   double cv[N];
   load_col<N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
   for (uint32_t g = 0; g < groups; ++g) {
-    group_steps<N, R, ACC, 0>(x, cv, cols, g, groups, zero_blk, acc);
-    if constexpr (!ACC) {
+    group_steps<N, R, MODE, 0>(x, xl, cv, cols, g, groups, zero_blk, acc);
+    if constexpr (MODE == kFast) {
       if ((g & 31u) == 31u) {  // fold into double-double every 32 groups
         s = dd_add_d(s, acc.p);
         acc.p = 0.0;
       }
     }
   }
-  if constexpr (ACC) s = quick_two_sum(acc.p, acc.plo);
-  else s = dd_add_d(s, acc.p);
+  if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
+  else s = two_sum(acc.p, acc.plo);
   double a = acc.a;
   if (parity) {
     s.hi = -s.hi;
@@ -224,11 +271,15 @@ template <int N>
 static hipError_t launch_dense_n(const DenseLaunch& L, hipStream_t stream) {
   constexpr int R = N - 1 < 3 ? N - 1 : 3;
   if (L.walk_bits < R) return hipErrorInvalidValue;
-  if (L.accurate)
-    ryser_dense_kernel<N, R, true><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
+  const dim3 grid((unsigned)L.blocks), block(256);
+  if (L.mode == kDDProd)
+    ryser_dense_kernel<N, R, kDDProd><<<grid, block, 0, stream>>>(
+        L.colpm, L.x0, L.partials, L.lanes, L.walk_bits, L.high_bits);
+  else if (L.mode == kDDAll)
+    ryser_dense_kernel<N, R, kDDAll><<<grid, block, 0, stream>>>(
         L.colpm, L.x0, L.partials, L.lanes, L.walk_bits, L.high_bits);
   else
-    ryser_dense_kernel<N, R, false><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
+    ryser_dense_kernel<N, R, kFast><<<grid, block, 0, stream>>>(
         L.colpm, L.x0, L.partials, L.lanes, L.walk_bits, L.high_bits);
   return hipGetLastError();
 }

@@ -74,17 +74,29 @@ inline Split choose_split(int n, int unroll_bits, int walk_override) {
 //    plus up to 2^w signed column additions.  Estimated as a random walk of
 //    rounding errors, n sqrt(2^w) u A.  This term is an estimate, not a
 //    bound; for integer matrices X is exact and it vanishes.
-// With double-double products (accurate mode) the first two terms shrink
-// to (n + 8 + 2^(w+1)) 2^-104 A.
+// With double-double products (mode 1) the first two terms shrink to
+// (n + 8 + 2^(w+1)) 2^-104 A; with X in double-double as well (mode 2) the
+// drift term vanishes too.
 inline double error_statement(double abs_sum_scaled, int n, int walk_bits, int unroll_bits,
-                              bool integral, bool accurate) {
+                              bool integral, int mode) {
   const double u = std::ldexp(1.0, -53);
+  const double dd_terms = std::ldexp((double)n + 8.0 + std::ldexp(2.0, walk_bits), -104);
+  if (mode == 2) return abs_sum_scaled * 4.0 * dd_terms;
   const double drift = integral ? 0.0 : n * std::sqrt(std::ldexp(1.0, walk_bits));
-  if (accurate)
-    return abs_sum_scaled *
-           (std::ldexp((double)n + 8.0 + std::ldexp(2.0, walk_bits), -104) + u * drift);
+  if (mode == 1) return abs_sum_scaled * (dd_terms + u * drift);
   const double fold = std::ldexp(1.0, unroll_bits + 5);
   return abs_sum_scaled * u * ((double)n + 1.0 + drift + fold);
+}
+
+// Expected error when the roundings behave as independent random errors
+// (the alternating Gray sum turns the correlated X drift of a lane into an
+// effectively independent error per step): about (n + sqrt(n)) u A / sqrt(N)
+// with N = 2^(n-1) terms; reported with a factor 3.  Calibrated against the
+// double-double mode at n = 36 and 40 and the exact oracle for n <= 26.
+inline double error_estimate(double abs_sum_scaled, int n, int mode) {
+  const double u = mode == 0 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -104);
+  const double terms = std::ldexp(1.0, n - 1);
+  return 3.0 * ((double)n + std::sqrt((double)n)) * u * abs_sum_scaled / std::sqrt(terms);
 }
 
 }  // namespace perm

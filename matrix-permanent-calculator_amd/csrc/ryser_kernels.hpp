@@ -21,7 +21,7 @@ struct DenseLaunch {
   int n;
   int walk_bits;        // subsets walked per lane = 2^walk_bits
   int high_bits;        // walk_bits + high_bits = n - 1
-  int accurate;         // 1: double-double products and lane sums
+  int mode;             // 0 FP64, 1 double-double products, 2 double-double X too
 };
 
 // Enqueue the dense kernel and the grid reduction on `stream`.

@@ -38,7 +38,7 @@ class PermResultC(ctypes.Structure):
                 ("sum_lo", ctypes.c_double), ("abs_sum", ctypes.c_double),
                 ("err_bound", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
-                ("mode", ctypes.c_int), ("integral", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double)]
 
 
 @dataclass
@@ -53,11 +53,12 @@ class PermResult:
     n: int
     mode: int
     integral: int
+    err_est: float
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
-                   r.n, r.mode, r.integral)
+                   r.n, r.mode, r.integral, r.err_est)
 
 
 _LIB = None
@@ -83,6 +84,7 @@ def lib():
         L.perm_plan_sparse.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int,
                                        ctypes.POINTER(vp)]
         L.perm_plan_run.argtypes = [vp, ctypes.POINTER(PermResultC)]
+        L.perm_plan_run_mode.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PermResultC)]
         L.perm_plan_destroy.argtypes = [vp]
         L.perm_plan_destroy.restype = None
         for v in DENSE_VARIANTS:
@@ -207,9 +209,13 @@ class Plan:
             st = lib().perm_plan_dense(dt, a.ctypes.data, a.shape[0], ctypes.byref(self._h))
         _check(st)
 
-    def run(self) -> PermResult:
+    def run(self, mode: int = -1) -> PermResult:
+        """mode -1: automatic; 0 FP64; 1 double-double products; 2 double-double X too."""
         r = PermResultC()
-        _check(lib().perm_plan_run(self._h, ctypes.byref(r)))
+        if mode < 0:
+            _check(lib().perm_plan_run(self._h, ctypes.byref(r)))
+        else:
+            _check(lib().perm_plan_run_mode(self._h, mode, ctypes.byref(r)))
         return PermResult.from_c(r)
 
     def close(self):

@@ -1,0 +1,122 @@
+"""Generate the golden fixtures under tests/golden/ (run from the repo root:
+`python tests/golden/make_golden.py`).  Outputs are data only: input
+matrices and expected permanents.
+
+* small_exact.json -- synthetic matrices (n <= 24) with their EXACT
+  permanents from the multi-modular oracle, each also checked here against
+  brute force (n <= 8) or a closed form where one exists.
+* c1_exact.json    -- the CPU configuration matrix (n=20, density 0.5,
+  nonzeros U(0,1], seed 1) and its exact permanent.
+* refdata.json     -- a few input files of the reference's own data set
+  (int/, float/, double/, n = 30) copied verbatim as text, with their
+  permanents from the double-double oracle (rounded to the integer for the
+  int file), plus the reference's double-X serial value (perman64
+  restatement) for information.
+
+The reference's data files are inputs only -- the reference stores no
+expected outputs -- so every expected value here comes from the oracle.
+"""
+import json
+import math
+import os
+import sys
+from concurrent.futures import ProcessPoolExecutor
+from fractions import Fraction
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
+import oracle as O  # noqa: E402
+from permanent_amd import synthetic_matrix  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+REFDIR = "/root/reference"
+
+
+def frac_str(f: Fraction) -> str:
+    return f"{f.numerator}/{f.denominator}"
+
+
+def small_cases():
+    cases = []
+    for n in range(1, 19):
+        for dens, seed in ((0.3, 1), (0.6, 2), (1.0, 3)):
+            cases.append((f"fp64_n{n}_d{dens}_s{seed}", synthetic_matrix(n, dens, 1000 * n + seed)))
+    for n in range(2, 25, 2):
+        cases.append((f"bin_n{n}_d0.5", synthetic_matrix(n, 0.5, 5000 + n, binary=True)))
+    for n in range(2, 13):
+        a = synthetic_matrix(n, 1.0, 7000 + n) - 0.5  # mixed signs
+        cases.append((f"signed_n{n}", a))
+    for n in range(2, 17, 2):
+        rng = np.random.default_rng(9000 + n)
+        a = rng.integers(1, 6, size=(n, n)) * (rng.random((n, n)) < 0.5)
+        cases.append((f"int_n{n}", a.astype(np.float64)))
+    return cases
+
+
+def main():
+    if "--refdata-only" in sys.argv:
+        refdata()
+        return
+    out = []
+    for name, a in small_cases():
+        ex = O.exact_permanent(a)
+        n = a.shape[0]
+        if n <= 8:
+            assert O.brute_permanent(a) == ex, name
+        out.append({"name": name, "n": n, "matrix": a.tolist(), "exact": frac_str(ex),
+                    "value": float(ex)})
+        print(name, float(ex), flush=True)
+    # closed forms, checked against the exact oracle as well
+    for n in (6, 10, 14):
+        J = np.ones((n, n))
+        assert O.exact_permanent(J) == math.factorial(n)
+        D = J - np.eye(n)
+        d = round(math.factorial(n) * sum((-1) ** k / math.factorial(k) for k in range(n + 1)))
+        assert O.exact_permanent(D) == d
+    with open(os.path.join(OUT, "small_exact.json"), "w") as fh:
+        json.dump(out, fh)
+
+    c1 = synthetic_matrix(20, 0.5, 1)
+    ex = O.exact_permanent(c1)
+    with open(os.path.join(OUT, "c1_exact.json"), "w") as fh:
+        json.dump({"name": "c1_n20_d0.5_seed1", "n": 20, "matrix": c1.tolist(),
+                   "exact": frac_str(ex), "value": float(ex)}, fh)
+    print("c1", float(ex), flush=True)
+    refdata()
+
+
+def refdata():
+    if os.path.isdir(REFDIR):
+        files = ["int/30_0.30_0", "float/30_0.50_0", "double/30_0.50_0", "double/30_0.20_1"]
+        ref = []
+        with ProcessPoolExecutor(max_workers=4) as ex_pool:
+            futs = {f: ex_pool.submit(_ref_case, f) for f in files}
+            for f in files:
+                rec = futs[f].result()
+                ref.append(rec)
+                print(f, rec["value"], flush=True)
+        with open(os.path.join(OUT, "refdata.json"), "w") as fh:
+            json.dump(ref, fh)
+
+
+def _ref_case(rel):
+    path = os.path.join(REFDIR, rel)
+    text = open(path).read()
+    mat, n, nnz, typ = O.read_matrix(path)
+    rec = {"file": rel, "type": typ, "n": n, "text": text}
+    hi, lo = O.ryser_dd(mat.astype(np.float64), threads=2)
+    rec["dd_hi"], rec["dd_lo"] = hi, lo
+    rec["value"] = hi + lo
+    if typ == "int":
+        # integer matrix: X is exact in double-double, products carry ~1e-31
+        # relative error, so the rounded double-double sum is the integer
+        rec["rounded"] = str(round(Fraction(hi) + Fraction(lo)))
+    rec["perman64"] = O.perman64(mat)
+    return rec
+
+
+if __name__ == "__main__":
+    main()

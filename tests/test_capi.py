@@ -1,0 +1,165 @@
+"""C ABI and CLI checks that need no GPU: the library loads and exports
+every symbol include/perman_hip.h declares, the CPU path (`perman -c`,
+parallel_perman64 contract) is exact to 1e-12 on small matrices, the CLI
+keeps the reference's flags and result line, and the GPU entry points fail
+loudly (NaN / PERM_ERR_NODEV) rather than falling back to the CPU when no
+device is present."""
+import ctypes
+import json
+import math
+import os
+import re
+import subprocess
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "perman_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(perm_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(P.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_dropin_names_cover_every_reference_id():
+    names = set(declared_functions())
+    for v in P.DENSE_VARIANTS:
+        for s in ("i32", "f32", "f64"):
+            assert f"perm_{v}_dense_{s}" in names
+    for v in P.SPARSE_VARIANTS[1:]:
+        for s in ("i32", "f32", "f64"):
+            assert f"perm_{v}_sparse_{s}" in names
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 10, 14])
+def test_cpu_path_matches_exact(n):
+    a = P.synthetic_matrix(n, 0.6, seed=40 + n)
+    ex = float(O.exact_permanent(a))
+    assert P.cpu_dense(a, 4) == pytest.approx(ex, rel=1e-12, abs=1e-300)
+    assert P.cpu_dense(a.astype(np.float32), 2) == pytest.approx(
+        float(O.exact_permanent(a.astype(np.float32))), rel=1e-12, abs=1e-300)
+    assert P.cpu_sparse(a, 3) == pytest.approx(ex, rel=1e-12, abs=1e-300)
+
+
+def test_cpu_path_c1_configuration():
+    """The CPU configuration (n=20, density 0.5, U(0,1], seed 1) through
+    the product CPU path, against the exact permanent."""
+    c = json.load(open(os.path.join(GOLDEN, "c1_exact.json")))
+    a = np.array(c["matrix"])
+    ex = Fraction(c["exact"])
+    v = P.cpu_dense(a, 8)
+    assert abs(Fraction(v) - ex) <= Fraction(1, 10 ** 10) * abs(ex)
+
+
+def test_cpu_path_integer_is_exact():
+    a = (np.random.default_rng(3).random((16, 16)) < 0.5).astype(np.int32)
+    ex = O.exact_permanent(a)
+    assert P.cpu_dense(a, 4) == float(ex)
+
+
+def test_gpu_entry_points_fail_loudly_without_device():
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU may be present")
+    a = np.eye(4)
+    v = P.dropin_dense(a, 4)
+    assert math.isnan(v)
+    assert P.lib().perm_last_status() == 4
+    with pytest.raises(P.PermError):
+        P.dense(a)
+
+
+def test_size_and_argument_errors():
+    L = P.lib()
+    r = P.PermResultC()
+    a = np.zeros((70, 70))
+    assert L.perm_dense(2, a.ctypes.data, 70, 4, ctypes.byref(r)) == 2  # PERM_ERR_SIZE
+    assert L.perm_dense(2, a.ctypes.data, 4, 9, ctypes.byref(r)) == 1  # unknown ID
+    assert L.perm_dense(2, None, 4, 4, ctypes.byref(r)) == 1
+    assert L.perm_max_n() >= 40
+
+
+# --- CLI ----------------------------------------------------------------------
+def write_matrix(path, a, typ="double"):
+    n = a.shape[0]
+    nz = np.argwhere(a != 0)
+    with open(path, "w") as fh:
+        fh.write(f"{n} {len(nz)} {typ}\n")
+        for i, j in nz:
+            v = a[i, j]
+            fh.write(f"{i} {j} {int(v) if typ == 'int' else repr(float(v))}\n")
+
+
+def run_cli(*args):
+    return subprocess.run([P.CLI_PATH, *args], capture_output=True, text=True, timeout=120)
+
+
+def test_cli_cpu_result_line(tmp_path):
+    a = P.synthetic_matrix(12, 0.5, seed=5)
+    f = tmp_path / "m12"
+    write_matrix(f, a)
+    r = run_cli("-f", str(f), "-c", "-t", "4")
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"^Result: parallel_perman64 (\S+) in (\S+)$", r.stdout, re.M)
+    assert m, r.stdout
+    assert float(m.group(1)) == pytest.approx(float(O.exact_permanent(a)), rel=1e-12)
+
+
+def test_cli_binary_and_int_types(tmp_path):
+    a = (np.random.default_rng(1).random((10, 10)) < 0.5).astype(float) * 3
+    f = tmp_path / "m10"
+    write_matrix(f, a, "int")
+    r = run_cli("-f", str(f), "-c", "-b")  # -b: every listed entry becomes 1
+    assert r.returncode == 0, r.stderr
+    v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+    assert v == float(O.exact_permanent((a != 0).astype(float)))
+    r = run_cli("-f", str(f), "-c", "-s", "-p", "1")
+    assert "Result: parallel_perman64_sparse" in r.stdout
+    v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+    assert v == float(O.exact_permanent(a))
+
+
+def test_cli_preprocessing_orders_keep_the_permanent(tmp_path):
+    a = P.synthetic_matrix(11, 0.4, seed=9, full_support=True)
+    f = tmp_path / "m11"
+    write_matrix(f, a)
+    ex = float(O.exact_permanent(a))
+    for r_opt in ("1", "2"):
+        r = run_cli("-f", str(f), "-c", "-r", r_opt)
+        v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+        assert v == pytest.approx(ex, rel=1e-12)
+
+
+def test_cli_flag_errors(tmp_path):
+    assert run_cli().returncode == 1  # -f is required
+    assert "required" in run_cli().stderr
+    f = tmp_path / "m"
+    write_matrix(f, np.eye(3))
+    assert run_cli("-f", str(f), "-a").returncode == 2   # approximations not in this build
+    assert run_cli("-f", str(f), "-c", "-s", "-p", "2").returncode == 2
+    r = run_cli("-f", str(f), "-t", "-3")
+    assert r.returncode == 1 and "requires an argument" in r.stderr
+
+
+def test_cli_gpu_without_device_fails_loudly(tmp_path):
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU may be present")
+    f = tmp_path / "m"
+    write_matrix(f, np.eye(3))
+    r = run_cli("-f", str(f), "-g", "-p", "4")
+    assert r.returncode == 1 and "no HIP device" in r.stderr

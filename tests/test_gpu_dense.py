@@ -35,14 +35,24 @@ def test_dense_all_ids_same_value(algo):
     assert rel(v, ex) < 1e-10
 
 
-@pytest.mark.parametrize("n", [8, 12, 16, 20, 24])
-def test_binary_rounds_to_exact_integer(n):
-    a = P.synthetic_matrix(n, 0.5, seed=300 + n, binary=True)
+@pytest.mark.parametrize("n,density", [(8, 0.5), (12, 0.5), (16, 0.5), (20, 0.4), (24, 0.3)])
+def test_binary_rounds_to_exact_integer(n, density):
+    """0/1 matrices whose permanent is below 2^53 come out as the exact
+    integer (automatic double-double product pass when FP64 cannot
+    guarantee it)."""
+    a = P.synthetic_matrix(n, density, seed=300 + n, binary=True)
     ex = O.exact_permanent(a)
-    assert ex.denominator == 1
+    assert ex.denominator == 1 and ex.numerator < 2 ** 53
     r = P.dense(a.astype(np.int32), 1)
-    assert round(r.value) == ex.numerator
-    assert abs(r.value - ex.numerator) < 0.5
+    assert r.value == float(ex.numerator), (r, ex)
+
+
+def test_binary_above_2_53_relative():
+    a = P.synthetic_matrix(24, 0.5, seed=324, binary=True)
+    ex = O.exact_permanent(a)
+    assert ex > 2 ** 53
+    r = P.dense(a.astype(np.int32), 1)
+    assert abs(r.value - float(ex)) <= max(1e-12 * float(ex), r.err_bound)
 
 
 @pytest.mark.parametrize("n", [5, 9, 12])

@@ -1,0 +1,136 @@
+"""Full-size checks at the benchmark configurations, where no CPU oracle
+finishes in seconds.  They rest on known answers and on properties that do
+not depend on size:
+
+* domino tilings of the reference's grid graphs (8x9 -> n=36, 8x10 -> n=40),
+  exact integers through the 0/1 path;
+* the FP64 kernel against the same sum with X and products in
+  double-double (mode 2: no X drift, ~1e-30 relative per operation);
+* exact doubling when one row is multiplied by 2 (every rounding scales);
+* invariance under row/column permutation and transposition (different
+  rounding, same permanent) within the stated error;
+* the sparse path against the dense path (same column store: same bits).
+Tolerance for FP64 matrices: 1e-10 relative (the target), and never more
+than the stated error bound."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    """n=40, FP64, density 0.5, U(0,1], seed 2 (the benchmark matrix)."""
+    return P.synthetic_matrix(40, 0.5, seed=2)
+
+
+def test_c2_fp64_vs_double_double(c2):
+    plan = P.Plan(c2)
+    fast = plan.run(0)
+    ref = plan.run(2)
+    plan.close()
+    print(f"n=40 fast={fast.value!r} dd={ref.value!r} rel={rel(fast.value, ref.value):.3e} "
+          f"stated={fast.err_bound / abs(ref.value):.3e} cond={fast.abs_sum / abs(ref.value):.3e}")
+    assert rel(fast.value, ref.value) < 1e-10
+    assert abs(fast.value - ref.value) <= fast.err_bound
+
+
+def test_grid_8x10_n40_exact_tilings():
+    G = O.grid_graph_matrix(8, 10).astype(np.int32)
+    assert G.shape == (40, 40)
+    r = P.dense(G, 4)
+    assert r.value == 1031151241.0, r
+
+
+def test_grid_8x9_n36_sparse_and_dense_exact_tilings():
+    G = O.grid_graph_matrix(8, 9).astype(np.int32)
+    assert G.shape == (36, 36)
+    want = float(O.domino_tilings(8, 9))
+    assert P.sparse(G, algo_id=4).value == want
+    assert P.dense(G, 1).value == want
+
+
+@pytest.fixture(scope="module")
+def c3():
+    """n=36, density 0.2, every row/column nonzero, U(0,1], seed 4."""
+    return P.synthetic_matrix(36, 0.2, seed=4, full_support=True)
+
+
+def test_c3_sparse_fp64(c3):
+    s = P.sparse(c3, algo_id=4)
+    d = P.dense(c3, 4)
+    assert s.value == d.value  # nonnegative: identical column store
+    plan = P.Plan(c3, sparse_path=True)
+    ref = plan.run(2)
+    plan.close()
+    print(f"n=36 sparse={s.value!r} dd={ref.value!r} rel={rel(s.value, ref.value):.3e}")
+    assert rel(s.value, ref.value) < 1e-10
+    assert abs(s.value - ref.value) <= s.err_bound
+
+
+def test_c3_sparse_binary_exact(c3):
+    b = (c3 != 0).astype(np.int32)  # the -b variant: all ones
+    s = P.sparse(b, algo_id=4)
+    plan = P.Plan(b)
+    d = plan.run(1)  # double-double products: exact for an integer matrix
+    plan.close()
+    assert s.integral == 1 and d.value == round(d.value)
+    assert abs(s.value - d.value) <= s.err_bound
+    if abs(d.value) < 2 ** 53:
+        assert s.value == d.value
+
+
+def test_row_doubling_is_exact():
+    a = P.synthetic_matrix(36, 0.5, seed=21)
+    v = P.dense(a).value
+    b = a.copy()
+    b[7] *= 2.0
+    assert P.dense(b).value == 2.0 * v
+
+
+def test_permutation_and_transpose_invariance():
+    a = P.synthetic_matrix(34, 0.5, seed=22)
+    r0 = P.dense(a)
+    g = np.random.default_rng(1)
+    p = a[g.permutation(34)][:, g.permutation(34)]
+    for m in (p, a.T):
+        r = P.dense(np.ascontiguousarray(m))
+        assert abs(r.value - r0.value) <= r0.err_bound + r.err_bound
+        assert rel(r.value, r0.value) < 1e-10
+
+
+def test_c1_configuration_on_gpu():
+    c = json.load(open(os.path.join(GOLDEN, "c1_exact.json")))
+    a = np.array(c["matrix"])
+    r = P.dense(a, 4)
+    assert rel(r.value, c["value"]) < 1e-12
+
+
+def test_reference_data_files_n30():
+    path = os.path.join(GOLDEN, "refdata.json")
+    if not os.path.exists(path):
+        pytest.skip("refdata fixture not generated")
+    for rec in json.load(open(path)):
+        tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), rec["file"].replace("/", "_"))
+        with open(tmp, "w") as fh:
+            fh.write(rec["text"])
+        mat, n, _, typ = O.read_matrix(tmp)
+        r = P.dense(mat, 4)
+        want = rec["dd_hi"] + rec["dd_lo"]
+        print(f"{rec['file']}: gpu={r.value!r} dd={want!r} rel={rel(r.value, want):.2e} "
+              f"perman64(ref double-X)={rec['perman64']!r}")
+        if typ == "int":
+            want = float(int(rec["rounded"]))
+        assert rel(r.value, want) < 1e-10
+        assert abs(r.value - want) <= r.err_bound
