@@ -1,0 +1,121 @@
+// perman.hpp -- C++ drop-in for the reference's exact-permanent entry points.
+//
+// Re-creates, at global scope, the template names and signatures the
+// reference's main.cu calls (RunAlgo, main.cu:20-248), forwarding to the C
+// ABI of libperman_hip.so (perman_hip.h):
+//   gpu_exact_dense.cu:402,467,525,583,641   gpu_perman64_<variant>(T*, int, int, int)
+//   gpu_exact_sparse.cu:674,734,794,854      gpu_perman64_<variant>_sparse(...)
+//   algo.h:663, algo.h:569                   parallel_perman64 / _sparse
+// for T in {int, float, double}.  A caller that included the reference's .cu
+// files replaces those includes with this header and links
+// -lperman_hip.  Results are returned by value as double (NaN on error; see
+// perm_last_error()).
+#ifndef PERMAN_HPP
+#define PERMAN_HPP
+
+#include "perman_hip.h"
+
+namespace perman_detail {
+// Overload sets per element type; the templates below dispatch on T.
+#define PERMAN_DENSE_OVERLOADS(NAME)                                                             \
+  inline double NAME(int* m, int n, int g, int b) { return perm_##NAME##_dense_i32(m, n, g, b); } \
+  inline double NAME(float* m, int n, int g, int b) {                                            \
+    return perm_##NAME##_dense_f32(m, n, g, b);                                                  \
+  }                                                                                              \
+  inline double NAME(double* m, int n, int g, int b) {                                           \
+    return perm_##NAME##_dense_f64(m, n, g, b);                                                  \
+  }
+PERMAN_DENSE_OVERLOADS(xglobal)
+PERMAN_DENSE_OVERLOADS(xlocal)
+PERMAN_DENSE_OVERLOADS(xshared)
+PERMAN_DENSE_OVERLOADS(xshared_coalescing)
+PERMAN_DENSE_OVERLOADS(xshared_coalescing_mshared)
+#undef PERMAN_DENSE_OVERLOADS
+
+#define PERMAN_SPARSE_OVERLOADS(NAME)                                                       \
+  inline double NAME##_sparse(int* m, int* cp, int* r, int* cv, int n, int g, int b) {      \
+    return perm_##NAME##_sparse_i32(m, cp, r, cv, n, g, b);                                 \
+  }                                                                                         \
+  inline double NAME##_sparse(float* m, int* cp, int* r, float* cv, int n, int g, int b) {  \
+    return perm_##NAME##_sparse_f32(m, cp, r, cv, n, g, b);                                 \
+  }                                                                                         \
+  inline double NAME##_sparse(double* m, int* cp, int* r, double* cv, int n, int g, int b) { \
+    return perm_##NAME##_sparse_f64(m, cp, r, cv, n, g, b);                                 \
+  }
+PERMAN_SPARSE_OVERLOADS(xlocal)
+PERMAN_SPARSE_OVERLOADS(xshared)
+PERMAN_SPARSE_OVERLOADS(xshared_coalescing)
+PERMAN_SPARSE_OVERLOADS(xshared_coalescing_mshared)
+#undef PERMAN_SPARSE_OVERLOADS
+
+inline double cpu(int* m, int n, int t) { return perm_parallel_perman64_i32(m, n, t); }
+inline double cpu(float* m, int n, int t) { return perm_parallel_perman64_f32(m, n, t); }
+inline double cpu(double* m, int n, int t) { return perm_parallel_perman64_f64(m, n, t); }
+inline double cpu_sparse(int* m, int* cp, int* r, int* cv, int n, int t) {
+  return perm_parallel_perman64_sparse_i32(m, cp, r, cv, n, t);
+}
+inline double cpu_sparse(float* m, int* cp, int* r, float* cv, int n, int t) {
+  return perm_parallel_perman64_sparse_f32(m, cp, r, cv, n, t);
+}
+inline double cpu_sparse(double* m, int* cp, int* r, double* cv, int n, int t) {
+  return perm_parallel_perman64_sparse_f64(m, cp, r, cv, n, t);
+}
+}  // namespace perman_detail
+
+// ---- dense, algorithm IDs 0-4 -------------------------------------------
+template <class T>
+double gpu_perman64_xglobal(T* mat, int nov, int grid_dim, int block_dim) {
+  return perman_detail::xglobal(mat, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xlocal(T* mat, int nov, int grid_dim, int block_dim) {
+  return perman_detail::xlocal(mat, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared(T* mat, int nov, int grid_dim, int block_dim) {
+  return perman_detail::xshared(mat, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing(T* mat, int nov, int grid_dim, int block_dim) {
+  return perman_detail::xshared_coalescing(mat, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared(T* mat, int nov, int grid_dim, int block_dim) {
+  return perman_detail::xshared_coalescing_mshared(mat, nov, grid_dim, block_dim);
+}
+
+// ---- sparse (-s), algorithm IDs 1-4 ---------------------------------------
+template <class T>
+double gpu_perman64_xlocal_sparse(T* mat, int* cptrs, int* rows, T* cvals, int nov, int grid_dim,
+                                  int block_dim) {
+  return perman_detail::xlocal_sparse(mat, cptrs, rows, cvals, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_sparse(T* mat, int* cptrs, int* rows, T* cvals, int nov, int grid_dim,
+                                   int block_dim) {
+  return perman_detail::xshared_sparse(mat, cptrs, rows, cvals, nov, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_sparse(T* mat, int* cptrs, int* rows, T* cvals, int nov,
+                                              int grid_dim, int block_dim) {
+  return perman_detail::xshared_coalescing_sparse(mat, cptrs, rows, cvals, nov, grid_dim,
+                                                  block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_sparse(T* mat, int* cptrs, int* rows, T* cvals,
+                                                      int nov, int grid_dim, int block_dim) {
+  return perman_detail::xshared_coalescing_mshared_sparse(mat, cptrs, rows, cvals, nov, grid_dim,
+                                                          block_dim);
+}
+
+// ---- CPU (`-c`) ------------------------------------------------------------
+template <class T>
+double parallel_perman64(T* mat, int nov, int threads) {
+  return perman_detail::cpu(mat, nov, threads);
+}
+template <class T>
+double parallel_perman64_sparse(T* mat, int* cptrs, int* rows, T* cvals, int nov, int threads) {
+  return perman_detail::cpu_sparse(mat, cptrs, rows, cvals, nov, threads);
+}
+
+#endif  // PERMAN_HPP

@@ -57,11 +57,12 @@ __device__ __forceinline__ void load_col(double (&cv)[N], ColPtr base, int blk) 
 enum Mode : int { kFast = 0, kDDProd = 1, kDDAll = 2 };
 
 #ifndef PERM_FAST_WAVES_PER_EU
-#define PERM_FAST_WAVES_PER_EU 4
+#define PERM_FAST_WAVES_PER_EU 1
 #endif
 // Minimum waves per SIMD the fast kernel is compiled for (caps its VGPRs at
-// 512/waves): enough waves to cover the scalar-load latency of the column
-// prefetch.
+// 512/waves).  1 = no cap: unconstrained, n <= 40 already fits in 128 VGPRs
+// (4 waves/SIMD) without spills; measured, 3 and 4 waves run at the same
+// VALU issue rate.
 constexpr int kFastWavesPerEU = PERM_FAST_WAVES_PER_EU;
 
 // Per-lane accumulator.  kFast: plain double `p` folded into double-double

@@ -163,3 +163,33 @@ def test_cli_gpu_without_device_fails_loudly(tmp_path):
     write_matrix(f, np.eye(3))
     r = run_cli("-f", str(f), "-g", "-p", "4")
     assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def test_cpp_dropin_header_compiles_and_forwards(tmp_path):
+    """include/perman.hpp recreates the reference's template names; a
+    main.cu-style caller compiles with g++ and gets the CPU results."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(r"""
+#include "perman.hpp"
+#include <cstdio>
+int main() {
+  double m[9] = {1, 2, 0, 0, 3, 4, 5, 0, 6};
+  int cp[4] = {0, 2, 4, 6}, rows[6] = {0, 2, 0, 1, 1, 2};
+  double cv[6] = {1, 5, 2, 3, 4, 6};
+  int mi[9] = {1, 2, 0, 0, 3, 4, 5, 0, 6};
+  double (*g)(double*, int, int, int) = &gpu_perman64_xshared_coalescing_mshared<double>;
+  double (*s)(float*, int*, int*, float*, int, int, int) = &gpu_perman64_xlocal_sparse<float>;
+  std::printf("%.17g %.17g %.17g %d\n", parallel_perman64(m, 3, 2), parallel_perman64(mi, 3, 1),
+              parallel_perman64_sparse(m, cp, rows, cv, 3, 1), g != nullptr && s != nullptr);
+  return 0;
+}
+""")
+    exe = tmp_path / "caller"
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(P.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{inc}", str(src), f"-L{libdir}",
+                    "-lperman_hip", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    m = np.array([[1, 2, 0], [0, 3, 4], [5, 0, 6]], dtype=float)
+    want = float(O.exact_permanent(m))
+    assert [float(v) for v in out[:3]] == [want, want, want] and out[3] == "1"
