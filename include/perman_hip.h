@@ -58,6 +58,7 @@ typedef struct perm_result {
   int mode;
   int integral;         /* 1 if every entry is an integer (X is then exact) */
   double err_est;       /* statistical error estimate (independent roundings) */
+  int nu;               /* FP64 kernel: rows touched by Gray column 0 (pair variant), or n */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */

@@ -10,6 +10,7 @@
 // checked.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -71,6 +72,7 @@ struct perm_plan {
   DeviceBuffers mem;
   perm::DenseLaunch L{};
   int unroll_bits = 0;
+  int nu = 0;             // active rows of Gray column 0 for the FP64 kernel (n: none)
   bool integral = false;  // every entry an integer of magnitude < 2^31
   bool borrowed = false;  // stream/events/memory belong to the device cache
 };
@@ -178,11 +180,13 @@ int upload_plan(perm_plan* P, const std::vector<double>& colstore, const std::ve
 }
 
 // Create an owning plan (its own stream, events and device memory).
-int make_plan(int n, const std::vector<double>& colstore, const std::vector<double>& x0,
-              perm_plan** out) {
+int make_plan(int n, const std::vector<double>& colstore, const std::vector<double>& x0, int nu,
+              bool integral, perm_plan** out) {
   if (int s = select_device()) return s;
   auto* P = new perm_plan();
   P->device = t_device;
+  P->nu = nu;
+  P->integral = integral;
   P->mem.bytes = layout_plan(P, n, colstore.size());
   auto cleanup = [&](int code, const std::string& msg) {
     perm_plan_destroy(P);
@@ -205,31 +209,82 @@ int make_plan(int n, const std::vector<double>& colstore, const std::vector<doub
   return ok();
 }
 
-// Host preparation of a dense matrix: start vector and signed column store.
+// Host preparation.  Produces the start vector x0 and the Gray columns in
+// column-major order (cols[k*n + j] = a[j][k] for the walked columns
+// k < n-1), then chooses the walk order (pair_order) and builds the signed
+// column store.
+struct Prepared {
+  std::vector<double> x0, cs;
+  int nu = 0;
+  bool integral = false;
+};
+
+// Zero-skipping order for the pair kernels: the walked column with the
+// fewest nonzeros goes to Gray position 0 and the rows it touches move to
+// the front, padded to a multiple of 4 (nu).  Row and column permutations
+// leave the permanent unchanged; PERMAN_PAIR=0 keeps the natural order.
+void pair_order(int n, std::vector<double>& x0, std::vector<double>& cols, int& nu) {
+  nu = n;
+  if (n < perm::kMinPairN || env_int("PERMAN_PAIR", 1) == 0) return;
+  const int B = n - 1;
+  int best = 0, best_nnz = n + 1;
+  for (int k = 0; k < B; ++k) {
+    int c = 0;
+    for (int j = 0; j < n; ++j) c += cols[(size_t)k * n + j] != 0.0;
+    if (c < best_nnz) {
+      best_nnz = c;
+      best = k;
+    }
+  }
+  const int padded = std::max(4, (best_nnz + 3) / 4 * 4);
+  if (padded > n - 4) return;
+  std::vector<int> rp, cp;
+  for (int j = 0; j < n; ++j)
+    if (cols[(size_t)best * n + j] != 0.0) rp.push_back(j);
+  for (int j = 0; j < n; ++j)
+    if (cols[(size_t)best * n + j] == 0.0) rp.push_back(j);
+  cp.push_back(best);
+  for (int k = 0; k < B; ++k)
+    if (k != best) cp.push_back(k);
+  std::vector<double> x2(n), c2(cols.size());
+  for (int j = 0; j < n; ++j) x2[j] = x0[rp[j]];
+  for (int k = 0; k < B; ++k)
+    for (int j = 0; j < n; ++j) c2[(size_t)k * n + j] = cols[(size_t)cp[k] * n + rp[j]];
+  x0.swap(x2);
+  cols.swap(c2);
+  nu = padded;
+}
+
 bool integral_matrix(const std::vector<double>& a) {
   for (double v : a)
     if (!(std::fabs(v) < 2147483648.0) || v != std::nearbyint(v)) return false;
   return true;
 }
 
-int prepare_dense(int dtype, const void* mat, int n, std::vector<double>& cs,
-                  std::vector<double>& x0, bool& integral) {
+void finish_prepare(int n, std::vector<double>& cols, Prepared& p) {
+  pair_order(n, p.x0, cols, p.nu);
+  perm::build_column_store(n, [&](int k, int j) { return cols[(size_t)k * n + j]; }, p.cs);
+}
+
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p) {
   if (!mat) return fail(PERM_ERR_ARG, "null pointer");
   if (int s = check_n(n)) return s;
   std::vector<double> a;
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
-  integral = integral_matrix(a);
-  x0.assign(n, 0.0);
-  perm::nw_start_vector(a.data(), n, x0.data());
-  perm::build_column_store(n, [&](int k, int j) { return a[(size_t)j * n + k]; }, cs);
+  p.integral = integral_matrix(a);
+  p.x0.assign(n, 0.0);
+  perm::nw_start_vector(a.data(), n, p.x0.data());
+  std::vector<double> cols((size_t)(n - 1) * n);
+  for (int k = 0; k < n - 1; ++k)
+    for (int j = 0; j < n; ++j) cols[(size_t)k * n + j] = a[(size_t)j * n + k];
+  finish_prepare(n, cols, p);
   return PERM_OK;
 }
 
 // Sparse: x0 from the dense matrix, column updates from the CCS arrays
 // (reference: gpu_exact_sparse.cu:854-871 and the kernels' CCS walk).
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, std::vector<double>& cs, std::vector<double>& x0,
-                   bool& integral) {
+                   const void* cvals, int n, Prepared& p) {
   if (!mat || !cptrs) return fail(PERM_ERR_ARG, "null pointer");
   if (int s = check_n(n)) return s;
   const int nnz = cptrs[n];
@@ -241,20 +296,22 @@ int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows
     if (rows[e] < 0 || rows[e] >= n) return fail(PERM_ERR_ARG, "row index out of range");
   std::vector<double> a;
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
-  x0.assign(n, 0.0);
-  perm::nw_start_vector(a.data(), n, x0.data());
-  std::vector<double> dense_cols((size_t)n * n, 0.0);
+  p.x0.assign(n, 0.0);
+  perm::nw_start_vector(a.data(), n, p.x0.data());
+  std::vector<double> cols((size_t)n * n, 0.0);
   for (int k = 0; k < n; ++k)
     for (int e = cptrs[k]; e < cptrs[k + 1]; ++e)
-      dense_cols[(size_t)k * n + rows[e]] += elem(dtype, cvals, (size_t)e);
-  integral = integral_matrix(a) && integral_matrix(dense_cols);
-  perm::build_column_store(n, [&](int k, int j) { return dense_cols[(size_t)k * n + j]; }, cs);
+      cols[(size_t)k * n + rows[e]] += elem(dtype, cvals, (size_t)e);
+  p.integral = integral_matrix(a) && integral_matrix(cols);
+  cols.resize((size_t)(n - 1) * n);  // the last column lives in x0
+  finish_prepare(n, cols, p);
   return PERM_OK;
 }
 
 // Launch one pass (FP64 or double-double products) and read the result.
 int run_pass(perm_plan* P, int mode, perm::Partial& r, float& ms) {
   P->L.mode = mode;
+  P->L.nu = mode == 0 ? P->nu : P->n;
   HIP_TRY(hipEventRecord(P->ev0, P->stream));
   HIP_TRY(perm::launch_dense(P->L, P->stream));
   HIP_TRY(hipEventRecord(P->ev1, P->stream));
@@ -278,6 +335,7 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->err_est = perm::error_estimate(out->abs_sum, n, mode);
   out->status = PERM_OK;
   out->mode = mode;
+  out->nu = mode == 0 ? P->nu : P->n;
   out->integral = P->integral ? 1 : 0;
 }
 
@@ -306,8 +364,9 @@ int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
 }
 
 // One-shot run through the per-device cache.
-int run_oneshot(int n, const std::vector<double>& cs, const std::vector<double>& x0,
-                bool integral, perm_result* out) {
+int run_oneshot(int n, const Prepared& prep, perm_result* out) {
+  const std::vector<double>& cs = prep.cs;
+  const std::vector<double>& x0 = prep.x0;
   if (int s = select_device()) return s;
   DeviceCache& C = device_cache(t_device);
   std::lock_guard<std::mutex> guard(C.mu);
@@ -322,7 +381,8 @@ int run_oneshot(int n, const std::vector<double>& cs, const std::vector<double>&
   P.stream = C.stream;
   P.ev0 = C.ev0;
   P.ev1 = C.ev1;
-  P.integral = integral;
+  P.integral = prep.integral;
+  P.nu = prep.nu;
   const size_t need = layout_plan(&P, n, cs.size());
   if (need > C.bytes) {
     if (C.base) HIP_TRY(hipFree(C.base));
@@ -366,22 +426,16 @@ int perm_max_n(void) { return perm::kMaxDenseN; }
 
 int perm_plan_dense(int dtype, const void* mat, int nov, perm_plan** plan) {
   if (!plan) return fail(PERM_ERR_ARG, "null plan");
-  std::vector<double> cs, x0;
-  bool integral = false;
-  if (int s = prepare_dense(dtype, mat, nov, cs, x0, integral)) return s;
-  if (int s = make_plan(nov, cs, x0, plan)) return s;
-  (*plan)->integral = integral;
-  return PERM_OK;
+  Prepared p;
+  if (int s = prepare_dense(dtype, mat, nov, p)) return s;
+  return make_plan(nov, p.cs, p.x0, p.nu, p.integral, plan);
 }
 int perm_plan_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                      const void* cvals, int nov, perm_plan** plan) {
   if (!plan) return fail(PERM_ERR_ARG, "null plan");
-  std::vector<double> cs, x0;
-  bool integral = false;
-  if (int s = prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, cs, x0, integral)) return s;
-  if (int s = make_plan(nov, cs, x0, plan)) return s;
-  (*plan)->integral = integral;
-  return PERM_OK;
+  Prepared p;
+  if (int s = prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  return make_plan(nov, p.cs, p.x0, p.nu, p.integral, plan);
 }
 int perm_plan_run(perm_plan* plan, perm_result* out) { return plan_run_impl(plan, -1, out); }
 int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out) {
@@ -401,20 +455,18 @@ void perm_plan_destroy(perm_plan* P) {
 int perm_dense(int dtype, const void* mat, int nov, int algo_id, perm_result* out) {
   if (!out) return fail(PERM_ERR_ARG, "null result");
   if (algo_id < 0 || algo_id > 4) return fail(PERM_ERR_ARG, "unknown dense algorithm ID");
-  std::vector<double> cs, x0;
-  bool integral = false;
-  if (int s = prepare_dense(dtype, mat, nov, cs, x0, integral)) return s;
-  return run_oneshot(nov, cs, x0, integral, out);
+  Prepared p;
+  if (int s = prepare_dense(dtype, mat, nov, p)) return s;
+  return run_oneshot(nov, p, out);
 }
 
 int perm_sparse(int dtype, const void* mat, const int* cptrs, const int* rows, const void* cvals,
                 int nov, int algo_id, perm_result* out) {
   if (!out) return fail(PERM_ERR_ARG, "null result");
   if (algo_id < 1 || algo_id > 4) return fail(PERM_ERR_ARG, "unknown sparse algorithm ID");
-  std::vector<double> cs, x0;
-  bool integral = false;
-  if (int s = prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, cs, x0, integral)) return s;
-  return run_oneshot(nov, cs, x0, integral, out);
+  Prepared p;
+  if (int s = prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  return run_oneshot(nov, p, out);
 }
 
 #define PERM_DENSE_DROPIN(NAME, SFX, T, DT)                                   \

@@ -1,0 +1,341 @@
+// ryser_dense_impl.hpp -- the dense Ryser kernel family for gfx950
+// (Nijenhuis-Wilf form of Ryser's formula, Gray-code subset order).
+//
+// Reference semantics: gpu_exact_dense.cu:129-399 (kernels) and :466-699
+// (wrappers) of berkyaglioglu/matrix-permanent-calculator.  The sum is the
+// same -- over all subsets S of the first n-1 columns of
+// (-1)^|S| prod_j x_j(S), x(S) = x0 + sum_{k in S} a_k, scaled by
+// (4(n&1)-2) -- walked in a way designed for CDNA4:
+//
+//  * Every lane owns an ALIGNED block of 2^w subsets: its "high" columns
+//    [w, n-1) are fixed by the lane id (x built in closed form) and it walks
+//    the low columns [0, w) in standard Gray order.  All 64 lanes of a wave
+//    flip the SAME column with the SAME sign at every step: the column index
+//    is wave-uniform.
+//  * Because it is uniform, the column is fetched with scalar loads
+//    (s_load_dwordx16 through the scalar cache) into SGPRs and feeds
+//    v_add_f64 as the SGPR operand; each lane keeps x in VGPRs.  The store
+//    holds +a_k and -a_k, so a signed update is one add.
+//  * One column is prefetched per step while the previous step's product
+//    runs (sched_barrier separates the phases).
+//  * The innermost 2^R steps are unrolled with compile-time column indices.
+//  * Per step: n adds, n-2 multiplies, one FMA that multiplies in the last x
+//    and accumulates the signed term, one FMA for sum |term|: 2n FP64 VALU
+//    instructions per subset.
+//  * Pair variant (NU < N): Gray column 0 flips on every odd step.  The host
+//    places the sparsest column there and moves the rows it touches to the
+//    front (rows [0, NU)).  The two subsets of each pair then differ only in
+//    x[0..NU), so the product of the passive rows x[NU..N) is formed once per
+//    pair and the odd step adds and multiplies only NU rows:
+//    n + NU + 1/2 instructions per subset instead of 2n.  Adds of zero are
+//    exact, so the skipped work changes no bits of any x.
+//  * Lane sums are folded into double-double every 2^(R+5) steps; lanes,
+//    waves and workgroups are reduced in double-double in a fixed order.
+#pragma once
+#include <utility>
+
+#include "ryser_device.hpp"
+#include "ryser_kernels.hpp"
+
+namespace perm {
+
+// Pointer into the column store in the constant address space: loads through
+// it are known not to alias any store of the kernel, so uniform loads are
+// selected as scalar (SMEM) loads.
+typedef const double __attribute__((address_space(4)))* ColPtr;
+
+// Kernel modes.
+//  kFast: FP64 X, FP64 products (the production kernel).
+//  kDDProd: FP64 X, double-double products and lane sums -- exact products
+//    for integer matrices (whose X is exact), used to round 0/1 results to
+//    the exact integer.
+//  kDDAll: X itself kept as a double-double (every column addition through
+//    TwoSum), double-double products and sums: no X drift; a verification
+//    mode for large n where no CPU oracle finishes.
+enum Mode : int { kFast = 0, kDDProd = 1, kDDAll = 2 };
+
+#ifndef PERM_FAST_WAVES_PER_EU
+#define PERM_FAST_WAVES_PER_EU 1
+#endif
+// Minimum waves per SIMD the fast kernel is compiled for (caps its VGPRs at
+// 512/waves).  1 = no cap: unconstrained, n <= 40 already fits in 128 VGPRs
+// (4 waves/SIMD) without spills; measured, 3 and 4 waves run at the same
+// VALU issue rate.
+constexpr int kFastWavesPerEU = PERM_FAST_WAVES_PER_EU;
+
+// Load COUNT doubles of column block `blk` (wave-uniform address) into
+// SGPRs.  The pointer is laundered through an empty asm so that the compiler
+// neither merges repeated loads of one column across steps (which would pin
+// N SGPR pairs and spill) nor hoists them out of the loop.
+template <int N, int COUNT>
+__device__ __forceinline__ void load_col(double (&cv)[N], ColPtr base, int blk) {
+  ColPtr c = base + (size_t)blk * N;
+  asm volatile("" : "+s"(c));
+#pragma unroll
+  for (int j = 0; j < COUNT; ++j) cv[j] = c[j];
+}
+
+// Per-lane accumulator.  kFast: plain double `p` folded into double-double
+// every 32 groups.  Other modes: the signed sum is kept as an unevaluated
+// pair (p, plo) updated with TwoSum on every term.
+struct LaneAcc {
+  double p = 0.0, plo = 0.0, a = 0.0;
+};
+
+// Product of the factors {seed if SEED} + x[LO..HI) as four interleaved
+// chains (independent work for the scheduler): F-1 multiplies for F factors.
+template <int N, int LO, int HI, bool SEED>
+__device__ __forceinline__ double chain_prod(const double (&x)[N], double seed) {
+  constexpr int F = HI - LO + (SEED ? 1 : 0);
+  static_assert(F >= 1, "empty product");
+  double c[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    const double v = (SEED && f == 0) ? seed : x[LO + f - (SEED ? 1 : 0)];
+    if (f < 4) c[f] = v;
+    else c[f & 3] *= v;
+  }
+  if constexpr (F == 1) return c[0];
+  else if constexpr (F == 2) return c[0] * c[1];
+  else if constexpr (F == 3) return (c[0] * c[1]) * c[2];
+  else return (c[0] * c[1]) * (c[2] * c[3]);
+}
+
+// Signed accumulation of the term q * last: one FMA for the sum, one for
+// sum |term|.  The asm pins the |term| FMA to this step (left alone, the
+// compiler defers it to the end of the group and keeps 2 registers per step
+// live).
+template <bool NEG>
+__device__ __forceinline__ void add_term(double q, double last, LaneAcc& acc) {
+  acc.p = __builtin_fma(NEG ? -q : q, last, acc.p);
+  acc.a = __builtin_fma(__builtin_fabs(q), __builtin_fabs(last), acc.a);
+  asm volatile("" : "+v"(acc.a));
+}
+
+// p += sign * prod(x);  a += |prod(x)|  (all N factors).
+// kDDProd / kDDAll: the product is a double-double (TwoProd through FMA:
+// hi*x = hi' + e exactly), two interleaved chains joined by a double-double
+// multiply; the term enters (p, plo) through TwoSum.  In kDDAll each factor
+// is the pair (x[j], xl[j]).
+template <int N, bool NEG, int MODE>
+__device__ __forceinline__ void accumulate_term(const double (&x)[N], const double (&xl)[N],
+                                                LaneAcc& acc) {
+  if constexpr (N == 1) {
+    const double v = MODE == kDDAll ? x[0] + xl[0] : x[0];
+    acc.p += NEG ? -v : v;
+    acc.a += __builtin_fabs(v);
+  } else if constexpr (MODE == kFast) {
+    add_term<NEG>(chain_prod<N, 0, N - 1, false>(x, 0.0), x[N - 1], acc);
+  } else {
+    // chain 0 over even j, chain 1 over odd j
+    double h0 = x[0], l0 = MODE == kDDAll ? xl[0] : 0.0;
+    double h1 = x[1], l1 = MODE == kDDAll ? xl[1] : 0.0;
+#pragma unroll
+    for (int j = 2; j < N; ++j) {
+      double& h = (j & 1) ? h1 : h0;
+      double& l = (j & 1) ? l1 : l0;
+      const double ph = h * x[j];
+      double e = __builtin_fma(h, x[j], -ph);
+      if constexpr (MODE == kDDAll) e = __builtin_fma(h, xl[j], e);
+      l = __builtin_fma(l, x[j], e);
+      h = ph;
+    }
+    // (h0, l0) * (h1, l1)
+    double th = h0 * h1;
+    double tl = __builtin_fma(h0, h1, -th);
+    tl = __builtin_fma(h0, l1, __builtin_fma(l0, h1, tl));
+    if (NEG) {
+      th = -th;
+      tl = -tl;
+    }
+    // (p, plo) += (th, tl)
+    const double s = acc.p + th;
+    const double bb = s - acc.p;
+    const double err = (acc.p - (s - bb)) + (th - bb);
+    acc.p = s;
+    acc.plo += err + tl;
+    acc.a += __builtin_fabs(th);
+  }
+}
+
+// x[0..COUNT) += column.  kDDAll: (x, xl) += c through TwoSum; the rounding
+// error of every addition is carried in xl (exact, never renormalised).
+template <int N, int MODE, int COUNT>
+__device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], const double (&cv)[N]) {
+#pragma unroll
+  for (int j = 0; j < COUNT; ++j) {
+    if constexpr (MODE == kDDAll) {
+      const double s = x[j] + cv[j];
+      const double bb = s - x[j];
+      xl[j] += (x[j] - (s - bb)) + (cv[j] - bb);
+      x[j] = s;
+    } else {
+      x[j] += cv[j];
+    }
+  }
+}
+
+// Step v of group g (step index t = g*2^R + v).  Phase A adds the column
+// that the previous step prefetched into SGPRs; phase B issues the scalar
+// loads of the next step's column and, while they are in flight, forms the
+// product and accumulates the term.  sched_barrier keeps the phases apart so
+// the loads always overlap a product.  With the pair variant (NU < N) odd
+// steps touch only x[0..NU) and reuse the passive product `ppass` formed on
+// the even step before them.
+template <int N, int R, int MODE, int NU, int V>
+__device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], double (&cv)[N],
+                                            ColPtr cols, uint32_t g, uint32_t groups,
+                                            int zero_blk, LaneAcc& acc, double& ppass) {
+  constexpr bool kPair = NU < N;
+  if constexpr (V < (1 << R)) {
+    if constexpr (kPair && (V & 1)) add_column<N, MODE, NU>(x, xl, cv);
+    else add_column<N, MODE, N>(x, xl, cv);
+    __builtin_amdgcn_sched_barrier(0);
+    int blk;
+    if constexpr (V + 1 < (1 << R)) {
+      // Next step inside the group: column k = ctz(v+1); the new gray bit k
+      // is set iff bit k+1 of the step index is clear.  For k+1 < R that
+      // bit is inside the group, for k+1 == R it is bit 0 of g.
+      constexpr int k = __builtin_ctz(V + 1);
+      int neg;
+      if constexpr (k + 1 < R) neg = ((V + 1) >> (k + 1)) & 1;
+      else neg = (int)(g & 1u);
+      blk = 2 * k + neg;
+    } else {
+      // Next step opens group g+1: column R + ctz(g+1).  Past the last group
+      // the (unused) prefetch reads the zero column to stay in bounds.
+      const uint32_t gn = g + 1;
+      const int c = __builtin_ctz(gn);
+      const int neg = (int)((gn >> (c + 1)) & 1u);
+      blk = gn < groups ? 2 * (R + c) + neg : zero_blk;
+    }
+    if constexpr (kPair && ((V + 1) & 1) && V + 1 < (1 << R)) load_col<N, NU>(cv, cols, blk);
+    else load_col<N, N>(cv, cols, blk);
+    if constexpr (kPair) {
+      if constexpr ((V & 1) == 0) ppass = chain_prod<N, NU, N, false>(x, 0.0);
+      add_term<(V & 1) != 0>(chain_prod<N, 0, NU - 1, true>(x, ppass), x[NU - 1], acc);
+    } else {
+      accumulate_term<N, (V & 1) != 0, MODE>(x, xl, acc);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    group_steps<N, R, MODE, NU, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+  }
+}
+
+// colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
+// Gray columns k < n-1, block 2(n-1) = zeros.  With NU < N, column 0 is zero
+// in rows [NU, N).
+template <int N, int R, int MODE, int NU>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
+void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
+                        Partial* __restrict__ partials, uint64_t lanes, int n_walk_bits,
+                        int n_high_bits) {
+  static_assert(NU == N || (MODE == kFast && R >= 1 && NU >= 1), "pair variant: FP64 mode only");
+  __shared__ Partial scratch[256 / kWave];
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const ColPtr cols = (ColPtr)colpm;
+  const int zero_blk = 2 * (N - 1);
+
+  double x[N], xl[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    x[j] = x0[j];
+    xl[j] = 0.0;
+  }
+
+  // Closed-form start: add the high columns selected by the lane id
+  // (adding sel*c with sel in {0, 1} is exact selection, no branch).
+  int parity = 0;
+  for (int b = 0; b < n_high_bits; ++b) {
+    ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
+    const int bit = (int)((lane >> b) & 1);
+    parity ^= bit;
+    const double sel = bit ? 1.0 : 0.0;
+    double cv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
+    add_column<N, MODE, N>(x, xl, cv);
+  }
+
+  DD s = {0.0, 0.0};
+  LaneAcc acc;
+  double ppass = 1.0;
+  const uint32_t groups = 1u << (n_walk_bits - R);
+  double cv[N];
+  load_col<N, N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
+  for (uint32_t g = 0; g < groups; ++g) {
+    group_steps<N, R, MODE, NU, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+    if constexpr (MODE == kFast) {
+      if ((g & 31u) == 31u) {  // fold into double-double every 32 groups
+        s = dd_add_d(s, acc.p);
+        acc.p = 0.0;
+      }
+    }
+  }
+  if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
+  else s = two_sum(acc.p, acc.plo);
+  double a = acc.a;
+  if (parity) {
+    s.hi = -s.hi;
+    s.lo = -s.lo;
+  }
+  if (lane >= lanes) {
+    s = {0.0, 0.0};
+    a = 0.0;
+  }
+  block_reduce_store(s, a, scratch, partials + blockIdx.x);
+}
+
+template <int N>
+constexpr int unroll_bits_for() {
+  return N - 1 < 3 ? N - 1 : 3;
+}
+
+template <int N, int MODE, int NU>
+hipError_t launch_variant(const DenseLaunch& L, hipStream_t stream) {
+  constexpr int R = unroll_bits_for<N>();
+  ryser_dense_kernel<N, R, MODE, NU><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
+      L.colpm, L.x0, L.partials, L.lanes, L.walk_bits, L.high_bits);
+  return hipGetLastError();
+}
+
+// Pair variants exist for NU = 4, 8, ..., N-4 when N >= kMinPairN.
+template <int N, int... Q>
+hipError_t launch_pair(const DenseLaunch& L, hipStream_t stream,
+                       std::integer_sequence<int, Q...>) {
+  hipError_t e = hipErrorInvalidValue;
+  ((L.nu == 4 * (Q + 1) ? (e = launch_variant<N, kFast, 4 * (Q + 1)>(L, stream), 0) : 0), ...);
+  return e;
+}
+
+template <int N>
+hipError_t launch_n(const DenseLaunch& L, hipStream_t stream) {
+  if (L.walk_bits < unroll_bits_for<N>()) return hipErrorInvalidValue;
+  if (L.mode == kDDProd) return launch_variant<N, kDDProd, N>(L, stream);
+  if (L.mode == kDDAll) return launch_variant<N, kDDAll, N>(L, stream);
+  if (L.nu == N || L.nu <= 0) return launch_variant<N, kFast, N>(L, stream);
+  if constexpr (N >= kMinPairN) {
+    if (L.nu % 4 == 0 && L.nu < N)
+      return launch_pair<N>(L, stream, std::make_integer_sequence<int, (N - 1) / 4>{});
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace perm
+
+// Instantiate launch_n for a list of N in one translation unit; the
+// dispatcher in ryser_dense.hip calls launch_dense_range_<TAG>.
+#define PERM_DENSE_RANGE(TAG, LO, HI)                                                         \
+  namespace perm {                                                                            \
+  template <int... I>                                                                         \
+  static hipError_t dispatch_##TAG(const DenseLaunch& L, hipStream_t s,                       \
+                                   std::integer_sequence<int, I...>) {                        \
+    hipError_t e = hipErrorInvalidValue;                                                      \
+    ((L.n == LO + I ? (e = launch_n<LO + I>(L, s), 0) : 0), ...);                             \
+    return e;                                                                                 \
+  }                                                                                           \
+  hipError_t launch_dense_range_##TAG(const DenseLaunch& L, hipStream_t s) {                  \
+    return dispatch_##TAG(L, s, std::make_integer_sequence<int, HI - LO + 1>{});              \
+  }                                                                                           \
+  }

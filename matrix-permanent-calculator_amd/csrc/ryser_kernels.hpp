@@ -9,6 +9,9 @@ namespace perm {
 
 // Largest n the dense kernels are instantiated for (x lives in 2n VGPRs).
 constexpr int kMaxDenseN = 48;
+// Smallest n with pair (zero-skipping) variants; below it the whole
+// permanent takes microseconds.
+constexpr int kMinPairN = 16;
 
 // One dense launch over all 2^(n-1) subsets of one matrix.
 struct DenseLaunch {
@@ -22,6 +25,7 @@ struct DenseLaunch {
   int walk_bits;        // subsets walked per lane = 2^walk_bits
   int high_bits;        // walk_bits + high_bits = n - 1
   int mode;             // 0 FP64, 1 double-double products, 2 double-double X too
+  int nu;               // active rows of Gray column 0 (multiple of 4, < n) or n
 };
 
 // Enqueue the dense kernel and the grid reduction on `stream`.

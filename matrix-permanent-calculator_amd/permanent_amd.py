@@ -38,7 +38,8 @@ class PermResultC(ctypes.Structure):
                 ("sum_lo", ctypes.c_double), ("abs_sum", ctypes.c_double),
                 ("err_bound", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
-                ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double)]
+                ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
+                ("nu", ctypes.c_int)]
 
 
 @dataclass
@@ -54,11 +55,12 @@ class PermResult:
     mode: int
     integral: int
     err_est: float
+    nu: int
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
-                   r.n, r.mode, r.integral, r.err_est)
+                   r.n, r.mode, r.integral, r.err_est, r.nu)
 
 
 _LIB = None
