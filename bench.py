@@ -48,6 +48,176 @@ def cpu_baseline(mat, log2_sample: int, threads: int):
     }
 
 
+def sparse_leg(P, reps: int):
+    """Sparse path (-s, ID 4) at n=36, density 0.2, every row and column
+    nonzero, U(0,1], seed 4, resident plan: subsets/s, the dense-equivalent
+    2n flops/subset and the sparse-minimal n + nnz/n flops/subset."""
+    m = P.synthetic_matrix(36, 0.2, seed=4, full_support=True)
+    nnz = int((m > 0).sum())
+    plan = P.Plan(m, sparse_path=True)
+    plan.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = plan.run()
+    wall = (time.perf_counter() - t0) / reps
+    plan.close()
+    subsets = 2.0 ** 35
+    return {
+        "workload": "sparse path, n=36, density 0.2 (every row/column nonzero), U(0,1], "
+                    "seed 4, 2^35 subsets, matrix resident",
+        "subsets_per_s": subsets / wall,
+        "ms_per_run": wall * 1e3,
+        "kernel_ms": r.kernel_ms,
+        "nu": r.nu,
+        "pct_fp64_peak_dense_equivalent": 100.0 * 72.0 * subsets / wall / 1e12 / FP64_PEAK_TFLOPS,
+        "pct_fp64_peak_sparse_minimal": 100.0 * (36.0 + nnz / 36.0) * subsets / wall / 1e12
+                                        / FP64_PEAK_TFLOPS,
+        "value": r.value,
+        "err_bound": r.err_bound,
+    }
+
+
+def cpu_leg(P, threads: int, reps: int = 11):
+    """`perman -c` (the library's OpenMP FP64 Ryser) at n=20, density 0.5,
+    U(0,1], seed 1: median of `reps` runs."""
+    m = P.synthetic_matrix(20, 0.5, seed=1)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        v = P.cpu_dense(m, threads)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    med = ts[len(ts) // 2]
+    return {"workload": "CPU path (-c), n=20 FP64, density 0.5, U(0,1], seed 1",
+            "threads": threads, "median_s": med, "subsets_per_s": 2.0 ** 19 / med, "value": v}
+
+
+def batched_leg(P, count: int, seed: int):
+    """Batched mixed-size stream (4096 matrices, n uniform in [12, 28],
+    density uniform in [0.2, 0.8], nonzeros U(0,1], seed 5): one resident
+    batch replayed as a graph, against one drop-in call per matrix (the
+    reference's usage: host setup, upload, launch, download per matrix)."""
+    mats = P.mixed_batch(count, 12, 28, 0.2, 0.8, seed)
+    subsets = sum(2.0 ** (m.shape[0] - 1) for m in mats)
+    t0 = time.perf_counter()
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, _ = b.run()  # builds: plan items, upload, capture the graph
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tuned = b.autotune(reps=2)  # picks the work split on this batch
+    t_tune = time.perf_counter() - t0
+    res, _ = b.run()  # rebuild with the chosen split
+    reps = 3
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(reps):
+        res, ms = b.run()
+        dev.append(ms)
+    t_batch = (time.perf_counter() - t0) / reps
+    b.close()
+    t0 = time.perf_counter()
+    plain = [P.dense(m).value for m in mats]
+    t_plain = time.perf_counter() - t0
+    worst = max(abs(r.value - v) / max(abs(v), 1e-300) for r, v in zip(res, plain))
+    return {
+        "workload": f"{count} matrices, n~U{{12..28}}, density~U[0.2,0.8], U(0,1], seed {seed}",
+        "subsets": subsets,
+        "batched_perms_per_s": count / t_batch,
+        "batched_subsets_per_s": subsets / t_batch,
+        "batched_device_ms": sum(dev) / len(dev),
+        "batched_first_run_s": t_first,
+        "autotuned_split": tuned,
+        "autotune_s": t_tune,
+        "batched_end_to_end_perms_per_s": count / t_first,
+        "plain_perms_per_s": count / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "speedup_end_to_end_vs_plain": t_plain / t_first,
+        "max_rel_diff_vs_plain": worst,
+    }
+
+
+def served_leg(P, count: int, load: float, seed: int):
+    """Served stream: `count` requests, n in [16, 36] with P(n) ~ 2^(-(n-16)/4),
+    density U[0.1, 0.9], half 0/1 and half FP64, a third through the sparse
+    path (seed 6).  Capacity: every request submitted at once (closed loop).
+    Then an open loop of Poisson arrivals at `load` x capacity; p50/p99 of
+    submit-to-result latency.  Plain: one drop-in call per request."""
+    import threading
+
+    import numpy as np
+
+    reqs = P.served_workload(count, seed)
+    work = sum(2.0 * m.shape[0] * 2.0 ** (m.shape[0] - 1) for m, _ in reqs)
+    with P.Server() as srv:
+        for m, sp in reqs[:32]:  # warm the kernels of the common sizes
+            srv.wait(srv.submit(m, sparse_path=sp))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(m, sparse_path=sp) for m, sp in reqs]
+        closed = [srv.wait(t) for t in tickets]
+        t_closed = time.perf_counter() - t0
+    capacity = count / t_closed
+    rate = load * capacity
+    gaps = -np.log(1.0 - P.uniforms((seed << 20) + (1 << 18), count)) / rate
+    arrivals = np.cumsum(gaps)
+    lat = [None] * count
+    with P.Server() as srv:
+        done = threading.Event()
+        queue = []
+        cv = threading.Condition()
+
+        def collector():
+            k = 0
+            while k < count:
+                with cv:
+                    while len(queue) <= k:
+                        cv.wait()
+                    t = queue[k]
+                lat[k] = srv.wait(t)[1]
+                k += 1
+            done.set()
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        for i, (m, sp) in enumerate(reqs):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            t = srv.submit(m, sparse_path=sp)
+            with cv:
+                queue.append(t)
+                cv.notify()
+        done.wait()
+        th.join()
+        t_open = time.perf_counter() - t0
+    lat_ms = np.array(lat)
+    t0 = time.perf_counter()
+    plain = []
+    for m, sp in reqs:
+        plain.append(P.sparse(m, algo_id=4).value if sp else P.dense(m).value)
+    t_plain = time.perf_counter() - t0
+    worst = max(abs(r.value - v) / max(abs(v), r.err_bound, 1e-300)
+                for (r, _), v in zip(closed, plain))
+    return {
+        "workload": f"{count} requests, n in [16,36] with P(n)~2^(-(n-16)/4), density "
+                    f"U[0.1,0.9], half 0/1 half FP64, 1/3 sparse path, seed {seed}",
+        "algorithmic_flops": work,
+        "capacity_req_per_s": capacity,
+        "capacity_tflops": work / t_closed / 1e12,
+        "offered_load": load,
+        "offered_req_per_s": rate,
+        "achieved_req_per_s": count / t_open,
+        "latency_p50_ms": float(np.percentile(lat_ms, 50)),
+        "latency_p99_ms": float(np.percentile(lat_ms, 99)),
+        "latency_max_ms": float(lat_ms.max()),
+        "plain_req_per_s": count / t_plain,
+        "capacity_vs_plain": t_plain / t_closed,
+        "max_rel_diff_vs_plain": worst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +226,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=32)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batched/served legs")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-count", type=int, default=20000)
+    ap.add_argument("--serve-load", type=float, default=0.7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this "
@@ -69,11 +243,12 @@ def main():
     plan = P.Plan(mat)
     for _ in range(args.warmup):
         plan.run()
-    kernel_ms = []
+    kernel_ms, main_ms_l = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):  # each run() waits for its stream
         r = plan.run()
         kernel_ms.append(r.kernel_ms)
+        main_ms_l.append(r.main_ms)
     wall = time.perf_counter() - t0
     plan.close()
 
@@ -81,8 +256,9 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = subsets / (ms_per_step / 1e3)
     k_ms = sum(kernel_ms) / len(kernel_ms)
-    flops = 2.0 * N * subsets
-    achieved = flops / (k_ms / 1e3) / 1e12
+    main_ms = sum(main_ms_l) / len(main_ms_l)
+    flops = 2.0 * N * r.main_subsets
+    achieved = flops / (main_ms / 1e3) / 1e12
     out = {
         "metric": "Ryser subsets/s + %FP64 peak at n=40; mixed-size perms/s & p99 vs plain",
         "value": value,
@@ -105,8 +281,10 @@ def main():
             "subsets_per_step": subsets,
         },
         "pct_fp64_peak_algorithmic": 100.0 * achieved / FP64_PEAK_TFLOPS,
+        "pct_fp64_peak_whole_step": 100.0 * 2.0 * N * subsets / (ms_per_step / 1e3) / 1e12
+                                    / FP64_PEAK_TFLOPS,
         "result": {"value": r.value, "err_est": r.err_est, "err_bound": r.err_bound,
-                   "abs_sum": r.abs_sum, "nu": r.nu},
+                   "abs_sum": r.abs_sum, "nu": r.nu, "ext_k": r.ext_k},
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
@@ -114,16 +292,26 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": f"perm::ryser_dense_kernel<40,3,0,{r.nu}>",
-            "kernel_ms": k_ms,
+            "kernel": f"perm::ryser_dense_kernel<40,3,0,{r.nu},false> (main pass)",
+            "kernel_ms": main_ms,
+            "all_kernels_ms": k_ms,
+            "subsets_per_launch": r.main_subsets,
             "algorithmic_flops_per_launch": flops,
             "note": "FP64 VALU-bound. Algorithmic work is 2n flops per subset (n adds + n "
                     "multiplies); the pair kernel issues n + nu + 1/2 FP64 instructions per "
                     "subset (adds of zero skipped, the passive-row product shared by the two "
                     "subsets of a pair), so the rate can exceed the 50% of the FMA-counted peak "
-                    "that 2n non-fusable instructions would allow",
+                    "that 2n non-fusable instructions would allow. The main pass walks the "
+                    "lanes that meet no extreme subset (|S| <= ext_k or >= n-1-ext_k); the side "
+                    "pass and the double-double extreme-subset kernel are in all_kernels_ms",
         },
     }
+    out["sparse"] = sparse_leg(P, 3)
+    out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
+    if not args.no_mixed:
+        out["batched"] = batched_leg(P, args.batch_count, 5)
+        if args.serve_count > 0:
+            out["served"] = served_leg(P, args.serve_count, args.serve_load, 6)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mat, args.cpu_sample_log2, args.cpu_threads)
     print(json.dumps(out), flush=True)

@@ -31,6 +31,7 @@ extern "C" {
 #define PERM_ERR_SIZE 2     /* n above the largest compiled size */
 #define PERM_ERR_HIP 3      /* a HIP runtime call failed */
 #define PERM_ERR_NODEV 4    /* no usable GPU */
+#define PERM_PENDING 5      /* perm_server_poll: the request is not finished yet */
 
 #define PERM_DTYPE_I32 0
 #define PERM_DTYPE_F32 1
@@ -41,10 +42,13 @@ extern "C" {
  * |scale| * sum of |terms| (the condition of the alternating sum) and
  * err_bound the library's stated worst-case error for `value` (every rounding
  * aligned); err_est the expected error when roundings are independent,
- * 3 (n + sqrt(n)) u abs_sum / sqrt(2^(n-1)).  mode: 0 = FP64 products
- * (the fast kernel), 1 = double-double products (used for integer-valued
- * matrices whose FP64 result could not be guaranteed to round to the exact
- * integer). */
+ * (n + sqrt(n)) u sqrt(Q) with Q >= scale^2 * sum of term^2 measured by the
+ * kernel.  mode: 0 = FP64 products (the fast kernel), 1 = double-double
+ * products (used for integer-valued matrices whose FP64 result could not be
+ * guaranteed to round to the exact integer), 2 = double-double X and
+ * products.  ext_k >= 0 (mode 0): the subsets with |S| <= ext_k or
+ * |S| >= n-1-ext_k were summed in double-double with exact X instead of by
+ * the FP64 walk. */
 typedef struct perm_result {
   double value;
   double sum_hi;
@@ -59,6 +63,9 @@ typedef struct perm_result {
   int integral;         /* 1 if every entry is an integer (X is then exact) */
   double err_est;       /* statistical error estimate (independent roundings) */
   int nu;               /* FP64 kernel: rows touched by Gray column 0 (pair variant), or n */
+  int ext_k;            /* extreme-subset cut of the FP64 walk, -1 for none */
+  double main_ms;       /* single-matrix path: device time of the main walk kernel */
+  double main_subsets;  /* subsets that kernel evaluated */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */
@@ -150,6 +157,97 @@ int perm_plan_run(perm_plan* plan, perm_result* out);
  * integer matrix needs it to round to the exact integer. */
 int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out);
 void perm_plan_destroy(perm_plan* plan);
+
+/* ---- batches of mixed n (extension) --------------------------------------
+ * Many matrices (n <= 40) resident on the device, computed by persistent
+ * kernels (one per (n, nu) variant, concurrent on several streams) captured
+ * in one hipGraph; perm_batch_run replays it.  A matrix is cut into 64-lane
+ * items of about 2^item_log2 subsets that depend only on its own n, so its
+ * result is identical alone and in any mix.  item_log2 (default 18),
+ * streams (default 4) and max_grid (workgroups per variant kernel, default
+ * 2048) are the work-split parameters (<= 0 keeps a value). */
+typedef struct perm_batch perm_batch;
+int perm_batch_create(perm_batch** batch);
+int perm_batch_set(perm_batch* batch, int item_log2, int streams, int max_grid);
+int perm_batch_add_dense(perm_batch* batch, int dtype, const void* mat, int nov, int* index);
+int perm_batch_add_sparse(perm_batch* batch, int dtype, const void* mat, const int* cptrs,
+                          const int* rows, const void* cvals, int nov, int* index);
+int perm_batch_size(const perm_batch* batch);
+/* results: one per matrix, in insertion order; device_ms: graph time. */
+int perm_batch_run(perm_batch* batch, perm_result* results, double* device_ms);
+void perm_batch_destroy(perm_batch* batch);
+/* Work split of a batch and the autotuner that picks it: subsets per item
+ * (2^item_log2: Gray-code chunk of 2^(item_log2-6) subsets per lane),
+ * streams carrying the per-variant kernels, workgroups per variant kernel.
+ * perm_batch_autotune times candidate splits on the batch itself (a
+ * coordinate search, each candidate rebuilt and replayed `reps` times) and
+ * keeps the fastest; results then follow that split. */
+typedef struct perm_batch_tuning {
+  int item_log2, streams, max_grid;
+  double device_ms;  /* mean device time of the chosen split */
+  int trials;        /* candidate splits timed */
+} perm_batch_tuning;
+int perm_batch_autotune(perm_batch* batch, int reps, perm_batch_tuning* best);
+int perm_batch_get(const perm_batch* batch, perm_batch_tuning* out);
+
+/* ---- the reference's Python/MATLAB connector (py_connector.c:63-190) ------
+ * perm_read_calculate_return(filename, algorithm, nt, x, y, z) replaces
+ * read_calculate_return (py_connector.c:63) and
+ * perm_matlab_calculate_return(mat, algorithm, nt, x, y, z, nov, nnz) replaces
+ * matlab_calculate_return (py_connector.c:146); perm_connect replaces
+ * connect (:59).  Algorithms 4-8 (the exact ones: 4/6/7 sparse, 5/8 dense)
+ * run on the GPU; 0-3 (approximations) return NaN with PERM_ERR_ARG; nt, x,
+ * y, z are unused.  Unlike the reference's connector, the file's values are
+ * used (it reads every file as a 0/1 pattern) and the result is not
+ * truncated to int. */
+void perm_connect(void);
+double perm_read_calculate_return(const char* filename, int algorithm, int nt, int x, int y,
+                                  int z);
+double perm_matlab_calculate_return(const int* mat, int algorithm, int nt, int x, int y, int z,
+                                    int nov, int nnz);
+
+/* ---- server: continuous batching of requests (extension) ---------------
+ * No counterpart in the reference (its entry points compute one permanent
+ * per call).  Requests (n <= 40, dense or sparse) are submitted from any
+ * thread and return a ticket at once; a scheduler thread packs the 64-lane
+ * items of all admitted requests into iterations of a few milliseconds of
+ * GPU work, shortest remaining work first, and sums each request's items in
+ * a fixed order: a result is bitwise the same alone or in any mix.
+ * Admission is bounded by a device arena of resident matrices and by the
+ * queued work.  perm_server_wait blocks until the request is done and
+ * frees its ticket; perm_server_poll returns PERM_PENDING until then.  In a
+ * served result, kernel_ms holds the request's latency (submit to result,
+ * also returned in *latency_ms).  Integer results are exact under the same
+ * rule as perm_dense (rounded when the stated error is below 1/2, else a
+ * double-double product pass). */
+typedef struct perm_server perm_server;
+typedef struct perm_server_config {
+  int item_log2;          /* subsets per item, log2 (default 18) */
+  int streams;            /* compute streams (default 4, at most 8) */
+  int inflight;           /* iterations queued on the GPU (default 2, at most 4) */
+  int max_items;          /* items per iteration (default 32768) */
+  double iteration_us;    /* work per iteration at full FP64 issue rate, microseconds (4000) */
+  int max_resident;       /* matrices resident on the device (default 2048) */
+  double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */
+} perm_server_config;
+typedef struct perm_server_stats {
+  long long submitted, admitted, completed, iterations, items, pending;
+} perm_server_stats;
+void perm_server_default_config(perm_server_config* cfg);
+int perm_server_create(const perm_server_config* cfg, perm_server** server);
+int perm_server_submit_dense(perm_server* server, int dtype, const void* mat, int nov,
+                             unsigned long long* id);
+int perm_server_submit_sparse(perm_server* server, int dtype, const void* mat, const int* cptrs,
+                              const int* rows, const void* cvals, int nov,
+                              unsigned long long* id);
+int perm_server_wait(perm_server* server, unsigned long long id, perm_result* out,
+                     double* latency_ms);
+int perm_server_poll(perm_server* server, unsigned long long id, perm_result* out,
+                     double* latency_ms);
+int perm_server_stats_get(perm_server* server, perm_server_stats* out);
+/* Finishes every submitted request, then frees the server (uncollected
+ * results are dropped). */
+void perm_server_destroy(perm_server* server);
 
 /* ---- CPU path of `perman -c` (parallel_perman64, algo.h:663; and
  * parallel_perman64_sparse, algo.h:569).  Native OpenMP Ryser with FP64 X

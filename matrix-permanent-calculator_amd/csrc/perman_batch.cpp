@@ -1,0 +1,442 @@
+// perman_batch.cpp -- batched permanents of mixed n on one GPU (C ABI in
+// include/perman_hip.h, perm_batch_*).
+//
+// A batch holds many matrices resident on the device.  Each matrix is split
+// into items of one wave (64 lanes) walking 2^w Gray subsets per lane, with w
+// chosen from the batch's item size (2^item_log2 subsets per item) and n
+// alone -- so a matrix's items, and therefore its result, do not depend on
+// what else is in the batch.  Items are grouped by kernel variant (n, nu);
+// one persistent kernel per variant drains its group, the groups running
+// concurrently on several streams.  The whole launch sequence (counter reset,
+// variant kernels, per-matrix reduction, result copy) is captured once in a
+// hipGraph and replayed by perm_batch_run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/perman_hip.h"
+#include "perman_internal.hpp"
+#include "ryser_batch.hpp"
+#include "ryser_host.hpp"
+#include "ryser_kernels.hpp"
+
+namespace {
+
+using perm::BatchItem;
+using perm::BatchMatrix;
+using perm::Partial;
+
+constexpr int kMaxStreams = 8;
+
+struct Entry {
+  int n = 0, nu = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  bool integral = false;
+  size_t col_off = 0, x0_off = 0;  // offsets (doubles) into the data buffer
+  uint32_t first_item = 0, item_count = 0;
+};
+
+}  // namespace
+
+struct perm_batch {
+  int device = 0;
+  int item_log2 = 18;   // subsets per item (64 lanes x 2^(item_log2-6) steps)
+  int streams = 4;
+  int max_grid = 2048;  // workgroups per variant kernel
+  std::vector<double> data;
+  std::vector<Entry> ents;
+  std::vector<BatchItem> items;
+  struct Group {
+    int vid;
+    uint32_t begin, end;
+    double work;
+  };
+  std::vector<Group> groups;
+  // device
+  void* dmem = nullptr;
+  size_t dbytes = 0;
+  double* d_data = nullptr;
+  BatchMatrix* d_mats = nullptr;
+  BatchItem* d_items = nullptr;
+  Partial* d_partials = nullptr;
+  Partial* d_results = nullptr;
+  uint32_t* d_counters = nullptr;
+  Partial* h_results = nullptr;  // pinned
+  hipStream_t s[kMaxStreams] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams] = {}, ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph = nullptr;
+  bool built = false;
+};
+
+namespace {
+
+void release_device(perm_batch* B) {
+  if (B->graph) (void)hipGraphExecDestroy(B->graph);
+  B->graph = nullptr;
+  if (B->dmem) (void)hipFree(B->dmem);
+  B->dmem = nullptr;
+  if (B->h_results) (void)hipHostFree(B->h_results);
+  B->h_results = nullptr;
+  B->built = false;
+}
+
+
+int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
+  if (n > perm::kMaxBatchN)
+    return perm::fail(PERM_ERR_SIZE, "batch matrices are limited to n <= " +
+                                         std::to_string(perm::kMaxBatchN));
+  Entry e;
+  e.n = n;
+  e.integral = p.integral;
+  // the batch kernels carry a subset of the pair widths: widen nu to one
+  // of them (extra rows of zeros in column 0 are exact)
+  e.nu = perm::batch_nu(n, p.nu);
+  e.variant = perm::batch_variant(n, e.nu);
+  if (e.variant < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel for this n");
+  e.col_off = B->data.size();
+  B->data.insert(B->data.end(), p.cs.begin(), p.cs.end());
+  e.x0_off = B->data.size();
+  B->data.insert(B->data.end(), p.x0.begin(), p.x0.end());
+  while (B->data.size() % 32) B->data.push_back(0.0);  // 256-B alignment of the next store
+  B->ents.push_back(e);
+  B->built = false;
+  if (index) *index = (int)B->ents.size() - 1;
+  return perm::ok();
+}
+
+#define HIP_TRYB(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return perm::fail(PERM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Plan items, upload, capture the launch graph.
+int build(perm_batch* B) {
+  HIP_TRYB(hipSetDevice(B->device));
+  release_device(B);
+  const size_t nm = B->ents.size();
+  if (nm == 0) return perm::fail(PERM_ERR_ARG, "empty batch");
+  // items, grouped by variant, groups ordered by decreasing work
+  std::vector<std::vector<uint32_t>> by_var;
+  std::vector<int> var_ids;
+  for (size_t m = 0; m < nm; ++m) {
+    Entry& e = B->ents[m];
+    perm::batch_split(e.n, B->item_log2, e.walk_bits, e.high_bits);
+    auto it = std::find(var_ids.begin(), var_ids.end(), e.variant);
+    if (it == var_ids.end()) {
+      var_ids.push_back(e.variant);
+      by_var.emplace_back();
+      it = var_ids.end() - 1;
+    }
+    by_var[it - var_ids.begin()].push_back((uint32_t)m);
+  }
+  B->items.clear();
+  B->groups.clear();
+  std::vector<size_t> order(var_ids.size());
+  std::vector<double> work(var_ids.size(), 0.0);
+  for (size_t v = 0; v < var_ids.size(); ++v) {
+    order[v] = v;
+    for (uint32_t m : by_var[v]) {
+      const Entry& e = B->ents[m];
+      work[v] += std::ldexp((double)(e.n + e.nu), e.n - 1);
+    }
+  }
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+  for (size_t oi : order) {
+    perm_batch::Group g{var_ids[oi], (uint32_t)B->items.size(), 0, work[oi]};
+    for (uint32_t m : by_var[oi]) {
+      Entry& e = B->ents[m];
+      const uint64_t lanes = 1ull << e.high_bits;
+      const uint64_t cnt = (lanes + perm::kItemLanes - 1) / perm::kItemLanes;
+      e.first_item = (uint32_t)B->items.size();
+      e.item_count = (uint32_t)cnt;
+      for (uint64_t i = 0; i < cnt; ++i) B->items.push_back({m, 0u, i * perm::kItemLanes});
+    }
+    g.end = (uint32_t)B->items.size();
+    B->groups.push_back(g);
+  }
+  if (B->items.size() > 0xffffffffull) return perm::fail(PERM_ERR_SIZE, "too many items");
+  // device memory
+  const size_t data_b = align256(B->data.size() * sizeof(double));
+  const size_t mats_b = align256(nm * sizeof(BatchMatrix));
+  const size_t items_b = align256(B->items.size() * sizeof(BatchItem));
+  const size_t part_b = align256(B->items.size() * sizeof(Partial));
+  const size_t res_b = align256(nm * sizeof(Partial));
+  const size_t ctr_b = align256(B->groups.size() * sizeof(uint32_t));
+  B->dbytes = data_b + mats_b + items_b + part_b + res_b + ctr_b;
+  HIP_TRYB(hipMalloc(&B->dmem, B->dbytes));
+  char* p = (char*)B->dmem;
+  B->d_data = (double*)p;
+  B->d_mats = (BatchMatrix*)(p + data_b);
+  B->d_items = (BatchItem*)(p + data_b + mats_b);
+  B->d_partials = (Partial*)(p + data_b + mats_b + items_b);
+  B->d_results = (Partial*)(p + data_b + mats_b + items_b + part_b);
+  B->d_counters = (uint32_t*)(p + data_b + mats_b + items_b + part_b + res_b);
+  HIP_TRYB(hipHostMalloc((void**)&B->h_results, nm * sizeof(Partial), hipHostMallocDefault));
+  std::vector<BatchMatrix> mats(nm);
+  for (size_t m = 0; m < nm; ++m) {
+    const Entry& e = B->ents[m];
+    mats[m] = {B->d_data + e.col_off, B->d_data + e.x0_off, 1ull << e.high_bits,
+               e.walk_bits, e.high_bits, e.variant, e.first_item, e.item_count, 0u};
+  }
+  hipStream_t s0 = B->s[0];
+  HIP_TRYB(hipMemcpyAsync(B->d_data, B->data.data(), B->data.size() * sizeof(double),
+                          hipMemcpyHostToDevice, s0));
+  HIP_TRYB(hipMemcpyAsync(B->d_mats, mats.data(), nm * sizeof(BatchMatrix), hipMemcpyHostToDevice,
+                          s0));
+  HIP_TRYB(hipMemcpyAsync(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
+                          hipMemcpyHostToDevice, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  // capture: reset -> fork -> variant kernels round-robin over streams -> join
+  // -> reduce -> result copy
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  hipGraph_t g;
+  HIP_TRYB(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)B->groups.size(), s0));
+  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+  for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+    const auto& G = B->groups[gi];
+    const uint32_t waves = G.end - G.begin;
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, B->max_grid));
+    HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
+                                        B->d_counters + gi, B->d_partials, grid,
+                                        B->s[gi % ns]));
+  }
+  for (int i = 1; i < ns; ++i) {
+    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+  }
+  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, (uint32_t)nm, s0));
+  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
+                          s0));
+  HIP_TRYB(hipStreamEndCapture(s0, &g));
+  hipError_t ie = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess)
+    return perm::fail(PERM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  B->built = true;
+  return perm::ok();
+}
+
+}  // namespace
+
+namespace perm {
+
+void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial& r,
+                      perm_result& o) {
+  std::memset(&o, 0, sizeof(o));
+  const double scale = (double)(4 * (n & 1) - 2);
+  o.n = n;
+  o.sum_hi = r.hi;
+  o.sum_lo = r.lo;
+  o.value = scale * (r.hi + r.lo);
+  o.abs_sum = std::fabs(scale) * r.abs_sum;
+  o.err_bound =
+      error_statement(o.abs_sum, o.value, n, walk_bits, dense_unroll_bits(n), integral, 0);
+  o.err_est = error_estimate(scale * scale * r.sq_sum, o.value, n, 0, integral);
+  o.subsets = std::ldexp(1.0, n - 1);
+  o.integral = integral ? 1 : 0;
+  o.nu = nu;
+  o.ext_k = -1;
+  o.mode = 0;
+  round_integral(integral, o.value, o.err_bound, o.err_est);
+  o.status = PERM_OK;
+}
+
+bool needs_dd_pass(const perm_result& o) {
+  return o.integral && o.mode == 0 && o.err_bound >= 0.5 &&
+         std::fabs(o.value) < 9007199254740992.0;
+}
+
+void batch_split(int n, int item_log2, int& walk, int& high) {
+  const int B = n - 1;
+  const int R = dense_unroll_bits(n);
+  int w = std::min(item_log2 - 6, B - 6);
+  if (w < R) w = std::min(R, B);
+  walk = w;
+  high = B - w;
+}
+}  // namespace perm
+
+extern "C" {
+
+int perm_batch_create(perm_batch** out) {
+  if (!out) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (int s = perm::select_device()) return s;
+  auto* B = new perm_batch();
+  B->device = perm::current_device();
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < kMaxStreams && e == hipSuccess; ++i)
+    e = hipStreamCreateWithFlags(&B->s[i], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming);
+  for (int i = 0; i < kMaxStreams && e == hipSuccess; ++i)
+    e = hipEventCreateWithFlags(&B->ev_join[i], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&B->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&B->ev1);
+  if (e != hipSuccess) {
+    perm_batch_destroy(B);
+    return perm::fail(PERM_ERR_HIP, std::string("batch streams: ") + hipGetErrorString(e));
+  }
+  *out = B;
+  return perm::ok();
+}
+
+int perm_batch_set(perm_batch* B, int item_log2, int streams, int max_grid) {
+  if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
+  if (item_log2 > 0) {
+    if (item_log2 < 6 || item_log2 > 30) return perm::fail(PERM_ERR_ARG, "item_log2 in [6, 30]");
+    B->item_log2 = item_log2;
+  }
+  if (streams > 0) B->streams = std::min(streams, kMaxStreams);
+  if (max_grid > 0) B->max_grid = max_grid;
+  B->built = false;
+  return perm::ok();
+}
+
+int perm_batch_add_dense(perm_batch* B, int dtype, const void* mat, int nov, int* index) {
+  if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
+  perm::Prepared p;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
+  return add_prepared(B, nov, p, index);
+}
+
+int perm_batch_add_sparse(perm_batch* B, int dtype, const void* mat, const int* cptrs,
+                          const int* rows, const void* cvals, int nov, int* index) {
+  if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
+  perm::Prepared p;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  return add_prepared(B, nov, p, index);
+}
+
+int perm_batch_size(const perm_batch* B) { return B ? (int)B->ents.size() : 0; }
+
+int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
+  if (!B || !results) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (!B->built)
+    if (int s = build(B)) return s;
+  HIP_TRYB(hipSetDevice(B->device));
+  HIP_TRYB(hipEventRecord(B->ev0, B->s[0]));
+  HIP_TRYB(hipGraphLaunch(B->graph, B->s[0]));
+  HIP_TRYB(hipEventRecord(B->ev1, B->s[0]));
+  HIP_TRYB(hipStreamSynchronize(B->s[0]));
+  float ms = 0.f;
+  HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+  if (device_ms) *device_ms = ms;
+  for (size_t m = 0; m < B->ents.size(); ++m) {
+    const Entry& e = B->ents[m];
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.nu, B->h_results[m], results[m]);
+    results[m].kernel_ms = ms;
+    if (perm::needs_dd_pass(results[m])) {
+      // integer matrix whose FP64 error could reach 1/2: double-double
+      // products through the single-matrix path (same column store)
+      perm::Prepared p;
+      const size_t cs_len = (size_t)(2 * (e.n - 1) + 1) * e.n;
+      p.cs.assign(B->data.begin() + e.col_off, B->data.begin() + e.col_off + cs_len);
+      p.x0.assign(B->data.begin() + e.x0_off, B->data.begin() + e.x0_off + e.n);
+      p.nu = e.n;
+      p.integral = true;
+      if (int s = perm::run_prepared(e.n, p, 1, &results[m])) return s;
+    }
+  }
+  return perm::ok();
+}
+
+// Autotuner of the batch's work split: item size (Gray-code chunk length per
+// lane, hence lanes per matrix), streams (variant kernels in flight) and the
+// grid of each persistent variant kernel (waves per variant, hence matrices
+// in flight per CU).  Coordinate search from the current setting: item_log2
+// over {12, 14, 16, 18, 20}, then streams over {1, 2, 4, 8}, then max_grid
+// over {512, 1024, 2048, 4096}; each candidate is rebuilt (plan, upload,
+// graph capture) and timed over `reps` graph replays (device time, HIP
+// events); the fastest setting is kept.
+int perm_batch_autotune(perm_batch* B, int reps, perm_batch_tuning* best) {
+  if (!B || !best) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (B->ents.empty()) return perm::fail(PERM_ERR_ARG, "empty batch");
+  if (reps < 1) reps = 1;
+  std::vector<perm_result> res(B->ents.size());
+  int trials = 0;
+  auto measure = [&](int il, int ns, int grid, double& ms_out) -> int {
+    B->item_log2 = il;
+    B->streams = ns;
+    B->max_grid = grid;
+    B->built = false;
+    double ms = 0.0, sum = 0.0;
+    if (int s = perm_batch_run(B, res.data(), &ms)) return s;  // build + warm
+    for (int r = 0; r < reps; ++r) {
+      if (int s = perm_batch_run(B, res.data(), &ms)) return s;
+      sum += ms;
+    }
+    ms_out = sum / reps;
+    ++trials;
+    return PERM_OK;
+  };
+  int bi = B->item_log2, bs = B->streams, bg = B->max_grid;
+  double bms = 0.0;
+  if (int s = measure(bi, bs, bg, bms)) return s;
+  for (int il : {12, 14, 16, 18, 20}) {
+    if (il == bi) continue;
+    double ms;
+    if (int s = measure(il, bs, bg, ms)) return s;
+    if (ms < bms) bms = ms, bi = il;
+  }
+  const int i0 = bi;
+  for (int ns : {1, 2, 4, 8}) {
+    if (ns == bs) continue;
+    double ms;
+    if (int s = measure(i0, ns, bg, ms)) return s;
+    if (ms < bms) bms = ms, bs = ns;
+  }
+  const int s0 = bs;
+  for (int grid : {512, 1024, 2048, 4096}) {
+    if (grid == bg) continue;
+    double ms;
+    if (int s = measure(i0, s0, grid, ms)) return s;
+    if (ms < bms) bms = ms, bg = grid;
+  }
+  B->item_log2 = bi;
+  B->streams = bs;
+  B->max_grid = bg;
+  B->built = false;
+  best->item_log2 = bi;
+  best->streams = bs;
+  best->max_grid = bg;
+  best->device_ms = bms;
+  best->trials = trials;
+  return perm::ok();
+}
+
+int perm_batch_get(const perm_batch* B, perm_batch_tuning* out) {
+  if (!B || !out) return perm::fail(PERM_ERR_ARG, "null pointer");
+  out->item_log2 = B->item_log2;
+  out->streams = B->streams;
+  out->max_grid = B->max_grid;
+  out->device_ms = 0.0;
+  out->trials = 0;
+  return perm::ok();
+}
+
+void perm_batch_destroy(perm_batch* B) {
+  if (!B) return;
+  (void)hipSetDevice(B->device);
+  for (int i = 0; i < kMaxStreams; ++i)
+    if (B->s[i]) (void)hipStreamSynchronize(B->s[i]);
+  release_device(B);
+  for (int i = 0; i < kMaxStreams; ++i) {
+    if (B->s[i]) (void)hipStreamDestroy(B->s[i]);
+    if (B->ev_join[i]) (void)hipEventDestroy(B->ev_join[i]);
+  }
+  if (B->ev_fork) (void)hipEventDestroy(B->ev_fork);
+  if (B->ev0) (void)hipEventDestroy(B->ev0);
+  if (B->ev1) (void)hipEventDestroy(B->ev1);
+  delete B;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// perman_internal.hpp -- host functions shared by the C ABI translation units
+// (single-matrix, batch, serving).
+#pragma once
+#include <string>
+#include <vector>
+
+namespace perm {
+
+// Per-thread status (perm_last_status / perm_last_error) and device.
+int fail(int code, const std::string& msg);
+int ok();
+int last_status();
+const char* last_error();
+int current_device();
+void set_device(int device);
+// hipSetDevice(current device) after checking it exists.
+int select_device();
+int env_int(const char* name, int dflt);
+int check_n(int n);
+
+// A matrix prepared for the Ryser kernels: start vector, signed column
+// store (walk order applied), active rows of Gray column 0 (nu; n when the
+// pair kernels do not apply), whether every entry is an integer.
+struct Prepared {
+  std::vector<double> x0, cs;
+  int nu = 0;
+  bool integral = false;
+};
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p);
+int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
+                   const void* cvals, int n, Prepared& p);
+
+}  // namespace perm
+
+struct perm_result;
+
+namespace perm {
+
+// One prepared matrix through the single-matrix path on the current device
+// (per-device cached buffers).  mode -1: automatic (FP64, then double-double
+// products for integer matrices whose FP64 error could reach 1/4); 0, 1, 2:
+// that kernel mode only.
+int run_prepared(int n, const Prepared& p, int mode, perm_result* out);
+
+struct Partial;
+// Result of a matrix computed as 64-lane items (batch and server paths):
+// value, error statement and estimate from the fixed-order sum of its item
+// partials; integer results with error below 1/2 rounded (round_integral).
+void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial& r,
+                      perm_result& o);
+// An integer matrix whose FP64 result could be off by 1/2 or more while
+// below 2^53: run again with double-double products (mode 1).
+bool needs_dd_pass(const perm_result& o);
+
+// Split of an n-matrix into 64-lane batch items of about 2^item_log2
+// subsets (walk_bits + high_bits = n - 1); depends on n and item_log2 only.
+void batch_split(int n, int item_log2, int& walk_bits, int& high_bits);
+
+}  // namespace perm

@@ -1,0 +1,664 @@
+// perman_serve.cpp -- a permanent server on one GPU: continuous batching of
+// requests of mixed n (C ABI in include/perman_hip.h, perm_server_*).
+//
+// Callers submit matrices from any thread and wait on (or poll) a ticket.
+// A scheduler thread owns the GPU work:
+//
+//  * admission: a request gets a device slot for its column store and start
+//    vector (a fixed arena, so admission is bounded by device memory) while
+//    the queued work stays under a cap;
+//  * iterations: each iteration takes 64-lane items (the batch kernels'
+//    items, ryser_batch.hip) from the admitted requests, shortest remaining
+//    work first, up to a work budget of a few milliseconds, and launches one
+//    persistent kernel per (n, nu) variant present over several streams;
+//    up to `inflight` iterations are queued on the GPU at once, so the next
+//    one is ready when the previous drains;
+//  * completion: an iteration's item partials come back to pinned memory
+//    and are stored per request by item index; a request whose items are
+//    all done is summed on the host in item order (double-double) -- so its
+//    result depends only on the matrix, bitwise the same alone or in any mix
+//    and whatever iterations its items ran in;
+//  * integer matrices whose FP64 error bound reaches 1/2 get a
+//    double-double product pass through the single-matrix path on a helper
+//    thread, so the scheduler never blocks on it.
+//
+// Iterations change every time, so they are launched directly (no graph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/perman_hip.h"
+#include "perman_internal.hpp"
+#include "ryser_batch.hpp"
+#include "ryser_host.hpp"
+#include "ryser_kernels.hpp"
+
+namespace {
+
+using perm::BatchItem;
+using perm::BatchMatrix;
+using perm::Partial;
+using Clock = std::chrono::steady_clock;
+
+constexpr int kMaxStreams = 8;
+constexpr int kMaxInflight = 4;
+// FP64 VALU lane-instructions per second of one MI355X at full issue
+// (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz): converts the iteration time
+// target into a work budget.
+constexpr double kLaneInstrPerSec = 256.0 * 4 * 16 * 2.4e9;
+// Slot of one resident matrix: column store (2(n-1)+1) n and x0 (n), for
+// n <= kMaxBatchN, rounded up to 256 bytes.
+constexpr size_t kSlotDoubles =
+    ((size_t)(2 * (perm::kMaxBatchN - 1) + 1) * perm::kMaxBatchN + perm::kMaxBatchN + 31) / 32 *
+    32;
+
+struct Request {
+  uint64_t id = 0;
+  int n = 0, nu = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  bool integral = false;
+  perm::Prepared prep;
+  int slot = -1;
+  uint32_t items = 0, issued = 0, done = 0;
+  double item_cost = 0.0;  // lane-instructions per item (scheduling only)
+  std::vector<Partial> parts;
+  Clock::time_point t_submit, t_done;
+  perm_result result{};
+  int status = PERM_OK;
+  std::string error;
+  bool finished = false;
+  double remaining_cost() const { return (double)(items - issued) * item_cost; }
+};
+
+struct Segment {
+  Request* r;
+  uint32_t first, count, pos;  // items [first, first+count) of r at iteration position pos
+};
+
+struct Iteration {
+  BatchMatrix* h_mats = nullptr;
+  BatchItem* h_items = nullptr;
+  Partial* h_parts = nullptr;
+  BatchMatrix* d_mats = nullptr;
+  BatchItem* d_items = nullptr;
+  Partial* d_parts = nullptr;
+  uint32_t* d_counters = nullptr;
+  void* dmem = nullptr;
+  void* hmem = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<Segment> segs;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct perm_server {
+  perm_server_config cfg{};
+  int device = 0;
+  size_t max_items = 0;
+  double iter_cost = 0.0;
+  std::thread worker, helper;
+  std::mutex mu;
+  std::condition_variable cv_submit, cv_done, cv_helper;
+  std::deque<Request*> pending;                  // submitted, not admitted (mu)
+  std::deque<Request*> dd_queue;                 // waiting for a double-double pass (mu)
+  std::unordered_map<uint64_t, Request*> table;  // every uncollected request (mu)
+  uint64_t next_id = 1;
+  bool stop = false;
+  bool worker_done = false;  // the scheduler has returned (mu)
+  int fatal = PERM_OK;  // scheduler error (mu): every later call fails with it
+  std::string fatal_msg;
+  perm_server_stats stats{};  // (mu)
+  // scheduler-owned
+  std::vector<Request*> active;
+  std::vector<int> free_slots;
+  double queued_cost = 0.0;
+  double* d_arena = nullptr;
+  double* h_arena = nullptr;  // pinned mirror of the arena (upload staging)
+  Iteration it[kMaxInflight];
+  int head = 0, n_busy = 0;  // oldest busy iteration, number busy
+  hipStream_t s0 = nullptr, s[kMaxStreams] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams] = {};
+};
+
+namespace {
+
+#define HIP_TRYS(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return perm::fail(PERM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+struct HDD {
+  double hi, lo;
+};
+HDD two_sum(double a, double b) {
+  const double s = a + b;
+  const double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+HDD quick_two_sum(double a, double b) {
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+HDD dd_add(HDD a, HDD b) {
+  HDD s = two_sum(a.hi, b.hi);
+  const HDD t = two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = quick_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+
+void fail_request(perm_server* S, Request* r, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->status = code;
+  r->error = msg;
+  r->finished = true;
+  r->t_done = Clock::now();
+  S->cv_done.notify_all();
+}
+
+void publish(perm_server* S, Request* r) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->finished = true;
+  r->t_done = Clock::now();
+  S->stats.completed += 1;
+  S->cv_done.notify_all();
+}
+
+// All items of r are done: sum its partials in item order, state the
+// error, hand integer matrices that need it to the double-double helper.
+void finalize(perm_server* S, Request* r) {
+  HDD s = {0.0, 0.0};
+  Partial tot{0.0, 0.0, 0.0, 0.0};
+  for (const Partial& p : r->parts) {
+    s = dd_add(s, HDD{p.hi, p.lo});
+    tot.abs_sum += p.abs_sum;
+    tot.sq_sum += p.sq_sum;
+  }
+  tot.hi = s.hi;
+  tot.lo = s.lo;
+  perm::fill_item_result(r->n, r->walk_bits, r->integral, r->nu, tot, r->result);
+  r->parts.clear();
+  r->parts.shrink_to_fit();
+  S->free_slots.push_back(r->slot);
+  r->slot = -1;
+  if (perm::needs_dd_pass(r->result)) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->dd_queue.push_back(r);
+    S->cv_helper.notify_one();
+    return;
+  }
+  r->prep = perm::Prepared();
+  publish(S, r);
+}
+
+// Copy newly admitted requests into their arena slots (pinned mirror, then
+// one asynchronous copy each on s0, ahead of any kernel that reads them).
+int upload(perm_server* S, Request* r) {
+  double* h = S->h_arena + (size_t)r->slot * kSlotDoubles;
+  std::memcpy(h, r->prep.cs.data(), r->prep.cs.size() * sizeof(double));
+  std::memcpy(h + r->prep.cs.size(), r->prep.x0.data(), r->prep.x0.size() * sizeof(double));
+  const size_t bytes = (r->prep.cs.size() + r->prep.x0.size()) * sizeof(double);
+  HIP_TRYS(hipMemcpyAsync(S->d_arena + (size_t)r->slot * kSlotDoubles, h, bytes,
+                          hipMemcpyHostToDevice, S->s0));
+  return PERM_OK;
+}
+
+int admit(perm_server* S) {
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (S->pending.empty() || S->free_slots.empty()) return PERM_OK;
+      Request* c = S->pending.front();
+      const double cost = (double)c->items * c->item_cost;
+      if (!S->active.empty() && S->queued_cost + cost > S->cfg.max_queued_work) return PERM_OK;
+      S->pending.pop_front();
+      r = c;
+      S->stats.admitted += 1;
+    }
+    r->slot = S->free_slots.back();
+    S->free_slots.pop_back();
+    r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0});
+    S->queued_cost += (double)r->items * r->item_cost;
+    if (int st = upload(S, r)) return st;
+    S->active.push_back(r);
+  }
+}
+
+// Choose the next iteration's items and launch it on a free context.
+int issue(perm_server* S, bool& issued) {
+  issued = false;
+  if (S->n_busy >= S->cfg.inflight) return PERM_OK;
+  std::vector<Request*> cand;
+  for (Request* r : S->active)
+    if (r->issued < r->items) cand.push_back(r);
+  if (cand.empty()) return PERM_OK;
+  std::sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
+    const double ca = a->remaining_cost(), cb = b->remaining_cost();
+    return ca != cb ? ca < cb : a->id < b->id;
+  });
+  Iteration& I = S->it[(S->head + S->n_busy) % S->cfg.inflight];
+  I.segs.clear();
+  double budget = S->iter_cost;
+  size_t cap = S->max_items;
+  for (Request* r : cand) {
+    if (budget <= 0.0 || cap == 0) break;
+    uint64_t take = r->items - r->issued;
+    const uint64_t by_budget = (uint64_t)std::ceil(budget / r->item_cost);
+    take = std::min<uint64_t>(take, std::max<uint64_t>(1, by_budget));
+    take = std::min<uint64_t>(take, cap);
+    I.segs.push_back({r, r->issued, (uint32_t)take, 0});
+    r->issued += (uint32_t)take;
+    budget -= (double)take * r->item_cost;
+    cap -= take;
+  }
+  // group by variant: segments of one variant are contiguous in the item list
+  std::stable_sort(I.segs.begin(), I.segs.end(), [](const Segment& a, const Segment& b) {
+    return a.r->variant < b.r->variant;
+  });
+  struct Group {
+    int vid;
+    uint32_t begin, end;
+  };
+  std::vector<Group> groups;
+  uint32_t pos = 0;
+  for (size_t m = 0; m < I.segs.size(); ++m) {
+    Segment& g = I.segs[m];
+    Request* r = g.r;
+    g.pos = pos;
+    I.h_mats[m] = {S->d_arena + (size_t)r->slot * kSlotDoubles,
+                   S->d_arena + (size_t)r->slot * kSlotDoubles + r->prep.cs.size(),
+                   1ull << r->high_bits, r->walk_bits, r->high_bits, r->variant, 0u, 0u, 0u};
+    for (uint32_t i = 0; i < g.count; ++i)
+      I.h_items[pos + i] = {(uint32_t)m, 0u, (uint64_t)(g.first + i) * perm::kItemLanes};
+    if (groups.empty() || groups.back().vid != r->variant)
+      groups.push_back({r->variant, pos, pos + g.count});
+    else
+      groups.back().end = pos + g.count;
+    pos += g.count;
+  }
+  const uint32_t n_items = pos;
+  hipStream_t s0 = S->s0;
+  HIP_TRYS(hipMemcpyAsync(I.d_mats, I.h_mats, I.segs.size() * sizeof(BatchMatrix),
+                          hipMemcpyHostToDevice, s0));
+  HIP_TRYS(hipMemcpyAsync(I.d_items, I.h_items, n_items * sizeof(BatchItem),
+                          hipMemcpyHostToDevice, s0));
+  HIP_TRYS(perm::launch_batch_reset(I.d_counters, (uint32_t)groups.size(), s0));
+  const int ns = std::max(1, std::min(S->cfg.streams, kMaxStreams));
+  HIP_TRYS(hipEventRecord(S->ev_fork, s0));
+  for (int i = 0; i < ns; ++i) HIP_TRYS(hipStreamWaitEvent(S->s[i], S->ev_fork, 0));
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& G = groups[gi];
+    const uint32_t waves = G.end - G.begin;
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, 2048u));
+    HIP_TRYS(perm::launch_batch_variant(G.vid, I.d_mats, I.d_items, G.begin, G.end,
+                                        I.d_counters + gi, I.d_parts, grid, S->s[gi % ns]));
+  }
+  for (int i = 0; i < ns; ++i) {
+    HIP_TRYS(hipEventRecord(S->ev_join[i], S->s[i]));
+    HIP_TRYS(hipStreamWaitEvent(s0, S->ev_join[i], 0));
+  }
+  HIP_TRYS(hipMemcpyAsync(I.h_parts, I.d_parts, n_items * sizeof(Partial),
+                          hipMemcpyDeviceToHost, s0));
+  HIP_TRYS(hipEventRecord(I.done, s0));
+  I.busy = true;
+  S->n_busy += 1;
+  issued = true;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->stats.iterations += 1;
+  S->stats.items += n_items;
+  return PERM_OK;
+}
+
+// Retire the oldest iteration if it is done (wait = block until it is).
+int retire(perm_server* S, bool wait, bool& retired) {
+  retired = false;
+  if (S->n_busy == 0) return PERM_OK;
+  Iteration& I = S->it[S->head];
+  if (wait) {
+    HIP_TRYS(hipEventSynchronize(I.done));
+  } else {
+    const hipError_t q = hipEventQuery(I.done);
+    if (q == hipErrorNotReady) return PERM_OK;
+    if (q != hipSuccess)
+      return perm::fail(PERM_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+  }
+  for (const Segment& g : I.segs) {
+    Request* r = g.r;
+    std::copy(I.h_parts + g.pos, I.h_parts + g.pos + g.count, r->parts.begin() + g.first);
+    r->done += g.count;
+    S->queued_cost -= (double)g.count * r->item_cost;
+    if (r->done == r->items) {
+      S->active.erase(std::find(S->active.begin(), S->active.end(), r));
+      finalize(S, r);
+    }
+  }
+  I.segs.clear();
+  I.busy = false;
+  S->head = (S->head + 1) % S->cfg.inflight;
+  S->n_busy -= 1;
+  retired = true;
+  return PERM_OK;
+}
+
+int scheduler_step(perm_server* S, bool& idle) {
+  idle = false;
+  if (int st = admit(S)) return st;
+  bool issued = false, retired = false;
+  if (int st = issue(S, issued)) return st;
+  if (int st = retire(S, false, retired)) return st;
+  if (issued || retired) return PERM_OK;
+  if (S->n_busy >= S->cfg.inflight) return retire(S, true, retired);  // GPU queue full
+  if (S->n_busy > 0) {
+    // nothing to issue: wait briefly for the GPU or a new request
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_submit.wait_for(lk, std::chrono::microseconds(100),
+                          [&] { return S->stop || !S->pending.empty(); });
+    return PERM_OK;
+  }
+  idle = true;
+  return PERM_OK;
+}
+
+void scheduler_loop(perm_server* S) {
+  for (;;) {
+    bool idle = false;
+    const int st = scheduler_step(S, idle);
+    if (st != PERM_OK) {
+      // a HIP failure leaves the device state unknown: fail everything
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->fatal = st;
+      S->fatal_msg = perm::last_error();
+      for (auto& kv : S->table)
+        if (!kv.second->finished) {
+          kv.second->status = st;
+          kv.second->error = S->fatal_msg;
+          kv.second->finished = true;
+        }
+      S->dd_queue.clear();
+      S->cv_done.notify_all();
+      return;
+    }
+    if (idle) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (S->stop && S->pending.empty() && S->active.empty()) return;
+      S->cv_submit.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+    }
+  }
+}
+
+void scheduler_main(perm_server* S) {
+  perm::set_device(S->device);
+  (void)hipSetDevice(S->device);
+  scheduler_loop(S);
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->worker_done = true;
+  S->cv_helper.notify_all();
+}
+
+// Double-double product passes of integer matrices (single-matrix path).
+void helper_main(perm_server* S) {
+  perm::set_device(S->device);
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      // runs until the scheduler has returned and the queue is empty
+      S->cv_helper.wait(lk, [&] { return !S->dd_queue.empty() || S->worker_done; });
+      if (S->dd_queue.empty()) return;
+      r = S->dd_queue.front();
+      S->dd_queue.pop_front();
+    }
+    perm_result out;
+    std::memset(&out, 0, sizeof(out));
+    const int st = perm::run_prepared(r->n, r->prep, 1, &out);
+    if (st != PERM_OK) {
+      fail_request(S, r, st, perm::last_error());
+      continue;
+    }
+    r->result = out;
+    r->prep = perm::Prepared();
+    publish(S, r);
+  }
+}
+
+int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
+  auto* r = new Request();
+  r->n = n;
+  r->integral = p.integral;
+  perm::batch_split(n, S->cfg.item_log2, r->walk_bits, r->high_bits);
+  const uint64_t lanes = 1ull << r->high_bits;
+  r->items = (uint32_t)((lanes + perm::kItemLanes - 1) / perm::kItemLanes);
+  // pair kernels for requests with real work; tiny ones use the plain
+  // variant (fewer kernels per iteration)
+  r->nu = r->items >= 8 ? perm::batch_nu(n, p.nu) : n;
+  r->variant = perm::batch_variant(n, r->nu);
+  if (r->variant < 0) {
+    delete r;
+    return perm::fail(PERM_ERR_ARG, "no batch kernel for this n");
+  }
+  r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
+                 (double)(r->nu < n ? n + r->nu + 2 : 2 * n + 2);
+  r->prep = std::move(p);
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->fatal != PERM_OK) {
+      delete r;
+      return perm::fail(S->fatal, "server failed: " + S->fatal_msg);
+    }
+    if (S->stop) {
+      delete r;
+      return perm::fail(PERM_ERR_ARG, "server is shutting down");
+    }
+    r->id = S->next_id++;
+    S->table[r->id] = r;
+    S->pending.push_back(r);
+    S->stats.submitted += 1;
+  }
+  S->cv_submit.notify_one();
+  *id = r->id;
+  return perm::ok();
+}
+
+// Collect a finished request (mu held): copy out, erase, free.
+int collect(perm_server* S, std::unordered_map<uint64_t, Request*>::iterator it,
+            perm_result* out, double* latency_ms) {
+  Request* r = it->second;
+  const int st = r->status;
+  if (out) *out = r->result;
+  const double lat = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  if (latency_ms) *latency_ms = lat;
+  if (out && st == PERM_OK) out->kernel_ms = lat;
+  const std::string err = r->error;
+  S->table.erase(it);
+  delete r;
+  if (st != PERM_OK) return perm::fail(st, err);
+  return perm::ok();
+}
+
+void release(perm_server* S) {
+  for (auto& I : S->it) {
+    if (I.dmem) (void)hipFree(I.dmem);
+    if (I.hmem) (void)hipHostFree(I.hmem);
+    if (I.done) (void)hipEventDestroy(I.done);
+  }
+  if (S->d_arena) (void)hipFree(S->d_arena);
+  if (S->h_arena) (void)hipHostFree(S->h_arena);
+  for (int i = 0; i < kMaxStreams; ++i) {
+    if (S->s[i]) (void)hipStreamDestroy(S->s[i]);
+    if (S->ev_join[i]) (void)hipEventDestroy(S->ev_join[i]);
+  }
+  if (S->ev_fork) (void)hipEventDestroy(S->ev_fork);
+  if (S->s0) (void)hipStreamDestroy(S->s0);
+  for (auto& kv : S->table) delete kv.second;
+  delete S;
+}
+
+int setup(perm_server* S) {
+  perm_server_config& c = S->cfg;
+  const size_t slots = (size_t)c.max_resident;
+  HIP_TRYS(hipMalloc((void**)&S->d_arena, slots * kSlotDoubles * sizeof(double)));
+  HIP_TRYS(hipHostMalloc((void**)&S->h_arena, slots * kSlotDoubles * sizeof(double),
+                         hipHostMallocDefault));
+  S->free_slots.resize(slots);
+  for (size_t i = 0; i < slots; ++i) S->free_slots[i] = (int)(slots - 1 - i);
+  HIP_TRYS(hipStreamCreateWithFlags(&S->s0, hipStreamNonBlocking));
+  for (int i = 0; i < kMaxStreams; ++i) {
+    HIP_TRYS(hipStreamCreateWithFlags(&S->s[i], hipStreamNonBlocking));
+    HIP_TRYS(hipEventCreateWithFlags(&S->ev_join[i], hipEventDisableTiming));
+  }
+  HIP_TRYS(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
+  const size_t mi = S->max_items;
+  const size_t mats_b = align256(mi * sizeof(BatchMatrix));
+  const size_t items_b = align256(mi * sizeof(BatchItem));
+  const size_t parts_b = align256(mi * sizeof(Partial));
+  const size_t ctr_b = align256(mi * sizeof(uint32_t));
+  for (int k = 0; k < c.inflight; ++k) {
+    Iteration& I = S->it[k];
+    HIP_TRYS(hipMalloc(&I.dmem, mats_b + items_b + parts_b + ctr_b));
+    HIP_TRYS(hipHostMalloc(&I.hmem, mats_b + items_b + parts_b, hipHostMallocDefault));
+    char* d = (char*)I.dmem;
+    char* h = (char*)I.hmem;
+    I.d_mats = (BatchMatrix*)d;
+    I.d_items = (BatchItem*)(d + mats_b);
+    I.d_parts = (Partial*)(d + mats_b + items_b);
+    I.d_counters = (uint32_t*)(d + mats_b + items_b + parts_b);
+    I.h_mats = (BatchMatrix*)h;
+    I.h_items = (BatchItem*)(h + mats_b);
+    I.h_parts = (Partial*)(h + mats_b + items_b);
+    HIP_TRYS(hipEventCreateWithFlags(&I.done, hipEventDisableTiming));
+  }
+  return PERM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void perm_server_default_config(perm_server_config* c) {
+  if (!c) return;
+  c->item_log2 = 18;
+  c->streams = 4;
+  c->inflight = 2;
+  c->max_items = 32768;
+  c->iteration_us = 4000.0;
+  c->max_resident = 2048;
+  c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)
+}
+
+int perm_server_create(const perm_server_config* cfg, perm_server** out) {
+  if (!out) return perm::fail(PERM_ERR_ARG, "null pointer");
+  perm_server_config c;
+  perm_server_default_config(&c);
+  if (cfg) {
+    if (cfg->item_log2 > 0) c.item_log2 = cfg->item_log2;
+    if (cfg->streams > 0) c.streams = cfg->streams;
+    if (cfg->inflight > 0) c.inflight = cfg->inflight;
+    if (cfg->max_items > 0) c.max_items = cfg->max_items;
+    if (cfg->iteration_us > 0) c.iteration_us = cfg->iteration_us;
+    if (cfg->max_resident > 0) c.max_resident = cfg->max_resident;
+    if (cfg->max_queued_work > 0) c.max_queued_work = cfg->max_queued_work;
+  }
+  if (c.item_log2 < 6 || c.item_log2 > 30) return perm::fail(PERM_ERR_ARG, "item_log2 in [6, 30]");
+  if (c.streams > kMaxStreams) c.streams = kMaxStreams;
+  if (c.inflight > kMaxInflight) c.inflight = kMaxInflight;
+  if (int s = perm::select_device()) return s;
+  auto* S = new perm_server();
+  S->cfg = c;
+  S->device = perm::current_device();
+  S->max_items = (size_t)c.max_items;
+  S->iter_cost = c.iteration_us * 1e-6 * kLaneInstrPerSec;
+  if (int st = setup(S)) {
+    const std::string msg = perm::last_error();
+    release(S);
+    return perm::fail(st, msg);
+  }
+  S->worker = std::thread(scheduler_main, S);
+  S->helper = std::thread(helper_main, S);
+  *out = S;
+  return perm::ok();
+}
+
+int perm_server_submit_dense(perm_server* S, int dtype, const void* mat, int nov,
+                             unsigned long long* id) {
+  if (!S || !id) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (nov > perm::kMaxBatchN)
+    return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
+                                         std::to_string(perm::kMaxBatchN));
+  perm::Prepared p;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
+  return submit(S, nov, p, id);
+}
+
+int perm_server_submit_sparse(perm_server* S, int dtype, const void* mat, const int* cptrs,
+                              const int* rows, const void* cvals, int nov,
+                              unsigned long long* id) {
+  if (!S || !id) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (nov > perm::kMaxBatchN)
+    return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
+                                         std::to_string(perm::kMaxBatchN));
+  perm::Prepared p;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  return submit(S, nov, p, id);
+}
+
+int perm_server_wait(perm_server* S, unsigned long long id, perm_result* out,
+                     double* latency_ms) {
+  if (!S) return perm::fail(PERM_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->table.find(id);
+  if (it == S->table.end()) return perm::fail(PERM_ERR_ARG, "unknown request id");
+  Request* r = it->second;
+  S->cv_done.wait(lk, [&] { return r->finished; });
+  return collect(S, S->table.find(id), out, latency_ms);
+}
+
+int perm_server_poll(perm_server* S, unsigned long long id, perm_result* out,
+                     double* latency_ms) {
+  if (!S) return perm::fail(PERM_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->table.find(id);
+  if (it == S->table.end()) return perm::fail(PERM_ERR_ARG, "unknown request id");
+  if (!it->second->finished) return PERM_PENDING;
+  return collect(S, it, out, latency_ms);
+}
+
+int perm_server_stats_get(perm_server* S, perm_server_stats* out) {
+  if (!S || !out) return perm::fail(PERM_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->stats;
+  out->pending = (long long)S->pending.size();
+  return perm::ok();
+}
+
+void perm_server_destroy(perm_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_submit.notify_all();
+  if (S->worker.joinable()) S->worker.join();  // drains admitted and pending work
+  if (S->helper.joinable()) S->helper.join();  // then the double-double passes
+  (void)hipSetDevice(S->device);
+  if (S->s0) (void)hipStreamSynchronize(S->s0);
+  release(S);
+}
+
+}  // extern "C"

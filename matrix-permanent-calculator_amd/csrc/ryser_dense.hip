@@ -5,11 +5,11 @@
 
 namespace perm {
 
-hipError_t launch_dense_range_a(const DenseLaunch& L, hipStream_t s);
-hipError_t launch_dense_range_b(const DenseLaunch& L, hipStream_t s);
-hipError_t launch_dense_range_c(const DenseLaunch& L, hipStream_t s);
-hipError_t launch_dense_range_d(const DenseLaunch& L, hipStream_t s);
-hipError_t launch_dense_range_e(const DenseLaunch& L, hipStream_t s);
+hipError_t launch_dense_range_a(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_b(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_c(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_d(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_e(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
 
 // Grid-level reduction of `count` partials into out[0] (one workgroup of
 // 256 threads, fixed order).
@@ -18,26 +18,28 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const Partial* __r
                                                               Partial* __restrict__ out) {
   __shared__ Partial scratch[256 / kWave];
   DD s = {0.0, 0.0};
-  double a = 0.0;
+  double a = 0.0, q = 0.0;
   for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
     s = dd_add(s, DD{in[i].hi, in[i].lo});
     a += in[i].abs_sum;
+    q += in[i].sq_sum;
   }
-  block_reduce_store(s, a, scratch, out);
+  block_reduce_store(s, a, q, scratch, out);
 }
 
 int dense_unroll_bits(int n) { return n - 1 < 3 ? n - 1 : 3; }
 
-hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream) {
+hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   if (L.n < 1 || L.n > kMaxDenseN) return hipErrorInvalidValue;
   hipError_t e;
-  if (L.n <= 20) e = launch_dense_range_a(L, stream);
-  else if (L.n <= 30) e = launch_dense_range_b(L, stream);
-  else if (L.n <= 36) e = launch_dense_range_c(L, stream);
-  else if (L.n <= 42) e = launch_dense_range_d(L, stream);
-  else e = launch_dense_range_e(L, stream);
+  if (L.n <= 20) e = launch_dense_range_a(L, stream, main_done);
+  else if (L.n <= 30) e = launch_dense_range_b(L, stream, main_done);
+  else if (L.n <= 36) e = launch_dense_range_c(L, stream, main_done);
+  else if (L.n <= 42) e = launch_dense_range_d(L, stream, main_done);
+  else e = launch_dense_range_e(L, stream, main_done);
   if (e != hipSuccess) return e;
-  reduce_partials_kernel<<<1, 256, 0, stream>>>(L.partials, L.blocks, L.result);
+  if (L.mode == 0 && L.ext_k >= 0 && (e = launch_extreme(L, stream)) != hipSuccess) return e;
+  reduce_partials_kernel<<<1, 256, 0, stream>>>(L.partials, L.total_blocks, L.result);
   return hipGetLastError();
 }
 

@@ -82,6 +82,20 @@ struct LaneAcc {
   double p = 0.0, plo = 0.0, a = 0.0;
 };
 
+// Extreme-subset cut.  The few subsets S with |S| <= k or |S| >= n-1-k have
+// by far the largest terms (x_j near +-rowsum_j/2); their FP64 rounding and
+// X drift dominate the error of the whole sum.  The FP64 walk drops them
+// and ryser_extreme.hip sums them separately in double-double with exact X.
+// A lane's subset is (high pattern = lane id, walk pattern = gray(t)), so
+// |S| = pl + popcount(gray(t)) with pl = popcount(high pattern).  Per wave,
+// lo_thr = k - min(pl) and hi_thr = n-1-k - max(pl) bound the walk patterns
+// for which some lane may be extreme; waves that cannot meet one get
+// thresholds outside [0, w] and never take the per-lane test.
+struct ExtCut {
+  int lo_thr, hi_thr;  // scalar: test lanes when pw <= lo_thr or pw >= hi_thr
+  int lo, hi;          // per lane: drop when pw <= lo or pw >= hi (k - pl, n-1-k - pl)
+};
+
 // Product of the factors {seed if SEED} + x[LO..HI) as four interleaved
 // chains (independent work for the scheduler): F-1 multiplies for F factors.
 template <int N, int LO, int HI, bool SEED>
@@ -175,6 +189,19 @@ __device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], cons
   }
 }
 
+// Zero the term q of step t = g*2^R + V when this lane's subset is extreme.
+// popcount(gray(t)) is wave-uniform (scalar); only when some lane of the
+// wave can be extreme at this step does the per-lane test run.
+template <int R, int V>
+__device__ __forceinline__ double drop_extreme(double q, uint32_t g, const ExtCut& ec) {
+  const uint32_t t = (g << R) | (uint32_t)V;
+  const int pw = __builtin_popcount(t ^ (t >> 1));
+  if (pw <= ec.lo_thr || pw >= ec.hi_thr) {
+    if (pw <= ec.lo || pw >= ec.hi) q = 0.0;
+  }
+  return q;
+}
+
 // Step v of group g (step index t = g*2^R + v).  Phase A adds the column
 // that the previous step prefetched into SGPRs; phase B issues the scalar
 // loads of the next step's column and, while they are in flight, forms the
@@ -182,10 +209,11 @@ __device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], cons
 // the loads always overlap a product.  With the pair variant (NU < N) odd
 // steps touch only x[0..NU) and reuse the passive product `ppass` formed on
 // the even step before them.
-template <int N, int R, int MODE, int NU, int V>
+template <int N, int R, int MODE, int NU, bool CHK, int V>
 __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], double (&cv)[N],
                                             ColPtr cols, uint32_t g, uint32_t groups,
-                                            int zero_blk, LaneAcc& acc, double& ppass) {
+                                            int zero_blk, LaneAcc& acc, double& ppass,
+                                            const ExtCut& ec) {
   constexpr bool kPair = NU < N;
   if constexpr (V < (1 << R)) {
     if constexpr (kPair && (V & 1)) add_column<N, MODE, NU>(x, xl, cv);
@@ -213,27 +241,84 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
     else load_col<N, N>(cv, cols, blk);
     if constexpr (kPair) {
       if constexpr ((V & 1) == 0) ppass = chain_prod<N, NU, N, false>(x, 0.0);
-      add_term<(V & 1) != 0>(chain_prod<N, 0, NU - 1, true>(x, ppass), x[NU - 1], acc);
+      double q = chain_prod<N, 0, NU - 1, true>(x, ppass);
+      if constexpr (CHK) q = drop_extreme<R, V>(q, g, ec);
+      add_term<(V & 1) != 0>(q, x[NU - 1], acc);
+    } else if constexpr (CHK && MODE == kFast && N > 1) {
+      const double q = drop_extreme<R, V>(chain_prod<N, 0, N - 1, false>(x, 0.0), g, ec);
+      add_term<(V & 1) != 0>(q, x[N - 1], acc);
     } else {
       accumulate_term<N, (V & 1) != 0, MODE>(x, xl, acc);
     }
     __builtin_amdgcn_sched_barrier(0);
-    group_steps<N, R, MODE, NU, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+    group_steps<N, R, MODE, NU, CHK, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass,
+                                            ec);
   }
 }
 
+// The Gray walk of one lane over its groups.  Every 32 groups (a run of
+// 2^(R+5) terms) the FP64 lane sum is folded into the double-double s and
+// the square of the run's sum of |terms| is added to q.
+template <int N, int R, int MODE, int NU, bool CHK>
+__device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], ColPtr cols,
+                                            uint32_t groups, int zero_blk, DD& s, LaneAcc& acc,
+                                            double& q, const ExtCut& ec) {
+  double ppass = 1.0;
+  double a_run = 0.0;  // acc.a at the start of the current run
+  double cv[N];
+  load_col<N, N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
+  for (uint32_t g = 0; g < groups; ++g) {
+    group_steps<N, R, MODE, NU, CHK, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass, ec);
+    if ((g & 31u) == 31u) {
+      if constexpr (MODE == kFast) {  // fold into double-double
+        s = dd_add_d(s, acc.p);
+        acc.p = 0.0;
+      }
+      const double d = acc.a - a_run;
+      q = __builtin_fma(d, d, q);
+      a_run = acc.a;
+    }
+  }
+  const double d = acc.a - a_run;
+  q = __builtin_fma(d, d, q);
+}
+
+// Wave-wide min and max of v (result in every lane, scalar).
+__device__ __forceinline__ void wave_min_max(int v, int& mn, int& mx) {
+  int lo = v, hi = v;
+#define PERM_MM_STEP(M)                                                      \
+  {                                                                          \
+    constexpr int pat = (M << 10) | 0x1f;                                    \
+    lo = min(lo, __builtin_amdgcn_ds_swizzle(lo, pat));                      \
+    hi = max(hi, __builtin_amdgcn_ds_swizzle(hi, pat));                      \
+  }
+  PERM_MM_STEP(1)
+  PERM_MM_STEP(2)
+  PERM_MM_STEP(4)
+  PERM_MM_STEP(8)
+  PERM_MM_STEP(16)
+#undef PERM_MM_STEP
+  mn = min(__builtin_amdgcn_readlane(lo, 0), __builtin_amdgcn_readlane(lo, 32));
+  mx = max(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(hi, 32));
+}
+
+// The walk of one lane: closed-form start from its high columns, then 2^w
+// subsets of its low columns.  Returns the lane's signed double-double sum
+// (sign of the high pattern applied), its sum of |terms| and its sum of
+// squared run sums (see walk_groups).
 // colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
 // Gray columns k < n-1, block 2(n-1) = zeros.  With NU < N, column 0 is zero
-// in rows [NU, N).
-template <int N, int R, int MODE, int NU>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
-void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
-                        Partial* __restrict__ partials, uint64_t lanes, int n_walk_bits,
-                        int n_high_bits) {
+// in rows [NU, N).  CHK (FP64 mode): the subsets with |S| <= ext_k or
+// |S| >= n-1-ext_k are left out (summed elsewhere); a wave tests its lanes
+// only at steps where one of them can be extreme.  Every lane of the wave
+// must take part (wave-wide min/max).
+template <int N, int R, int MODE, int NU, bool CHK>
+__device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
+                                          const double* __restrict__ x0, uint64_t pattern,
+                                          int n_walk_bits, int n_high_bits, int ext_k, DD& s,
+                                          double& a, double& q) {
   static_assert(NU == N || (MODE == kFast && R >= 1 && NU >= 1), "pair variant: FP64 mode only");
-  __shared__ Partial scratch[256 / kWave];
-  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  static_assert(!CHK || MODE == kFast, "extreme-subset cut: FP64 mode only");
   const ColPtr cols = (ColPtr)colpm;
   const int zero_blk = 2 * (N - 1);
 
@@ -244,12 +329,12 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
     xl[j] = 0.0;
   }
 
-  // Closed-form start: add the high columns selected by the lane id
+  // Closed-form start: add the high columns selected by the pattern
   // (adding sel*c with sel in {0, 1} is exact selection, no branch).
   int parity = 0;
   for (int b = 0; b < n_high_bits; ++b) {
     ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
-    const int bit = (int)((lane >> b) & 1);
+    const int bit = (int)((pattern >> b) & 1);
     parity ^= bit;
     const double sel = bit ? 1.0 : 0.0;
     double cv[N];
@@ -258,33 +343,57 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
     add_column<N, MODE, N>(x, xl, cv);
   }
 
-  DD s = {0.0, 0.0};
+  s = {0.0, 0.0};
+  q = 0.0;
   LaneAcc acc;
-  double ppass = 1.0;
   const uint32_t groups = 1u << (n_walk_bits - R);
-  double cv[N];
-  load_col<N, N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
-  for (uint32_t g = 0; g < groups; ++g) {
-    group_steps<N, R, MODE, NU, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
-    if constexpr (MODE == kFast) {
-      if ((g & 31u) == 31u) {  // fold into double-double every 32 groups
-        s = dd_add_d(s, acc.p);
-        acc.p = 0.0;
-      }
-    }
+  ExtCut ec{-1, 1 << 30, -1, 1 << 30};
+  if constexpr (CHK) {
+    const int pl = __builtin_popcountll(pattern);
+    const int hi = n_walk_bits + n_high_bits - ext_k;
+    int pmin, pmax;
+    wave_min_max(pl, pmin, pmax);
+    ec = {ext_k - pmin, hi - pmax, ext_k - pl, hi - pl};
   }
+  walk_groups<N, R, MODE, NU, CHK>(x, xl, cols, groups, zero_blk, s, acc, q, ec);
   if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
   else s = two_sum(acc.p, acc.plo);
-  double a = acc.a;
+  a = acc.a;
   if (parity) {
     s.hi = -s.hi;
     s.lo = -s.lo;
   }
+}
+
+// Pattern of lane `lane` in a pass (see LanePass).
+__device__ __forceinline__ uint64_t lane_pattern(uint64_t lane, int n_high_bits, int free_bits,
+                                                 uint64_t class_mask) {
+  if (class_mask == ~0ull) return lane;
+  const uint64_t top = unrank_pattern(lane >> free_bits, n_high_bits - free_bits, class_mask);
+  return (top << free_bits) | (lane & ((1ull << free_bits) - 1ull));
+}
+
+// One pass of one permanent: lane = global thread id, one partial per
+// workgroup.
+template <int N, int R, int MODE, int NU, bool CHK>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
+void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
+                        Partial* __restrict__ partials, uint64_t lanes, int n_walk_bits,
+                        int n_high_bits, int free_bits, uint64_t class_mask, int ext_k) {
+  __shared__ Partial scratch[256 / kWave];
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t pattern =
+      lane < lanes ? lane_pattern(lane, n_high_bits, free_bits, class_mask) : 0ull;
+  DD s;
+  double a, q;
+  lane_walk<N, R, MODE, NU, CHK>(colpm, x0, pattern, n_walk_bits, n_high_bits, ext_k, s, a, q);
   if (lane >= lanes) {
     s = {0.0, 0.0};
     a = 0.0;
+    q = 0.0;
   }
-  block_reduce_store(s, a, scratch, partials + blockIdx.x);
+  block_reduce_store(s, a, q, scratch, partials + blockIdx.x);
 }
 
 template <int N>
@@ -292,32 +401,54 @@ constexpr int unroll_bits_for() {
   return N - 1 < 3 ? N - 1 : 3;
 }
 
-template <int N, int MODE, int NU>
-hipError_t launch_variant(const DenseLaunch& L, hipStream_t stream) {
+template <int N, int MODE, int NU, bool CHK>
+hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
-  ryser_dense_kernel<N, R, MODE, NU><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
-      L.colpm, L.x0, L.partials, L.lanes, L.walk_bits, L.high_bits);
+  if (P.lanes == 0) return hipSuccess;
+  if (P.walk_bits < R) return hipErrorInvalidValue;
+  ryser_dense_kernel<N, R, MODE, NU, CHK><<<dim3((unsigned)P.blocks), dim3(256), 0, stream>>>(
+      L.colpm, L.x0, P.partials, P.lanes, P.walk_bits, P.high_bits, P.free_bits, P.class_mask,
+      L.ext_k);
   return hipGetLastError();
+}
+
+// FP64 passes: the main lanes, then (extreme-subset cut) the side lanes.
+template <int N, int NU>
+hipError_t launch_fast(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  hipError_t e = launch_pass<N, kFast, NU, false>(L, L.main, stream);
+  if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
+  if (e != hipSuccess || L.ext_k < 0 || L.side.lanes == 0) return e;
+  if constexpr (N >= 2) return launch_pass<N, kFast, NU, true>(L, L.side, stream);
+  return hipErrorInvalidValue;
 }
 
 // Pair variants exist for NU = 4, 8, ..., N-4 when N >= kMinPairN.
 template <int N, int... Q>
-hipError_t launch_pair(const DenseLaunch& L, hipStream_t stream,
+hipError_t launch_pair(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
                        std::integer_sequence<int, Q...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((L.nu == 4 * (Q + 1) ? (e = launch_variant<N, kFast, 4 * (Q + 1)>(L, stream), 0) : 0), ...);
+  ((L.nu == 4 * (Q + 1) ? (e = launch_fast<N, 4 * (Q + 1)>(L, stream, main_done), 0) : 0), ...);
   return e;
 }
 
 template <int N>
-hipError_t launch_n(const DenseLaunch& L, hipStream_t stream) {
-  if (L.walk_bits < unroll_bits_for<N>()) return hipErrorInvalidValue;
-  if (L.mode == kDDProd) return launch_variant<N, kDDProd, N>(L, stream);
-  if (L.mode == kDDAll) return launch_variant<N, kDDAll, N>(L, stream);
-  if (L.nu == N || L.nu <= 0) return launch_variant<N, kFast, N>(L, stream);
+hipError_t launch_mode(const DenseLaunch& L, hipStream_t stream) {
+  if (L.mode == kDDProd) return launch_pass<N, kDDProd, N, false>(L, L.main, stream);
+  return launch_pass<N, kDDAll, N, false>(L, L.main, stream);
+}
+
+template <int N>
+hipError_t launch_n(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  if (L.mode == kDDProd || L.mode == kDDAll) {
+    hipError_t e = launch_mode<N>(L, stream);
+    if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
+    return e;
+  }
+  if (L.nu == N || L.nu <= 0) return launch_fast<N, N>(L, stream, main_done);
   if constexpr (N >= kMinPairN) {
     if (L.nu % 4 == 0 && L.nu < N)
-      return launch_pair<N>(L, stream, std::make_integer_sequence<int, (N - 1) / 4>{});
+      return launch_pair<N>(L, stream, main_done,
+                            std::make_integer_sequence<int, (N - 1) / 4>{});
   }
   return hipErrorInvalidValue;
 }
@@ -329,13 +460,13 @@ hipError_t launch_n(const DenseLaunch& L, hipStream_t stream) {
 #define PERM_DENSE_RANGE(TAG, LO, HI)                                                         \
   namespace perm {                                                                            \
   template <int... I>                                                                         \
-  static hipError_t dispatch_##TAG(const DenseLaunch& L, hipStream_t s,                       \
+  static hipError_t dispatch_##TAG(const DenseLaunch& L, hipStream_t s, hipEvent_t ev,        \
                                    std::integer_sequence<int, I...>) {                        \
     hipError_t e = hipErrorInvalidValue;                                                      \
-    ((L.n == LO + I ? (e = launch_n<LO + I>(L, s), 0) : 0), ...);                             \
+    ((L.n == LO + I ? (e = launch_n<LO + I>(L, s, ev), 0) : 0), ...);                         \
     return e;                                                                                 \
   }                                                                                           \
-  hipError_t launch_dense_range_##TAG(const DenseLaunch& L, hipStream_t s) {                  \
-    return dispatch_##TAG(L, s, std::make_integer_sequence<int, HI - LO + 1>{});              \
+  hipError_t launch_dense_range_##TAG(const DenseLaunch& L, hipStream_t s, hipEvent_t ev) {   \
+    return dispatch_##TAG(L, s, ev, std::make_integer_sequence<int, HI - LO + 1>{});          \
   }                                                                                           \
   }

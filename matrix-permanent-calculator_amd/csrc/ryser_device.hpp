@@ -15,14 +15,54 @@ namespace perm {
 
 constexpr int kWave = 64;
 
-// One workgroup's contribution: signed sum as hi+lo, and sum of |terms|.
+// One workgroup's contribution: signed sum as hi+lo, sum of |terms|, and
+// sum over runs of consecutive terms of (sum of |terms| in the run)^2 -- an
+// upper bound on sum term^2 that sizes the statistical error estimate.
 struct Partial {
-  double hi, lo, abs_sum, pad;
+  double hi, lo, abs_sum, sq_sum;
 };
 
 struct DD {
   double hi, lo;
 };
+
+// Binomial coefficients C(c, p), 0 <= c < kBinomRows, in constant memory
+// (one copy per code object).
+constexpr int kBinomRows = 48;
+struct BinomTable {
+  uint64_t v[kBinomRows][kBinomRows + 1];
+};
+constexpr BinomTable make_binom() {
+  BinomTable t{};
+  for (int c = 0; c < kBinomRows; ++c)
+    for (int p = 0; p <= kBinomRows; ++p)
+      t.v[c][p] = p == 0 ? 1 : (c == 0 ? 0 : t.v[c - 1][p - 1] + t.v[c - 1][p]);
+  return t;
+}
+static __constant__ BinomTable kBinom = make_binom();
+
+// The r-th h-bit pattern among those whose popcount c has bit c of
+// class_mask set, ordered by c, then colex (bit positions as a c-subset).
+// Requires r < sum of C(h, c) over the selected classes, h < kBinomRows.
+__device__ __forceinline__ uint64_t unrank_pattern(uint64_t r, int h, uint64_t class_mask) {
+  int c = 0;
+  for (; c < h; ++c) {
+    if ((class_mask >> c) & 1ull) {
+      const uint64_t cnt = kBinom.v[h][c];
+      if (r < cnt) break;
+      r -= cnt;
+    }
+  }
+  uint64_t pattern = 0;
+  int pos = h - 1;
+  for (int i = c; i >= 1; --i) {
+    while (kBinom.v[pos][i] > r) --pos;  // largest pos with C(pos, i) <= r
+    pattern |= 1ull << pos;
+    r -= kBinom.v[pos][i];
+    --pos;
+  }
+  return pattern;
+}
 
 __device__ __forceinline__ DD two_sum(double a, double b) {
   double s = a + b;
@@ -69,13 +109,15 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// Wave-wide dd sum (and plain sum of `a`); the full result is valid in lane 0.
-__device__ __forceinline__ void wave_reduce(DD& s, double& a) {
+// Wave-wide dd sum (and plain sums of `a` and `q`); the full result is
+// valid in lane 0.
+__device__ __forceinline__ void wave_reduce(DD& s, double& a, double& q) {
 #define PERM_XOR_STEP(M)                                 \
   {                                                      \
     DD o = {swizzle_xor<M>(s.hi), swizzle_xor<M>(s.lo)}; \
     s = dd_add(s, o);                                    \
     a += swizzle_xor<M>(a);                              \
+    q += swizzle_xor<M>(q);                              \
   }
   PERM_XOR_STEP(1)
   PERM_XOR_STEP(2)
@@ -85,27 +127,30 @@ __device__ __forceinline__ void wave_reduce(DD& s, double& a) {
 #undef PERM_XOR_STEP
   DD up = {readlane_d(s.hi, 32), readlane_d(s.lo, 32)};
   double aup = readlane_d(a, 32);
+  double qup = readlane_d(q, 32);
   s = dd_add(s, up);
   a += aup;
+  q += qup;
 }
 
-// Workgroup reduction of (s, a) into one Partial written by thread 0.
+// Workgroup reduction of (s, a, q) into one Partial written by thread 0.
 // `scratch` must hold blockDim.x/64 entries.
-__device__ __forceinline__ void block_reduce_store(DD s, double a, Partial* scratch,
+__device__ __forceinline__ void block_reduce_store(DD s, double a, double q, Partial* scratch,
                                                    Partial* out) {
-  wave_reduce(s, a);
+  wave_reduce(s, a, q);
   const int wid = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) scratch[wid] = {s.hi, s.lo, a, 0.0};
+  if ((threadIdx.x & (kWave - 1)) == 0) scratch[wid] = {s.hi, s.lo, a, q};
   __syncthreads();
   if (threadIdx.x == 0) {
     DD t = {scratch[0].hi, scratch[0].lo};
-    double ta = scratch[0].abs_sum;
+    double ta = scratch[0].abs_sum, tq = scratch[0].sq_sum;
     for (int w = 1; w < nw; ++w) {
       t = dd_add(t, DD{scratch[w].hi, scratch[w].lo});
       ta += scratch[w].abs_sum;
+      tq += scratch[w].sq_sum;
     }
-    *out = {t.hi, t.lo, ta, 0.0};
+    *out = {t.hi, t.lo, ta, tq};
   }
 }
 

@@ -39,7 +39,8 @@ class PermResultC(ctypes.Structure):
                 ("err_bound", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
                 ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
-                ("nu", ctypes.c_int)]
+                ("nu", ctypes.c_int), ("ext_k", ctypes.c_int), ("main_ms", ctypes.c_double),
+                ("main_subsets", ctypes.c_double)]
 
 
 @dataclass
@@ -56,12 +57,36 @@ class PermResult:
     integral: int
     err_est: float
     nu: int
+    ext_k: int
+    main_ms: float
+    main_subsets: float
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
-                   r.n, r.mode, r.integral, r.err_est, r.nu)
+                   r.n, r.mode, r.integral, r.err_est, r.nu, r.ext_k, r.main_ms,
+                   r.main_subsets)
 
+
+class BatchTuningC(ctypes.Structure):
+    _fields_ = [("item_log2", ctypes.c_int), ("streams", ctypes.c_int),
+                ("max_grid", ctypes.c_int), ("device_ms", ctypes.c_double),
+                ("trials", ctypes.c_int)]
+
+
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("item_log2", ctypes.c_int), ("streams", ctypes.c_int),
+                ("inflight", ctypes.c_int), ("max_items", ctypes.c_int),
+                ("iteration_us", ctypes.c_double), ("max_resident", ctypes.c_int),
+                ("max_queued_work", ctypes.c_double)]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in
+                ("submitted", "admitted", "completed", "iterations", "items", "pending")]
+
+
+PERM_PENDING = 5
 
 _LIB = None
 
@@ -99,6 +124,34 @@ def lib():
                 f = getattr(L, f"perm_{v}_sparse_{sfx}")
                 f.restype = ctypes.c_double
                 f.argtypes = [vp, ip, ip, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.perm_batch_create.argtypes = [ctypes.POINTER(vp)]
+        L.perm_batch_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.perm_batch_add_dense.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ip]
+        L.perm_batch_add_sparse.argtypes = [vp, ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ip]
+        L.perm_batch_size.argtypes = [vp]
+        L.perm_batch_run.argtypes = [vp, ctypes.POINTER(PermResultC),
+                                     ctypes.POINTER(ctypes.c_double)]
+        L.perm_batch_destroy.argtypes = [vp]
+        L.perm_batch_destroy.restype = None
+        L.perm_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BatchTuningC)]
+        L.perm_batch_get.argtypes = [vp, ctypes.POINTER(BatchTuningC)]
+        u64p, dp = ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)
+        L.perm_server_default_config.argtypes = [ctypes.POINTER(ServerConfigC)]
+        L.perm_server_default_config.restype = None
+        L.perm_server_create.argtypes = [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]
+        L.perm_server_submit_dense.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, u64p]
+        L.perm_server_submit_sparse.argtypes = [vp, ctypes.c_int, vp, ip, ip, vp, ctypes.c_int,
+                                                u64p]
+        L.perm_server_wait.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
+        L.perm_server_poll.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
+        L.perm_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
+        L.perm_server_destroy.argtypes = [vp]
+        L.perm_server_destroy.restype = None
+        L.perm_connect.restype = None
+        L.perm_read_calculate_return.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5
+        L.perm_read_calculate_return.restype = ctypes.c_double
+        L.perm_matlab_calculate_return.argtypes = [ip] + [ctypes.c_int] * 7
+        L.perm_matlab_calculate_return.restype = ctypes.c_double
         for sfx in _SFX.values():
             f = getattr(L, f"perm_parallel_perman64_{sfx}")
             f.restype = ctypes.c_double
@@ -232,38 +285,211 @@ class Plan:
             pass
 
 
+class Batch:
+    """Many matrices (n <= 40) resident on the device; run() replays the
+    captured graph of persistent variant kernels and returns one result per
+    matrix (insertion order) and the device time in ms."""
+
+    def __init__(self, item_log2: int = 0, streams: int = 0, max_grid: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().perm_batch_create(ctypes.byref(self._h)))
+        _check(lib().perm_batch_set(self._h, item_log2, streams, max_grid))
+        self._keep = []
+
+    def add(self, mat, sparse_path: bool = False) -> int:
+        a, dt = _mat(mat)
+        idx = ctypes.c_int()
+        if sparse_path:
+            cptrs, rows, cvals = compress(a)
+            self._keep.append((cptrs, rows, cvals))
+            st = lib().perm_batch_add_sparse(self._h, dt, a.ctypes.data, _ip(cptrs), _ip(rows),
+                                             cvals.ctypes.data, a.shape[0], ctypes.byref(idx))
+        else:
+            st = lib().perm_batch_add_dense(self._h, dt, a.ctypes.data, a.shape[0],
+                                            ctypes.byref(idx))
+        _check(st)
+        return idx.value
+
+    def set(self, item_log2: int = 0, streams: int = 0, max_grid: int = 0):
+        _check(lib().perm_batch_set(self._h, item_log2, streams, max_grid))
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Time candidate work splits on this batch and keep the fastest."""
+        t = BatchTuningC()
+        _check(lib().perm_batch_autotune(self._h, reps, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuningC._fields_}
+
+    def tuning(self) -> dict:
+        t = BatchTuningC()
+        _check(lib().perm_batch_get(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuningC._fields_}
+
+    def run(self):
+        n = lib().perm_batch_size(self._h)
+        res = (PermResultC * n)()
+        ms = ctypes.c_double()
+        _check(lib().perm_batch_run(self._h, res, ctypes.byref(ms)))
+        return [PermResult.from_c(r) for r in res], ms.value
+
+    def close(self):
+        if self._h:
+            lib().perm_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_calculate_return(filename: str, algorithm: int, nt: int = 16, x: int = 0, y: int = 0,
+                          z: int = 0) -> float:
+    """The reference connector's entry (py_connector.c:63) on the GPU path:
+    NaN for the approximation IDs 0-3 (see perm_last_error)."""
+    return lib().perm_read_calculate_return(filename.encode(), algorithm, nt, x, y, z)
+
+
+def matlab_calculate_return(mat, algorithm: int, nt: int = 16, x: int = 0, y: int = 0,
+                            z: int = 0) -> float:
+    """matlab_calculate_return (py_connector.c:146): int32 row-major matrix."""
+    a = np.ascontiguousarray(mat, dtype=np.int32)
+    return lib().perm_matlab_calculate_return(_ip(a.reshape(-1)), algorithm, nt, x, y, z,
+                                              a.shape[0], int((a != 0).sum()))
+
+
+def last_error() -> str:
+    return lib().perm_last_error().decode()
+
+
+class Server:
+    """Continuous-batching permanent server on one GPU (perm_server_*).
+    submit() returns a ticket at once; wait() blocks for its result and
+    returns (PermResult, latency_ms); poll() returns None while pending.
+    Keyword arguments override perm_server_config fields."""
+
+    def __init__(self, **cfg):
+        c = ServerConfigC()
+        lib().perm_server_default_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self._h = ctypes.c_void_p()
+        _check(lib().perm_server_create(ctypes.byref(c), ctypes.byref(self._h)))
+
+    def submit(self, mat, sparse_path: bool = False) -> int:
+        a, dt = _mat(mat)
+        tid = ctypes.c_ulonglong()
+        if sparse_path:
+            cptrs, rows, cvals = compress(a)
+            st = lib().perm_server_submit_sparse(self._h, dt, a.ctypes.data, _ip(cptrs),
+                                                 _ip(rows), cvals.ctypes.data, a.shape[0],
+                                                 ctypes.byref(tid))
+        else:
+            st = lib().perm_server_submit_dense(self._h, dt, a.ctypes.data, a.shape[0],
+                                                ctypes.byref(tid))
+        _check(st)  # the matrix is copied before submit returns
+        return tid.value
+
+    def wait(self, ticket: int):
+        r, lat = PermResultC(), ctypes.c_double()
+        _check(lib().perm_server_wait(self._h, ticket, ctypes.byref(r), ctypes.byref(lat)))
+        return PermResult.from_c(r), lat.value
+
+    def poll(self, ticket: int):
+        r, lat = PermResultC(), ctypes.c_double()
+        st = lib().perm_server_poll(self._h, ticket, ctypes.byref(r), ctypes.byref(lat))
+        if st == PERM_PENDING:
+            return None
+        _check(st)
+        return PermResult.from_c(r), lat.value
+
+    def stats(self) -> dict:
+        s = ServerStatsC()
+        _check(lib().perm_server_stats_get(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStatsC._fields_}
+
+    def close(self):
+        if self._h:
+            lib().perm_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def served_workload(count: int, seed: int = 6, n_lo: int = 16, n_hi: int = 36):
+    """Request mix of the served configuration: n in [n_lo, n_hi] with
+    P(n) proportional to 2^(-(n - n_lo)/4), density U[0.1, 0.9], half of the
+    requests 0/1 (int32) and half FP64 U(0,1], every third one through the
+    sparse path.  Request i draws from stream seed*2^20 + i (sizes and kinds
+    from stream seed*2^20 + 2^19)."""
+    ns = np.arange(n_lo, n_hi + 1)
+    w = np.power(2.0, -(ns - n_lo) / 4.0)
+    cdf = np.cumsum(w / w.sum())
+    base = np.uint64(seed) << np.uint64(20)
+    u = uniforms(int(base) + (1 << 19), 3 * count)
+    reqs = []
+    for i in range(count):
+        n = int(ns[min(np.searchsorted(cdf, u[3 * i]), len(ns) - 1)])
+        density = 0.1 + 0.8 * float(u[3 * i + 1])
+        binary = u[3 * i + 2] < 0.5
+        m = synthetic_matrix(n, density, seed=int(base) + i, binary=bool(binary))
+        reqs.append((m.astype(np.int32) if binary else m, i % 3 == 2))
+    return reqs
+
+
 # --------------------------------------------------------------------------
-# Seeded synthetic matrices (splitmix64 stream; identical on every machine)
-_M64 = 0xFFFFFFFFFFFFFFFF
+# Seeded synthetic matrices: counter-based splitmix64, identical on every
+# machine.  Draw k of stream `seed` is mix(mix(seed) + (k+1)*G), i.e. the
+# k-th output of the splitmix64 sequence started at state mix(seed).
+_G = np.uint64(0x9E3779B97F4A7C15)
 
 
-class SplitMix64:
-    def __init__(self, seed: int):
-        self.state = seed & _M64
+def _mix(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
 
-    def next(self) -> int:
-        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
-        z = self.state
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
-        return z ^ (z >> 31)
 
-    def uniform(self) -> float:
-        """Uniform double in [0, 1) with 53 random bits."""
-        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+def uniforms(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    """Draws offset..offset+count-1 of stream `seed` as doubles in [0, 1)."""
+    base = _mix(np.uint64(seed & 0xFFFFFFFFFFFFFFFF))
+    k = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = _mix(base + k * _G)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
 def synthetic_matrix(n: int, density: float, seed: int, binary: bool = False,
                      full_support: bool = False) -> np.ndarray:
-    """n x n float64 matrix: each entry nonzero with probability `density`,
-    nonzeros uniform in (0, 1] (1.0 when `binary`).  With `full_support`,
-    the draw is repeated until every row and column has a nonzero."""
-    rng = SplitMix64(seed)
-    while True:
-        a = np.zeros((n, n), dtype=np.float64)
-        for i in range(n):
-            for j in range(n):
-                if rng.uniform() < density:
-                    a[i, j] = 1.0 if binary else 1.0 - rng.uniform()  # (0, 1]
+    """n x n float64 matrix: entry (i, j) is nonzero when draw 2(i*n+j) of the
+    stream is below `density`, its value 1 - draw 2(i*n+j)+1, in (0, 1] (1.0
+    when `binary`).  With `full_support`, attempts r = 1, 2, ... use stream
+    seed + r*1000003 until every row and column has a nonzero."""
+    for attempt in range(10000):
+        u = uniforms(seed + attempt * 1000003, 2 * n * n).reshape(n, n, 2)
+        a = np.where(u[:, :, 0] < density, 1.0 if binary else 1.0 - u[:, :, 1], 0.0)
         if not full_support or ((a != 0).any(axis=0).all() and (a != 0).any(axis=1).all()):
             return a
+    raise ValueError("no full-support draw found")
+
+
+def mixed_batch(count: int, n_lo: int, n_hi: int, d_lo: float, d_hi: float, seed: int):
+    """`count` matrices with n uniform in [n_lo, n_hi] and density uniform in
+    [d_lo, d_hi] (drawn from stream `seed`), matrix i from stream seed*2^20+i."""
+    u = uniforms(seed, 2 * count)
+    ns = n_lo + np.minimum((u[0::2] * (n_hi - n_lo + 1)).astype(int), n_hi - n_lo)
+    ds = d_lo + u[1::2] * (d_hi - d_lo)
+    return [synthetic_matrix(int(n), float(d), seed * (1 << 20) + i)
+            for i, (n, d) in enumerate(zip(ns, ds))]

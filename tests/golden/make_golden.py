@@ -85,7 +85,8 @@ def main():
         json.dump({"name": "c1_n20_d0.5_seed1", "n": 20, "matrix": c1.tolist(),
                    "exact": frac_str(ex), "value": float(ex)}, fh)
     print("c1", float(ex), flush=True)
-    refdata()
+    if "--no-refdata" not in sys.argv:
+        refdata()
 
 
 def refdata():

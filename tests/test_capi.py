@@ -193,3 +193,19 @@ int main() {
     m = np.array([[1, 2, 0], [0, 3, 4], [5, 0, 6]], dtype=float)
     want = float(O.exact_permanent(m))
     assert [float(v) for v in out[:3]] == [want, want, want] and out[3] == "1"
+
+
+def test_connector_entry_points_reject_without_touching_the_gpu(tmp_path):
+    """The connector equivalents fail loudly (NaN + message) for the
+    approximation IDs and unreadable files; these paths make no GPU call."""
+    f = tmp_path / "m.txt"
+    f.write_text("3 3 double\n0 0 1.5\n1 1 2.0\n2 2 0.5\n")
+    for algo in (0, 1, 2, 3, 9):
+        assert math.isnan(P.read_calculate_return(str(f), algo))
+        assert "algorithm" in P.last_error()
+    assert math.isnan(P.read_calculate_return(str(tmp_path / "missing.txt"), 5))
+    assert "cannot read" in P.last_error()
+    g = tmp_path / "bad.txt"
+    g.write_text("3 3 complex\n")
+    assert math.isnan(P.read_calculate_return(str(g), 5))
+    assert math.isnan(P.matlab_calculate_return(np.eye(3, dtype=np.int32), 1))

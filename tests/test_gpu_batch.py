@@ -1,0 +1,111 @@
+"""Batched mixed-size permanents (persistent variant kernels in one graph).
+
+* every result against the exact oracle (n <= 20) or the single-matrix
+  path (larger n, within the stated error);
+* a matrix gives the same bits alone and in a mix (its items depend only on
+  its own n), and across repeated graph replays;
+* dense and sparse (-s semantics) entries in one batch."""
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_mixed_small_vs_exact():
+    mats = [P.synthetic_matrix(n, d, seed=70 + n) for n in range(1, 21)
+            for d in (0.35, 0.8)]
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, ms = b.run()
+    b.close()
+    assert len(res) == len(mats) and ms > 0
+    for m, r in zip(mats, res):
+        ex = float(O.exact_permanent(m))
+        assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_bound), (m.shape[0], r.value, ex)
+
+
+def test_batch_large_vs_single_path():
+    mats = P.mixed_batch(24, 22, 30, 0.2, 0.8, seed=11)
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, _ = b.run()
+    b.close()
+    for m, r in zip(mats, res):
+        s = P.dense(m)
+        assert abs(r.value - s.value) <= r.err_bound + s.err_bound
+        assert abs(r.value - s.value) <= 1e-10 * abs(s.value)
+
+
+def test_same_alone_and_in_the_mix():
+    target = P.synthetic_matrix(27, 0.45, seed=123)
+    alone = P.Batch()
+    alone.add(target)
+    r_alone = alone.run()[0][0]
+    alone.close()
+    mix = P.Batch()
+    others = P.mixed_batch(60, 12, 30, 0.2, 0.8, seed=3)
+    for m in others[:30]:
+        mix.add(m)
+    k = mix.add(target)
+    for m in others[30:]:
+        mix.add(m, sparse_path=True)
+    res1, _ = mix.run()
+    res2, _ = mix.run()  # graph replay
+    mix.close()
+    for r in (res1[k], res2[k]):
+        assert (r.value, r.sum_hi, r.sum_lo, r.abs_sum) == (
+            r_alone.value, r_alone.sum_hi, r_alone.sum_lo, r_alone.abs_sum)
+    assert [r.value for r in res1] == [r.value for r in res2]
+
+
+def test_batch_sparse_entries_follow_reference_semantics():
+    a = P.synthetic_matrix(14, 0.6, seed=9) - 0.25 * (np.arange(14)[:, None] % 4 == 0)
+    b = P.Batch()
+    b.add(a, sparse_path=True)
+    b.add(np.abs(a))
+    res, _ = b.run()
+    b.close()
+    assert res[0].value == pytest.approx(O.cpu_perman64_sparse(a, 2), rel=1e-9)
+    assert res[1].value == pytest.approx(float(O.exact_permanent(np.abs(a))), rel=1e-10)
+
+
+def test_batch_item_size_parameter_keeps_results():
+    mats = P.mixed_batch(16, 16, 26, 0.3, 0.7, seed=21)
+    vals = []
+    for item_log2 in (12, 16, 20):
+        b = P.Batch(item_log2=item_log2, streams=2)
+        for m in mats:
+            b.add(m)
+        vals.append(b.run()[0])
+        b.close()
+    for rs in zip(*vals):
+        v0 = rs[0].value
+        for r in rs[1:]:
+            assert abs(r.value - v0) <= 1e-10 * abs(v0) + r.err_bound
+
+
+def test_batch_rejects_too_large():
+    b = P.Batch()
+    with pytest.raises(P.PermError):
+        b.add(np.ones((41, 41)))
+    b.close()
+
+
+def test_autotune_picks_a_split_and_keeps_results():
+    mats = P.mixed_batch(48, 12, 24, 0.2, 0.8, seed=44)
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    before, _ = b.run()
+    best = b.autotune(reps=1)
+    assert best["trials"] >= 10 and best["device_ms"] > 0
+    assert b.tuning()["item_log2"] == best["item_log2"]
+    after, _ = b.run()
+    b.close()
+    for x, y in zip(before, after):
+        assert abs(x.value - y.value) <= x.err_bound + y.err_bound

@@ -1,0 +1,120 @@
+"""Served form: continuous batching of requests of mixed n (perm_server_*).
+
+* results against the exact oracle (n <= 20) or the single-matrix path
+  (within the stated errors), integer inputs exact;
+* a request gives the same bits alone and in a concurrent mix, whatever
+  iterations its items were packed into;
+* poll/wait/stats semantics, rejection of bad requests, shutdown drains."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(m, r):
+    n = m.shape[0]
+    if n <= 20 or np.issubdtype(m.dtype, np.integer):
+        ex = O.exact_permanent(m)
+        if np.issubdtype(m.dtype, np.integer) and ex < 2 ** 53:
+            assert r.value == float(ex), (n, r.value, ex)
+        else:
+            assert abs(r.value - float(ex)) <= max(1e-12 * abs(float(ex)), r.err_bound), (n,)
+    else:
+        s = P.dense(m)
+        assert abs(r.value - s.value) <= r.err_bound + s.err_bound
+        assert abs(r.value - s.value) <= 1e-10 * abs(s.value)
+
+
+def test_server_results_vs_oracle():
+    reqs = []
+    for i, n in enumerate(list(range(1, 21)) + [22, 24, 26, 28, 30]):
+        binary = i % 3 == 0
+        m = P.synthetic_matrix(n, 0.3 + 0.5 * ((i * 7) % 10) / 10, seed=1000 + i, binary=binary)
+        reqs.append((m.astype(np.int32) if binary else m, i % 4 == 1))
+    with P.Server(item_log2=14) as srv:
+        tickets = [srv.submit(m, sparse_path=sp) for m, sp in reqs]
+        results = [srv.wait(t) for t in tickets]
+    for (m, sp), (r, lat) in zip(reqs, results):
+        assert lat > 0
+        if sp:
+            assert r.value == pytest.approx(O.cpu_perman64_sparse(m.astype(np.float64), 2),
+                                            rel=1e-9, abs=1e-300)
+        else:
+            _check(m, r)
+
+
+def test_server_same_bits_alone_and_in_mix():
+    target = P.synthetic_matrix(26, 0.5, seed=4242)
+    with P.Server(item_log2=12, iteration_us=300.0) as srv:
+        alone = srv.wait(srv.submit(target))[0]
+    others = P.served_workload(120, seed=9, n_lo=12, n_hi=28)
+    got = {}
+    with P.Server(item_log2=12, iteration_us=300.0, streams=3) as srv:
+        def client(part):
+            for m, sp in part:
+                srv.submit(m, sparse_path=sp)
+        threads = [threading.Thread(target=client, args=(others[k::4],)) for k in range(4)]
+        for th in threads:
+            th.start()
+        t1 = srv.submit(target)
+        for th in threads:
+            th.join()
+        t2 = srv.submit(target)
+        got[1] = srv.wait(t1)[0]
+        got[2] = srv.wait(t2)[0]
+        stats = srv.stats()
+    assert stats["iterations"] > 1
+    for r in got.values():
+        assert (r.value, r.sum_hi, r.sum_lo, r.abs_sum) == (
+            alone.value, alone.sum_hi, alone.sum_lo, alone.abs_sum)
+
+
+def test_server_matches_batch_path_bits():
+    mats = P.mixed_batch(12, 14, 26, 0.3, 0.8, seed=31)
+    b = P.Batch(item_log2=16)
+    for m in mats:
+        b.add(m)
+    bres, _ = b.run()
+    b.close()
+    with P.Server(item_log2=16) as srv:
+        sres = [srv.wait(srv.submit(m))[0] for m in mats]
+    for x, y in zip(bres, sres):
+        assert abs(x.value - y.value) <= x.err_bound + y.err_bound
+        assert abs(x.value - y.value) <= 1e-12 * abs(x.value)
+
+
+def test_server_poll_stats_and_errors():
+    with P.Server() as srv:
+        t = srv.submit(P.synthetic_matrix(30, 0.5, seed=5))
+        first = srv.poll(t)
+        r, lat = srv.wait(t) if first is None else first
+        assert r.n == 30 and lat > 0
+        with pytest.raises(P.PermError):
+            srv.wait(t)  # ticket already collected
+        with pytest.raises(P.PermError):
+            srv.submit(np.ones((41, 41)))
+        st = srv.stats()
+        assert st["submitted"] == 1 and st["completed"] == 1 and st["pending"] == 0
+
+
+def test_server_shutdown_drains():
+    srv = P.Server(item_log2=12)
+    for i in range(40):
+        srv.submit(P.synthetic_matrix(20 + i % 8, 0.5, seed=i))
+    srv.close()  # must finish the submitted work and return
+
+
+def test_server_integer_requests_exact():
+    reqs = [P.synthetic_matrix(n, d, seed=77 + n, binary=True).astype(np.int32)
+            for n, d in [(18, 0.5), (22, 0.35), (24, 0.3), (26, 0.25)]]
+    with P.Server() as srv:
+        res = [srv.wait(srv.submit(m))[0] for m in reqs]
+    for m, r in zip(reqs, res):
+        ex = O.exact_permanent(m)
+        if ex < 2 ** 53:
+            assert r.value == float(ex), (m.shape, r, ex)
