@@ -5,6 +5,7 @@
 // ABI of libperman_hip.so (perman_hip.h):
 //   gpu_exact_dense.cu:402,467,525,583,641   gpu_perman64_<variant>(T*, int, int, int)
 //   gpu_exact_sparse.cu:674,734,794,854      gpu_perman64_<variant>_sparse(...)
+//   gpu_exact_sparse.cu:1124                 gpu_perman64_xshared_coalescing_mshared_skipper
 //   algo.h:663, algo.h:569                   parallel_perman64 / _sparse
 // for T in {int, float, double}.  A caller that included the reference's .cu
 // files replaces those includes with this header and links
@@ -60,6 +61,17 @@ inline double cpu_sparse(float* m, int* cp, int* r, float* cv, int n, int t) {
 inline double cpu_sparse(double* m, int* cp, int* r, double* cv, int n, int t) {
   return perm_parallel_perman64_sparse_f64(m, cp, r, cv, n, t);
 }
+inline double skipper(int* m, int* rp, int* c, int* cp, int* r, int* cv, int n, int g, int b) {
+  return perm_xshared_coalescing_mshared_skipper_i32(m, rp, c, cp, r, cv, n, g, b);
+}
+inline double skipper(float* m, int* rp, int* c, int* cp, int* r, float* cv, int n, int g,
+                      int b) {
+  return perm_xshared_coalescing_mshared_skipper_f32(m, rp, c, cp, r, cv, n, g, b);
+}
+inline double skipper(double* m, int* rp, int* c, int* cp, int* r, double* cv, int n, int g,
+                      int b) {
+  return perm_xshared_coalescing_mshared_skipper_f64(m, rp, c, cp, r, cv, n, g, b);
+}
 }  // namespace perman_detail
 
 // ---- dense, algorithm IDs 0-4 -------------------------------------------
@@ -106,6 +118,14 @@ double gpu_perman64_xshared_coalescing_mshared_sparse(T* mat, int* cptrs, int* r
                                                       int nov, int grid_dim, int block_dim) {
   return perman_detail::xshared_coalescing_mshared_sparse(mat, cptrs, rows, cvals, nov, grid_dim,
                                                           block_dim);
+}
+
+// ---- sparse skipper, algorithm ID 7 (gpu_exact_sparse.cu:1124) -----------
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_skipper(T* mat, int* rptrs, int* cols, int* cptrs,
+                                                       int* rows, T* cvals, int nov,
+                                                       int grid_dim, int block_dim) {
+  return perman_detail::skipper(mat, rptrs, cols, cptrs, rows, cvals, nov, grid_dim, block_dim);
 }
 
 // ---- CPU (`-c`) ------------------------------------------------------------

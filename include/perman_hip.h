@@ -46,9 +46,7 @@ extern "C" {
  * kernel.  mode: 0 = FP64 products (the fast kernel), 1 = double-double
  * products (used for integer-valued matrices whose FP64 result could not be
  * guaranteed to round to the exact integer), 2 = double-double X and
- * products.  ext_k >= 0 (mode 0): the subsets with |S| <= ext_k or
- * |S| >= n-1-ext_k were summed in double-double with exact X instead of by
- * the FP64 walk. */
+ * products. */
 typedef struct perm_result {
   double value;
   double sum_hi;
@@ -63,7 +61,6 @@ typedef struct perm_result {
   int integral;         /* 1 if every entry is an integer (X is then exact) */
   double err_est;       /* statistical error estimate (independent roundings) */
   int nu;               /* FP64 kernel: rows touched by Gray column 0 (pair variant), or n */
-  int ext_k;            /* extreme-subset cut of the FP64 walk, -1 for none */
   double main_ms;       /* single-matrix path: device time of the main walk kernel */
   double main_subsets;  /* subsets that kernel evaluated */
 } perm_result;
@@ -189,6 +186,103 @@ typedef struct perm_batch_tuning {
 } perm_batch_tuning;
 int perm_batch_autotune(perm_batch* batch, int reps, perm_batch_tuning* best);
 int perm_batch_get(const perm_batch* batch, perm_batch_tuning* out);
+
+/* ---- sparse skipper, ID 7: gpu_perman64_xshared_coalescing_mshared_skipper(
+ * T* mat, int* rptrs, int* cols, int* cptrs, int* rows, T* cvals, int nov,
+ * int grid_dim, int block_dim), gpu_exact_sparse.cu:1124.  Skips, per wave,
+ * aligned runs of Gray steps whose terms are all zero (a row of x that is
+ * zero and untouched by the run's columns); needs exact zeros, so it runs
+ * for integer matrices and n >= 16 -- other inputs take the sparse path
+ * (same sum).  x0 from the dense matrix, column updates from the CCS;
+ * the CRS (rptrs/cols) is accepted and not needed.  perm_skip's result:
+ * main_subsets = subsets actually walked (the rest were skipped). */
+int perm_skip(int dtype, const void* mat, const int* rptrs, const int* cols, const int* cptrs,
+              const int* rows, const void* cvals, int nov, perm_result* out);
+double perm_xshared_coalescing_mshared_skipper_i32(const int* mat, const int* rptrs,
+                                                    const int* cols, const int* cptrs,
+                                                    const int* rows, const int* cvals, int nov,
+                                                    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_skipper_f32(const float* mat, const int* rptrs,
+                                                    const int* cols, const int* cptrs,
+                                                    const int* rows, const float* cvals, int nov,
+                                                    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_skipper_f64(const double* mat, const int* rptrs,
+                                                    const int* cols, const int* cptrs,
+                                                    const int* rows, const double* cvals,
+                                                    int nov, int grid_dim, int block_dim);
+
+/* ---- multi-GPU and hybrid IDs on one GPU ----------------------------------
+ * Dense ID 5 gpu_perman64_xshared_coalescing_mshared_multigpu (gpu_exact_dense.cu:702),
+ * ID 6 ..._multigpucpu_chunks (:777), ID 66 ..._multigpu_manual_distribution
+ * (:914); sparse ID 5 ..._multigpu_sparse (gpu_exact_sparse.cu:917), ID 6
+ * ..._multigpucpu_chunks_sparse (:996), ID 8 ..._multigpucpu_chunks_skipper
+ * (:1193), ID 66 ..._multigpu_sparse_manual_distribution (:1328).
+ * This is a single-GPU build: gpu_num is accepted and one GPU computes the
+ * whole Gray range.  The chunked forms (ID 6) cut the FP64 walk into
+ * 2^(n-29) chunks (2^(n-28) with cpu != 0, as the reference) handed out by a
+ * shared counter to the GPU and, with cpu != 0, to a CPU worker using
+ * `threads` OpenMP threads; partial sums are combined in chunk order.
+ * Without the CPU worker the value equals the one-launch value bitwise; with
+ * it, the chunks the CPU took differ in rounding only.  ID 8 runs the skipper
+ * on the GPU alone.  perm_chunked is the full-result form (cptrs NULL:
+ * dense); *cpu_chunks receives the number of chunks the CPU computed. */
+int perm_chunked(int dtype, const void* mat, const int* cptrs, const int* rows,
+                 const void* cvals, int nov, int use_cpu, int threads, perm_result* out,
+                 int* cpu_chunks);
+double perm_xshared_coalescing_mshared_multigpu_dense_i32(const int* mat, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_i32(
+    const int* mat, int nov, int gpu_num, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_i32(
+    const int* mat, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_i32(
+    const int* mat, const int* cptrs, const int* rows, const int* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_manual_distribution_i32(
+    const int* mat, const int* cptrs, const int* rows, const int* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_sparse_i32(
+    const int* mat, const int* cptrs, const int* rows, const int* cvals, int nov, int gpu_num,
+    int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_i32(
+    const int* mat, const int* rptrs, const int* cols, const int* cptrs, const int* rows,
+    const int* cvals, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_dense_f32(const float* mat, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_f32(
+    const float* mat, int nov, int gpu_num, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_f32(
+    const float* mat, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_f32(
+    const float* mat, const int* cptrs, const int* rows, const float* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_manual_distribution_f32(
+    const float* mat, const int* cptrs, const int* rows, const float* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_sparse_f32(
+    const float* mat, const int* cptrs, const int* rows, const float* cvals, int nov, int gpu_num,
+    int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_f32(
+    const float* mat, const int* rptrs, const int* cols, const int* cptrs, const int* rows,
+    const float* cvals, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_dense_f64(const double* mat, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_f64(
+    const double* mat, int nov, int gpu_num, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_f64(
+    const double* mat, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_f64(
+    const double* mat, const int* cptrs, const int* rows, const double* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpu_sparse_manual_distribution_f64(
+    const double* mat, const int* cptrs, const int* rows, const double* cvals, int nov, int gpu_num,
+    int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_sparse_f64(
+    const double* mat, const int* cptrs, const int* rows, const double* cvals, int nov, int gpu_num,
+    int cpu, int threads, int grid_dim, int block_dim);
+double perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_f64(
+    const double* mat, const int* rptrs, const int* cols, const int* cptrs, const int* rows,
+    const double* cvals, int nov, int gpu_num, int cpu, int threads, int grid_dim, int block_dim);
 
 /* ---- the reference's Python/MATLAB connector (py_connector.c:63-190) ------
  * perm_read_calculate_return(filename, algorithm, nt, x, y, z) replaces

@@ -118,6 +118,78 @@ double sparse_cpu(const T* mat, const int* cptrs, const int* rows, const T* cval
 
 }  // namespace
 
+namespace perm {
+
+// Lanes [l0, l1) of the GPU walk, computed on the host with `threads`
+// OpenMP threads: the same subsets as the kernel's lanes (closed-form start
+// from the column store blocks 2(w+b) selected by the lane id, Gray walk over
+// the low w columns), FP64 X and products, lane sums folded into
+// double-double every 256 steps and combined in lane order.
+// out = {hi, lo, sum |term|, sum over runs of (run sum of |term|)^2}.
+void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int n,
+               int walk_bits, int high_bits, uint64_t l0, uint64_t l1, int threads,
+               double out[4]) {
+  const uint64_t count = l1 > l0 ? l1 - l0 : 0;
+  std::vector<DDh> part(count);
+  std::vector<double> abs_part(count), sq_part(count);
+  if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (int64_t i = 0; i < (int64_t)count; ++i) {
+    const uint64_t pattern = l0 + i;
+    double x[64];
+    std::memcpy(x, x0.data(), sizeof(double) * n);
+    int parity = 0;
+    for (int b = 0; b < high_bits; ++b)
+      if ((pattern >> b) & 1) {
+        parity ^= 1;
+        const double* c = &cs[(size_t)(2 * (walk_bits + b)) * n];
+        for (int j = 0; j < n; ++j) x[j] += c[j];
+      }
+    DDh acc;
+    double p = 0.0, a = 0.0, a_run = 0.0, q = 0.0;
+    const uint64_t steps = 1ull << walk_bits;
+    for (uint64_t t = 0; t < steps; ++t) {
+      if (t) {
+        const int k = __builtin_ctzll(t);
+        const double* c = &cs[(size_t)(2 * k + ((t >> (k + 1)) & 1)) * n];
+        for (int j = 0; j < n; ++j) x[j] += c[j];
+      }
+      double prod = x[0];
+      for (int j = 1; j < n; ++j) prod *= x[j];
+      p += (t & 1) ? -prod : prod;
+      a += std::fabs(prod);
+      if ((t & 255) == 255) {
+        acc = dd_add_d(acc, p);
+        p = 0.0;
+        q += (a - a_run) * (a - a_run);
+        a_run = a;
+      }
+    }
+    acc = dd_add_d(acc, p);
+    q += (a - a_run) * (a - a_run);
+    if (parity) {
+      acc.hi = -acc.hi;
+      acc.lo = -acc.lo;
+    }
+    part[i] = acc;
+    abs_part[i] = a;
+    sq_part[i] = q;
+  }
+  DDh tot;
+  double a = 0.0, q = 0.0;
+  for (uint64_t i = 0; i < count; ++i) {
+    tot = dd_add(tot, part[i]);
+    a += abs_part[i];
+    q += sq_part[i];
+  }
+  out[0] = tot.hi;
+  out[1] = tot.lo;
+  out[2] = a;
+  out[3] = q;
+}
+
+}  // namespace perm
+
 extern "C" {
 double perm_parallel_perman64_i32(const int* mat, int nov, int threads) {
   return dense_cpu(mat, nov, threads);

@@ -232,6 +232,7 @@ namespace perm {
 
 void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial& r,
                       perm_result& o) {
+  const int high_bits = n - 1 - walk_bits;
   std::memset(&o, 0, sizeof(o));
   const double scale = (double)(4 * (n & 1) - 2);
   o.n = n;
@@ -241,11 +242,12 @@ void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial
   o.abs_sum = std::fabs(scale) * r.abs_sum;
   o.err_bound =
       error_statement(o.abs_sum, o.value, n, walk_bits, dense_unroll_bits(n), integral, 0);
-  o.err_est = error_estimate(scale * scale * r.sq_sum, o.value, n, 0, integral);
+  o.err_est =
+      error_estimate(scale * scale * r.sq_sum, o.value, n, walk_bits, high_bits, 0, integral);
+  o.err_est = std::min(o.err_est, o.err_bound);
   o.subsets = std::ldexp(1.0, n - 1);
   o.integral = integral ? 1 : 0;
   o.nu = nu;
-  o.ext_k = -1;
   o.mode = 0;
   round_integral(integral, o.value, o.err_bound, o.err_est);
   o.status = PERM_OK;

@@ -15,9 +15,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
+#include <atomic>
 #include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/perman_hip.h"
@@ -59,10 +62,8 @@ struct perm_plan {
   DeviceBuffers mem;
   perm::DenseLaunch L{};
   int unroll_bits = 0;
-  int walk_bits = 0, high_bits = 0;  // base split of the n-1 Gray columns
-  perm::PassGeometry cut_main{}, cut_side{};  // FP64 passes under the cut
+  int walk_bits = 0, high_bits = 0;  // split of the n-1 Gray columns
   int nu = 0;             // active rows of Gray column 0 for the FP64 kernel (n: none)
-  int ext_k = -1;         // extreme-subset cut of the FP64 walk (-1: none)
   bool integral = false;  // every entry an integer of magnitude < 2^31
   bool borrowed = false;  // stream/events/memory belong to the device cache
 };
@@ -89,30 +90,19 @@ DeviceCache& device_cache(int dev) {
   return *table[dev];
 }
 
-// Device layout of a plan: column store, x0, x_all (hi, lo), partial
-// records (sized for the larger of the plain and the cut configuration),
-// result record.
+// Device layout of a plan: column store, x0, partial records, result record.
 struct PlanLayout {
-  size_t col, x, xall, part, res;
-  size_t total() const { return col + x + xall + part + res; }
+  size_t col, x, part, res;
+  size_t total() const { return col + x + part + res; }
 };
 
 uint64_t blocks_of(uint64_t lanes) { return (lanes + 255) / 256; }
-
-uint64_t partial_records(const perm_plan* P) {
-  const uint64_t plain = blocks_of(1ull << P->high_bits);
-  if (P->ext_k < 0) return plain;
-  const uint64_t cut =
-      blocks_of(P->cut_main.lanes) + blocks_of(P->cut_side.lanes) + P->L.ext_blocks;
-  return std::max(plain, cut);
-}
 
 PlanLayout plan_layout(const perm_plan* P, size_t col_doubles) {
   PlanLayout l;
   l.col = align_up(col_doubles * sizeof(double), 256);
   l.x = align_up((size_t)P->n * sizeof(double), 256);
-  l.xall = align_up(2 * (size_t)P->n * sizeof(double), 256);
-  l.part = align_up(partial_records(P) * sizeof(perm::Partial), 256);
+  l.part = align_up(blocks_of(1ull << P->high_bits) * sizeof(perm::Partial), 256);
   l.res = align_up(sizeof(perm::Partial), 256);
   return l;
 }
@@ -126,12 +116,6 @@ size_t layout_plan(perm_plan* P, int n, size_t col_doubles) {
   P->walk_bits = sp.walk_bits;
   P->high_bits = sp.high_bits;
   P->L.n = n;
-  P->ext_k = perm::choose_ext_k(n, perm::kExtMaxK, env_int("PERMAN_EXT_K", -2));
-  P->L.ext_per_side = P->ext_k >= 0 ? perm::extreme_per_side(n, P->ext_k) : 0;
-  P->L.ext_blocks = blocks_of(2 * P->L.ext_per_side);
-  if (P->ext_k >= 0)
-    perm::cut_passes(n, sp.walk_bits, sp.high_bits, P->ext_k, P->unroll_bits, P->cut_main,
-                     P->cut_side);
   return plan_layout(P, col_doubles).total();
 }
 
@@ -140,54 +124,29 @@ void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
   char* b = (char*)base;
   P->L.colpm = (const double*)b;
   P->L.x0 = (const double*)(b + l.col);
-  P->L.xall = (const double*)(b + l.col + l.x);
-  P->L.partials = (perm::Partial*)(b + l.col + l.x + l.xall);
-  P->L.result = (perm::Partial*)(b + l.col + l.x + l.xall + l.part);
+  P->L.partials = (perm::Partial*)(b + l.col + l.x);
+  P->L.result = (perm::Partial*)(b + l.col + l.x + l.part);
 }
 
-perm::LanePass make_pass(const perm::PassGeometry& g, bool checked, perm::Partial* partials) {
-  perm::LanePass p;
-  p.lanes = g.lanes;
-  p.blocks = blocks_of(g.lanes);
-  p.class_mask = g.class_mask;
-  p.walk_bits = g.walk_bits;
-  p.high_bits = g.high_bits;
-  p.free_bits = g.free_bits;
-  p.checked = checked ? 1 : 0;
-  p.partials = partials;
-  return p;
-}
-
-// Passes of one run in `mode`: with the extreme-subset cut (FP64 mode and
-// ext_k >= 0) the main and side passes and the extreme-subset kernel, else
-// one pass over all 2^h lanes.  Partial records: [main | side | extreme].
+// The walk of one run in `mode`: all 2^h lanes.
 void configure_passes(perm_plan* P, int mode) {
   perm::DenseLaunch& L = P->L;
   L.mode = mode;
   L.nu = mode == 0 ? P->nu : P->n;
-  L.ext_k = mode == 0 ? P->ext_k : -1;
-  if (L.ext_k >= 0) {
-    L.main = make_pass(P->cut_main, false, L.partials);
-    L.side = make_pass(P->cut_side, true, L.partials + L.main.blocks);
-    L.ext_partials = L.side.partials + L.side.blocks;
-    L.total_blocks = L.main.blocks + L.side.blocks + L.ext_blocks;
-  } else {
-    const perm::PassGeometry all{1ull << P->high_bits, ~0ull, P->walk_bits, P->high_bits, 0};
-    L.main = make_pass(all, false, L.partials);
-    L.side = make_pass(perm::PassGeometry{0, 0, 0, 0, 0}, false, nullptr);
-    L.ext_partials = nullptr;
-    L.total_blocks = L.main.blocks;
-  }
+  L.main.lanes = 1ull << P->high_bits;
+  L.main.blocks = blocks_of(L.main.lanes);
+  L.main.walk_bits = P->walk_bits;
+  L.main.high_bits = P->high_bits;
+  L.main.partials = L.partials;
+  L.main.block_begin = 0;
+  L.main.block_end = 0;
+  L.total_blocks = L.main.blocks;
 }
 
 int upload_plan(perm_plan* P, const std::vector<double>& colstore, const std::vector<double>& x0) {
-  std::vector<double> xall;
-  perm::extreme_total_vector(P->n, colstore, x0, xall);
   HIP_TRY(hipMemcpyAsync((void*)P->L.colpm, colstore.data(), colstore.size() * sizeof(double),
                          hipMemcpyHostToDevice, P->stream));
   HIP_TRY(hipMemcpyAsync((void*)P->L.x0, x0.data(), x0.size() * sizeof(double),
-                         hipMemcpyHostToDevice, P->stream));
-  HIP_TRY(hipMemcpyAsync((void*)P->L.xall, xall.data(), xall.size() * sizeof(double),
                          hipMemcpyHostToDevice, P->stream));
   // the host vectors die on return: wait for the copies
   HIP_TRY(hipStreamSynchronize(P->stream));
@@ -250,9 +209,9 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->err_bound = perm::error_statement(out->abs_sum, out->value, n, P->walk_bits,
                                          P->unroll_bits, P->integral, mode);
   out->subsets = std::ldexp(1.0, n - 1);
-  out->err_est =
-      perm::error_estimate(scale * scale * r.sq_sum, out->value, n, mode, P->integral);
-  out->ext_k = mode == 0 ? P->ext_k : -1;
+  out->err_est = perm::error_estimate(scale * scale * r.sq_sum, out->value, n, P->walk_bits,
+                                      P->high_bits, mode, P->integral);
+  out->err_est = std::min(out->err_est, out->err_bound);
   perm::round_integral(P->integral, out->value, out->err_bound, out->err_est);
   out->status = PERM_OK;
   out->mode = mode;
@@ -285,20 +244,16 @@ int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
   return ok();
 }
 
-// One-shot run through the per-device cache (mode as in plan_run_impl).
-int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
+// Bind a borrowed plan to the per-device cache and upload a prepared
+// matrix (caller holds C.mu).
+int bind_oneshot(int n, const Prepared& prep, DeviceCache& C, perm_plan& P) {
   const std::vector<double>& cs = prep.cs;
-  const std::vector<double>& x0 = prep.x0;
-  if (int s = select_device()) return s;
-  DeviceCache& C = device_cache(perm::current_device());
-  std::lock_guard<std::mutex> guard(C.mu);
   if (!C.stream) {
     HIP_TRY(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&C.ev0));
     HIP_TRY(hipEventCreate(&C.ev1));
     HIP_TRY(hipEventCreate(&C.evm));
   }
-  perm_plan P;
   P.device = perm::current_device();
   P.borrowed = true;
   P.stream = C.stream;
@@ -316,8 +271,109 @@ int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
     C.bytes = need;
   }
   bind_plan_memory(&P, C.base, cs.size());
-  if (int s = upload_plan(&P, cs, x0)) return s;
+  return upload_plan(&P, cs, prep.x0);
+}
+
+// One-shot run through the per-device cache (mode as in plan_run_impl).
+int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
+  if (int s = select_device()) return s;
+  DeviceCache& C = device_cache(perm::current_device());
+  std::lock_guard<std::mutex> guard(C.mu);
+  perm_plan P;
+  if (int s = bind_oneshot(n, prep, C, P)) return s;
   return plan_run_impl(&P, mode, out);
+}
+
+// The reference's chunked GPU+CPU scheduler (multigpucpu_chunks,
+// gpu_exact_dense.cu:777-911) on one GPU.  The FP64 walk is cut into
+// chunks of workgroups (2^(n-29) chunks, 2^(n-28) with the CPU worker, as
+// the reference); a shared counter hands them out.  The calling thread is
+// the GPU worker: it takes runs of chunks large enough to fill the GPU and
+// launches the walk over their block range.  With `use_cpu` a second
+// thread takes one chunk at a time and computes its lanes on the host with
+// `threads` OpenMP threads.  When the counter is exhausted all partial
+// records come back and are summed on the host in block order, CPU chunks in
+// place of theirs.
+// Without the CPU worker the result equals the one-launch result bitwise;
+// with it, chunks the CPU took differ in rounding only.
+int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_result* out,
+                int* cpu_chunks_out) {
+  if (int s = select_device()) return s;
+  DeviceCache& C = device_cache(perm::current_device());
+  std::lock_guard<std::mutex> guard(C.mu);
+  perm_plan P;
+  if (int s = bind_oneshot(n, prep, C, P)) return s;
+  configure_passes(&P, 0);
+  perm::DenseLaunch& L = P.L;
+  const uint64_t mb = L.main.blocks;
+  const int shift = std::max(0, n - (use_cpu ? 28 : 29));
+  const uint64_t chunks =
+      std::max<uint64_t>(1, std::min<uint64_t>(mb, 1ull << std::min(shift, 20)));
+  const uint64_t chunk_blocks = (mb + chunks - 1) / chunks;
+  const uint64_t n_chunks = mb == 0 ? 0 : (mb + chunk_blocks - 1) / chunk_blocks;
+  const uint64_t gpu_take = std::max<uint64_t>(1, 1024 / chunk_blocks);  // ~2^18 lanes a launch
+  std::atomic<uint64_t> next{0};
+  std::vector<std::array<double, 4>> cpu_part;
+  std::vector<uint64_t> cpu_ids;
+  std::thread cpu;
+  if (use_cpu && n_chunks > 0) {
+    cpu = std::thread([&] {
+      for (;;) {
+        const uint64_t c = next.fetch_add(1);
+        if (c >= n_chunks) break;
+        const uint64_t l0 = c * chunk_blocks * 256;
+        const uint64_t l1 = std::min<uint64_t>((c + 1) * chunk_blocks * 256, L.main.lanes);
+        std::array<double, 4> r;
+        perm::cpu_lanes(prep.cs, prep.x0, n, L.main.walk_bits, L.main.high_bits, l0, l1,
+                        threads, r.data());
+        cpu_part.push_back(r);
+        cpu_ids.push_back(c);
+      }
+    });
+  }
+  int st = PERM_OK;
+  hipError_t e = hipEventRecord(P.ev0, P.stream);
+  for (;;) {
+    if (e != hipSuccess) break;
+    const uint64_t c0 = next.fetch_add(gpu_take);
+    if (c0 >= n_chunks) break;
+    const uint64_t c1 = std::min(n_chunks, c0 + gpu_take);
+    perm::DenseLaunch Lc = L;
+    Lc.main.block_begin = c0 * chunk_blocks;
+    Lc.main.block_end = std::min(mb, c1 * chunk_blocks);
+    e = perm::launch_dense_pass_only(Lc, P.stream);
+  }
+  if (cpu.joinable()) cpu.join();
+  std::vector<perm::Partial> parts(L.total_blocks);
+  if (e == hipSuccess) e = hipEventRecord(P.ev1, P.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(parts.data(), L.partials, parts.size() * sizeof(perm::Partial),
+                       hipMemcpyDeviceToHost, P.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P.stream);
+  if (e != hipSuccess)
+    return fail(PERM_ERR_HIP, std::string("chunked run: ") + hipGetErrorString(e));
+  for (size_t i = 0; i < cpu_ids.size(); ++i) {
+    const uint64_t b0 = cpu_ids[i] * chunk_blocks, b1 = std::min(mb, b0 + chunk_blocks);
+    for (uint64_t b = b0; b < b1; ++b) parts[b] = {0.0, 0.0, 0.0, 0.0};
+    parts[b0] = {cpu_part[i][0], cpu_part[i][1], cpu_part[i][2], cpu_part[i][3]};
+  }
+  perm::HostDD s{0.0, 0.0};
+  perm::Partial r{0.0, 0.0, 0.0, 0.0};
+  for (const perm::Partial& p : parts) {
+    s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
+    r.abs_sum += p.abs_sum;
+    r.sq_sum += p.sq_sum;
+  }
+  r.hi = s.hi;
+  r.lo = s.lo;
+  fill_result(&P, r, 0, out);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, P.ev0, P.ev1);
+  out->kernel_ms = ms;
+  if (cpu_chunks_out) *cpu_chunks_out = (int)cpu_ids.size();
+  if (perm::needs_dd_pass(*out)) return plan_run_impl(&P, 1, out);
+  (void)st;
+  return ok();
 }
 
 double nan_result() { return std::numeric_limits<double>::quiet_NaN(); }
@@ -424,5 +480,69 @@ PERM_ALL_TYPES(PERM_SPARSE_DROPIN, xlocal)
 PERM_ALL_TYPES(PERM_SPARSE_DROPIN, xshared)
 PERM_ALL_TYPES(PERM_SPARSE_DROPIN, xshared_coalescing)
 PERM_ALL_TYPES(PERM_SPARSE_DROPIN, xshared_coalescing_mshared)
+
+// Multi-GPU and hybrid IDs on one GPU (gpu_exact_dense.cu:702, :777, :914;
+// gpu_exact_sparse.cu:917, :996, :1193, :1328): gpu_num is accepted and one
+// GPU computes the whole range; the chunked forms run the chunk scheduler
+// (run_chunked) with the CPU worker when `cpu` is set.
+int perm_chunked(int dtype, const void* mat, const int* cptrs, const int* rows,
+                 const void* cvals, int nov, int use_cpu, int threads, perm_result* out,
+                 int* cpu_chunks) {
+  if (!out) return fail(PERM_ERR_ARG, "null result");
+  Prepared p;
+  const int s = cptrs ? perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)
+                      : perm::prepare_dense(dtype, mat, nov, p);
+  if (s) return s;
+  return run_chunked(nov, p, use_cpu != 0, threads, out, cpu_chunks);
+}
+
+#define PERM_HYBRID_DENSE(SFX, T, DT)                                                       \
+  double perm_xshared_coalescing_mshared_multigpu_dense_##SFX(const T* mat, int nov, int,   \
+                                                               int, int) {                  \
+    return dense_dropin(DT, mat, nov);                                                      \
+  }                                                                                         \
+  double perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_##SFX(          \
+      const T* mat, int nov, int, int, int) {                                               \
+    return dense_dropin(DT, mat, nov);                                                      \
+  }                                                                                         \
+  double perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_##SFX(                    \
+      const T* mat, int nov, int, int cpu, int threads, int, int) {                         \
+    perm_result r;                                                                          \
+    if (perm_chunked(DT, mat, nullptr, nullptr, nullptr, nov, cpu, threads, &r, nullptr))   \
+      return nan_result();                                                                  \
+    return r.value;                                                                         \
+  }
+#define PERM_HYBRID_SPARSE(SFX, T, DT)                                                      \
+  double perm_xshared_coalescing_mshared_multigpu_sparse_##SFX(                             \
+      const T* mat, const int* cptrs, const int* rows, const T* cvals, int nov, int, int,   \
+      int) {                                                                                \
+    return sparse_dropin(DT, mat, cptrs, rows, cvals, nov);                                 \
+  }                                                                                         \
+  double perm_xshared_coalescing_mshared_multigpu_sparse_manual_distribution_##SFX(         \
+      const T* mat, const int* cptrs, const int* rows, const T* cvals, int nov, int, int,   \
+      int) {                                                                                \
+    return sparse_dropin(DT, mat, cptrs, rows, cvals, nov);                                 \
+  }                                                                                         \
+  double perm_xshared_coalescing_mshared_multigpucpu_chunks_sparse_##SFX(                   \
+      const T* mat, const int* cptrs, const int* rows, const T* cvals, int nov, int,        \
+      int cpu, int threads, int, int) {                                                     \
+    perm_result r;                                                                          \
+    if (perm_chunked(DT, mat, cptrs, rows, cvals, nov, cpu, threads, &r, nullptr))          \
+      return nan_result();                                                                  \
+    return r.value;                                                                         \
+  }                                                                                         \
+  double perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_##SFX(                  \
+      const T* mat, const int* rptrs, const int* cols, const int* cptrs, const int* rows,   \
+      const T* cvals, int nov, int, int, int, int, int) {                                   \
+    perm_result r;                                                                          \
+    if (perm_skip(DT, mat, rptrs, cols, cptrs, rows, cvals, nov, &r)) return nan_result();  \
+    return r.value;                                                                         \
+  }
+PERM_HYBRID_DENSE(i32, int, PERM_DTYPE_I32)
+PERM_HYBRID_DENSE(f32, float, PERM_DTYPE_F32)
+PERM_HYBRID_DENSE(f64, double, PERM_DTYPE_F64)
+PERM_HYBRID_SPARSE(i32, int, PERM_DTYPE_I32)
+PERM_HYBRID_SPARSE(f32, float, PERM_DTYPE_F32)
+PERM_HYBRID_SPARSE(f64, double, PERM_DTYPE_F64)
 
 }  // extern "C"

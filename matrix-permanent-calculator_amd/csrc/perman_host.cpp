@@ -162,8 +162,10 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p) {
 
 // Sparse: x0 from the dense matrix, column updates from the CCS arrays
 // (reference: gpu_exact_sparse.cu:854-871 and the kernels' CCS walk).
-int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, Prepared& p) {
+// cols[k*n + j] = CCS value of (j, k) for the n-1 walked columns.
+int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
+                  const void* cvals, int n, std::vector<double>& x0, std::vector<double>& cols,
+                  bool& integral) {
   if (!mat || !cptrs) return fail(PERM_ERR_ARG, "null pointer");
   if (int s = check_n(n)) return s;
   const int nnz = cptrs[n];
@@ -175,14 +177,21 @@ int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows
     if (rows[e] < 0 || rows[e] >= n) return fail(PERM_ERR_ARG, "row index out of range");
   std::vector<double> a;
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
-  p.x0.assign(n, 0.0);
-  perm::nw_start_vector(a.data(), n, p.x0.data());
-  std::vector<double> cols((size_t)n * n, 0.0);
+  x0.assign(n, 0.0);
+  perm::nw_start_vector(a.data(), n, x0.data());
+  cols.assign((size_t)n * n, 0.0);
   for (int k = 0; k < n; ++k)
     for (int e = cptrs[k]; e < cptrs[k + 1]; ++e)
       cols[(size_t)k * n + rows[e]] += elem(dtype, cvals, (size_t)e);
-  p.integral = integral_matrix(a) && integral_matrix(cols);
+  integral = integral_matrix(a) && integral_matrix(cols);
   cols.resize((size_t)(n - 1) * n);  // the last column lives in x0
+  return PERM_OK;
+}
+
+int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
+                   const void* cvals, int n, Prepared& p) {
+  std::vector<double> cols;
+  if (int s = gather_sparse(dtype, mat, cptrs, rows, cvals, n, p.x0, cols, p.integral)) return s;
   finish_prepare(n, cols, p);
   return PERM_OK;
 }

@@ -1,6 +1,8 @@
 // perman_internal.hpp -- host functions shared by the C ABI translation units
 // (single-matrix, batch, serving).
 #pragma once
+#include <stdint.h>
+
 #include <string>
 #include <vector>
 
@@ -29,6 +31,11 @@ struct Prepared {
 int prepare_dense(int dtype, const void* mat, int n, Prepared& p);
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                    const void* cvals, int n, Prepared& p);
+// The sparse path's inputs before any reordering: x0 from the dense matrix,
+// cols[k*n + j] = CCS value of (j, k) for the walked columns k < n-1.
+int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
+                  const void* cvals, int n, std::vector<double>& x0, std::vector<double>& cols,
+                  bool& integral);
 
 }  // namespace perm
 
@@ -38,7 +45,7 @@ namespace perm {
 
 // One prepared matrix through the single-matrix path on the current device
 // (per-device cached buffers).  mode -1: automatic (FP64, then double-double
-// products for integer matrices whose FP64 error could reach 1/4); 0, 1, 2:
+// products for integer matrices whose FP64 error could reach 1/2); 0, 1, 2:
 // that kernel mode only.
 int run_prepared(int n, const Prepared& p, int mode, perm_result* out);
 
@@ -51,6 +58,12 @@ void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial
 // An integer matrix whose FP64 result could be off by 1/2 or more while
 // below 2^53: run again with double-double products (mode 1).
 bool needs_dd_pass(const perm_result& o);
+
+// Lanes [l0, l1) of the GPU walk computed on the host (the hybrid chunk
+// scheduler's CPU worker); out = {hi, lo, sum |term|, sq_sum}.
+void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int n,
+               int walk_bits, int high_bits, uint64_t l0, uint64_t l1, int threads,
+               double out[4]);
 
 // Split of an n-matrix into 64-lane batch items of about 2^item_log2
 // subsets (walk_bits + high_bits = n - 1); depends on n and item_log2 only.

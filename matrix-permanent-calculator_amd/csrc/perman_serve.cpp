@@ -141,26 +141,6 @@ namespace {
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-struct HDD {
-  double hi, lo;
-};
-HDD two_sum(double a, double b) {
-  const double s = a + b;
-  const double bb = s - a;
-  return {s, (a - (s - bb)) + (b - bb)};
-}
-HDD quick_two_sum(double a, double b) {
-  const double s = a + b;
-  return {s, b - (s - a)};
-}
-HDD dd_add(HDD a, HDD b) {
-  HDD s = two_sum(a.hi, b.hi);
-  const HDD t = two_sum(a.lo, b.lo);
-  s.lo += t.hi;
-  s = quick_two_sum(s.hi, s.lo);
-  s.lo += t.lo;
-  return quick_two_sum(s.hi, s.lo);
-}
 
 void fail_request(perm_server* S, Request* r, int code, const std::string& msg) {
   std::lock_guard<std::mutex> lk(S->mu);
@@ -182,10 +162,10 @@ void publish(perm_server* S, Request* r) {
 // All items of r are done: sum its partials in item order, state the
 // error, hand integer matrices that need it to the double-double helper.
 void finalize(perm_server* S, Request* r) {
-  HDD s = {0.0, 0.0};
+  perm::HostDD s = {0.0, 0.0};
   Partial tot{0.0, 0.0, 0.0, 0.0};
   for (const Partial& p : r->parts) {
-    s = dd_add(s, HDD{p.hi, p.lo});
+    s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
     tot.abs_sum += p.abs_sum;
     tot.sq_sum += p.sq_sum;
   }

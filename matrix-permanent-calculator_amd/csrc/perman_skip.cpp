@@ -1,0 +1,239 @@
+// perman_skip.cpp -- the sparse skipper (reference sparse ID 7,
+// gpu_perman64_xshared_coalescing_mshared_skipper, gpu_exact_sparse.cu:1123-1190)
+// on the zero-run skipping kernel (ryser_skip_impl.hpp).
+//
+// Inputs follow the reference's skipper: the dense matrix for x0 and the
+// CCS (entries > 0) for the column updates; its CRS (rptrs/cols) is accepted
+// and not needed.  Skipping relies on exact zeros of x, so it runs for
+// integer matrices (x exact); other matrices, and n below the smallest
+// skipping kernel, take the regular sparse path, which computes the same
+// sum.
+//
+// Host preparation: walk columns ordered greedily (next, the column adding
+// the fewest rows not yet touched; ties by fewer nonzeros, then index) so the
+// low columns touch few rows; rows ordered by the first walk column that
+// touches them (lev, ascending; rows untouched by the walked part get lev =
+// walk_bits).  Both orders leave the permanent unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/perman_hip.h"
+#include "perman_internal.hpp"
+#include "ryser_host.hpp"
+#include "ryser_kernels.hpp"
+
+namespace {
+
+#define HIP_TRYK(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return perm::fail(PERM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+struct SkipPrepared {
+  std::vector<double> x0, cs;
+  std::vector<int> lev;
+  int walk_bits = 0, high_bits = 0;
+};
+
+// Greedy column order and row levels; builds the permuted column store.
+void prepare_order(int n, const std::vector<double>& x0, const std::vector<double>& cols,
+                   int walk_bits, SkipPrepared& sp) {
+  const int B = n - 1;
+  std::vector<int> order;
+  std::vector<char> used(B, 0), touched(n, 0);
+  for (int step = 0; step < B; ++step) {
+    int best = -1, best_new = 0, best_nnz = 0;
+    for (int k = 0; k < B; ++k) {
+      if (used[k]) continue;
+      int nw = 0, nz = 0;
+      for (int j = 0; j < n; ++j)
+        if (cols[(size_t)k * n + j] != 0.0) {
+          ++nz;
+          nw += touched[j] ? 0 : 1;
+        }
+      if (best < 0 || nw < best_new || (nw == best_new && nz < best_nnz)) {
+        best = k;
+        best_new = nw;
+        best_nnz = nz;
+      }
+    }
+    used[best] = 1;
+    order.push_back(best);
+    for (int j = 0; j < n; ++j)
+      if (cols[(size_t)best * n + j] != 0.0) touched[j] = 1;
+  }
+  std::vector<int> lev(n, walk_bits);
+  for (int j = 0; j < n; ++j)
+    for (int c = 0; c < walk_bits; ++c)
+      if (cols[(size_t)order[c] * n + j] != 0.0) {
+        lev[j] = c;
+        break;
+      }
+  std::vector<int> rp(n);
+  std::iota(rp.begin(), rp.end(), 0);
+  std::stable_sort(rp.begin(), rp.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+  sp.x0.resize(n);
+  sp.lev.resize(n);
+  for (int r = 0; r < n; ++r) {
+    sp.x0[r] = x0[rp[r]];
+    sp.lev[r] = lev[rp[r]];
+  }
+  perm::build_column_store(
+      n, [&](int k, int r) { return cols[(size_t)order[k] * n + rp[r]]; }, sp.cs);
+}
+
+// One run on the device (mode 0 FP64, 1 double-double products).
+int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Partial& r,
+             double& walked_frac, float& ms) {
+  const uint64_t lanes = 1ull << sp.high_bits;
+  const uint64_t blocks = (lanes + 255) / 256;
+  const size_t cs_b = align256(sp.cs.size() * sizeof(double));
+  const size_t x_b = align256((size_t)n * sizeof(double));
+  const size_t lev_b = align256((size_t)n * sizeof(int));
+  const size_t part_b = align256(blocks * sizeof(perm::Partial));
+  const size_t walk_b = align256(blocks * 4 * sizeof(uint32_t));
+  const size_t res_b = align256(sizeof(perm::Partial));
+  char* d = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (d) (void)hipFree(d);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  struct Guard {
+    decltype(cleanup)& f;
+    ~Guard() { f(); }
+  } guard{cleanup};
+  HIP_TRYK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIP_TRYK(hipEventCreate(&e0));
+  HIP_TRYK(hipEventCreate(&e1));
+  HIP_TRYK(hipMalloc((void**)&d, cs_b + x_b + lev_b + part_b + walk_b + res_b));
+  perm::SkipLaunch L{};
+  L.colpm = (const double*)d;
+  L.x0 = (const double*)(d + cs_b);
+  L.row_level = (const int*)(d + cs_b + x_b);
+  L.partials = (perm::Partial*)(d + cs_b + x_b + lev_b);
+  L.walked = (uint32_t*)(d + cs_b + x_b + lev_b + part_b);
+  L.result = (perm::Partial*)(d + cs_b + x_b + lev_b + part_b + walk_b);
+  L.lanes = lanes;
+  L.blocks = blocks;
+  L.n = n;
+  L.walk_bits = sp.walk_bits;
+  L.high_bits = sp.high_bits;
+  L.check_bits = check_bits;
+  L.mode = mode;
+  HIP_TRYK(hipMemcpyAsync((void*)L.colpm, sp.cs.data(), sp.cs.size() * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+  HIP_TRYK(hipMemcpyAsync((void*)L.x0, sp.x0.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                          st));
+  HIP_TRYK(hipMemcpyAsync((void*)L.row_level, sp.lev.data(), n * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+  HIP_TRYK(hipEventRecord(e0, st));
+  HIP_TRYK(perm::launch_skip(L, st));
+  HIP_TRYK(hipEventRecord(e1, st));
+  std::vector<uint32_t> walked(blocks * 4);
+  HIP_TRYK(hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, st));
+  HIP_TRYK(hipMemcpyAsync(walked.data(), L.walked, walked.size() * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
+  HIP_TRYK(hipStreamSynchronize(st));
+  HIP_TRYK(hipEventElapsedTime(&ms, e0, e1));
+  // walked[] counts groups of 2^R steps per wave (64 lanes); lanes past
+  // `lanes` in the last block walk with their wave
+  const int R = perm::dense_unroll_bits(n);
+  double groups = 0.0;
+  for (uint32_t w : walked) groups += w;
+  const double waves = (double)blocks * 4.0;
+  walked_frac = groups * std::ldexp(1.0, R) / (waves * std::ldexp(1.0, sp.walk_bits));
+  return PERM_OK;
+}
+
+void fill(int n, const SkipPrepared& sp, const perm::Partial& r, int mode, perm_result* out) {
+  std::memset(out, 0, sizeof(*out));
+  const double scale = (double)(4 * (n & 1) - 2);
+  out->n = n;
+  out->sum_hi = r.hi;
+  out->sum_lo = r.lo;
+  out->value = scale * (r.hi + r.lo);
+  out->abs_sum = std::fabs(scale) * r.abs_sum;
+  out->err_bound = perm::error_statement(out->abs_sum, out->value, n, sp.walk_bits,
+                                         perm::dense_unroll_bits(n), true, mode);
+  out->err_est = std::min(out->err_bound,
+                          perm::error_estimate(scale * scale * r.sq_sum, out->value, n,
+                                               sp.walk_bits, sp.high_bits, mode, true));
+  out->subsets = std::ldexp(1.0, n - 1);
+  out->mode = mode;
+  out->integral = 1;
+  out->nu = n;
+    out->status = PERM_OK;
+  perm::round_integral(true, out->value, out->err_bound, out->err_est);
+}
+
+}  // namespace
+
+extern "C" {
+
+int perm_skip(int dtype, const void* mat, const int* rptrs, const int* cols, const int* cptrs,
+              const int* rows, const void* cvals, int nov, perm_result* out) {
+  (void)rptrs, (void)cols;
+  if (!out) return perm::fail(PERM_ERR_ARG, "null result");
+  std::vector<double> x0, cc;
+  bool integral = false;
+  if (int s = perm::gather_sparse(dtype, mat, cptrs, rows, cvals, nov, x0, cc, integral))
+    return s;
+  if (!integral || nov < perm::kMinSkipN)
+    return perm_sparse(dtype, mat, cptrs, rows, cvals, nov, 4, out);
+  if (int s = perm::select_device()) return s;
+  SkipPrepared sp;
+  const int R = perm::dense_unroll_bits(nov);
+  const perm::Split split =
+      perm::choose_skip_split(nov, R, perm::env_int("PERMAN_WALK_BITS", 0));
+  sp.walk_bits = split.walk_bits;
+  sp.high_bits = split.high_bits;
+  prepare_order(nov, x0, cc, sp.walk_bits, sp);
+  int check_bits = perm::env_int("PERMAN_SKIP_CHECK_BITS", 5);
+  check_bits = std::max(R, std::min(check_bits, sp.walk_bits));
+  perm::Partial r;
+  double walked = 0.0;
+  float ms = 0.f, ms2 = 0.f;
+  if (int s = run_skip(nov, sp, 0, check_bits, r, walked, ms)) return s;
+  fill(nov, sp, r, 0, out);
+  if (perm::needs_dd_pass(*out)) {
+    double walked2 = 0.0;
+    if (int s = run_skip(nov, sp, 1, check_bits, r, walked2, ms2)) return s;
+    fill(nov, sp, r, 1, out);
+  }
+  out->kernel_ms = (double)ms + (double)ms2;
+  out->main_ms = ms;
+  out->main_subsets = walked * out->subsets;  // subsets actually walked (rest skipped)
+  return perm::ok();
+}
+
+#define PERM_SKIP_DROPIN(SFX, T, DT)                                                        \
+  double perm_xshared_coalescing_mshared_skipper_##SFX(const T* mat, const int* rptrs,      \
+                                                        const int* cols, const int* cptrs,  \
+                                                        const int* rows, const T* cvals,    \
+                                                        int nov, int, int) {                \
+    perm_result r;                                                                          \
+    if (perm_skip(DT, mat, rptrs, cols, cptrs, rows, cvals, nov, &r) != PERM_OK)            \
+      return std::nan("");                                                                  \
+    return r.value;                                                                         \
+  }
+PERM_SKIP_DROPIN(i32, int, PERM_DTYPE_I32)
+PERM_SKIP_DROPIN(f32, float, PERM_DTYPE_F32)
+PERM_SKIP_DROPIN(f64, double, PERM_DTYPE_F64)
+
+}  // extern "C"

@@ -29,16 +29,33 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const Partial* __r
 
 int dense_unroll_bits(int n) { return n - 1 < 3 ? n - 1 : 3; }
 
-hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
-  if (L.n < 1 || L.n > kMaxDenseN) return hipErrorInvalidValue;
-  hipError_t e;
-  if (L.n <= 20) e = launch_dense_range_a(L, stream, main_done);
-  else if (L.n <= 30) e = launch_dense_range_b(L, stream, main_done);
-  else if (L.n <= 36) e = launch_dense_range_c(L, stream, main_done);
-  else if (L.n <= 42) e = launch_dense_range_d(L, stream, main_done);
-  else e = launch_dense_range_e(L, stream, main_done);
+hipError_t launch_skip_range_a(const SkipLaunch& L, hipStream_t s);
+hipError_t launch_skip_range_b(const SkipLaunch& L, hipStream_t s);
+
+hipError_t launch_skip(const SkipLaunch& L, hipStream_t stream) {
+  if (L.n < kMinSkipN || L.n > kMaxDenseN) return hipErrorInvalidValue;
+  hipError_t e = L.n <= 32 ? launch_skip_range_a(L, stream) : launch_skip_range_b(L, stream);
   if (e != hipSuccess) return e;
-  if (L.mode == 0 && L.ext_k >= 0 && (e = launch_extreme(L, stream)) != hipSuccess) return e;
+  reduce_partials_kernel<<<1, 256, 0, stream>>>(L.partials, L.blocks, L.result);
+  return hipGetLastError();
+}
+
+static hipError_t launch_passes(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  if (L.n < 1 || L.n > kMaxDenseN) return hipErrorInvalidValue;
+  if (L.n <= 20) return launch_dense_range_a(L, stream, main_done);
+  if (L.n <= 30) return launch_dense_range_b(L, stream, main_done);
+  if (L.n <= 36) return launch_dense_range_c(L, stream, main_done);
+  if (L.n <= 42) return launch_dense_range_d(L, stream, main_done);
+  return launch_dense_range_e(L, stream, main_done);
+}
+
+hipError_t launch_dense_pass_only(const DenseLaunch& L, hipStream_t stream) {
+  return launch_passes(L, stream, nullptr);
+}
+
+hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  hipError_t e = launch_passes(L, stream, main_done);
+  if (e != hipSuccess) return e;
   reduce_partials_kernel<<<1, 256, 0, stream>>>(L.partials, L.total_blocks, L.result);
   return hipGetLastError();
 }

@@ -82,20 +82,6 @@ struct LaneAcc {
   double p = 0.0, plo = 0.0, a = 0.0;
 };
 
-// Extreme-subset cut.  The few subsets S with |S| <= k or |S| >= n-1-k have
-// by far the largest terms (x_j near +-rowsum_j/2); their FP64 rounding and
-// X drift dominate the error of the whole sum.  The FP64 walk drops them
-// and ryser_extreme.hip sums them separately in double-double with exact X.
-// A lane's subset is (high pattern = lane id, walk pattern = gray(t)), so
-// |S| = pl + popcount(gray(t)) with pl = popcount(high pattern).  Per wave,
-// lo_thr = k - min(pl) and hi_thr = n-1-k - max(pl) bound the walk patterns
-// for which some lane may be extreme; waves that cannot meet one get
-// thresholds outside [0, w] and never take the per-lane test.
-struct ExtCut {
-  int lo_thr, hi_thr;  // scalar: test lanes when pw <= lo_thr or pw >= hi_thr
-  int lo, hi;          // per lane: drop when pw <= lo or pw >= hi (k - pl, n-1-k - pl)
-};
-
 // Product of the factors {seed if SEED} + x[LO..HI) as four interleaved
 // chains (independent work for the scheduler): F-1 multiplies for F factors.
 template <int N, int LO, int HI, bool SEED>
@@ -189,35 +175,26 @@ __device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], cons
   }
 }
 
-// Zero the term q of step t = g*2^R + V when this lane's subset is extreme.
-// popcount(gray(t)) is wave-uniform (scalar); only when some lane of the
-// wave can be extreme at this step does the per-lane test run.
-template <int R, int V>
-__device__ __forceinline__ double drop_extreme(double q, uint32_t g, const ExtCut& ec) {
-  const uint32_t t = (g << R) | (uint32_t)V;
-  const int pw = __builtin_popcount(t ^ (t >> 1));
-  if (pw <= ec.lo_thr || pw >= ec.hi_thr) {
-    if (pw <= ec.lo || pw >= ec.hi) q = 0.0;
-  }
-  return q;
-}
-
 // Step v of group g (step index t = g*2^R + v).  Phase A adds the column
 // that the previous step prefetched into SGPRs; phase B issues the scalar
 // loads of the next step's column and, while they are in flight, forms the
 // product and accumulates the term.  sched_barrier keeps the phases apart so
 // the loads always overlap a product.  With the pair variant (NU < N) odd
 // steps touch only x[0..NU) and reuse the passive product `ppass` formed on
-// the even step before them.
-template <int N, int R, int MODE, int NU, bool CHK, int V>
+// the even step before them.  NOADD0: step 0 of the group adds nothing (x
+// already holds its subset; cv is not read).
+template <int N, int R, int MODE, int NU, int V, bool NOADD0 = false>
 __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], double (&cv)[N],
                                             ColPtr cols, uint32_t g, uint32_t groups,
-                                            int zero_blk, LaneAcc& acc, double& ppass,
-                                            const ExtCut& ec) {
+                                            int zero_blk, LaneAcc& acc, double& ppass) {
   constexpr bool kPair = NU < N;
   if constexpr (V < (1 << R)) {
-    if constexpr (kPair && (V & 1)) add_column<N, MODE, NU>(x, xl, cv);
-    else add_column<N, MODE, N>(x, xl, cv);
+    if constexpr (NOADD0 && V == 0) {
+    } else if constexpr (kPair && (V & 1)) {
+      add_column<N, MODE, NU>(x, xl, cv);
+    } else {
+      add_column<N, MODE, N>(x, xl, cv);
+    }
     __builtin_amdgcn_sched_barrier(0);
     int blk;
     if constexpr (V + 1 < (1 << R)) {
@@ -241,34 +218,28 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
     else load_col<N, N>(cv, cols, blk);
     if constexpr (kPair) {
       if constexpr ((V & 1) == 0) ppass = chain_prod<N, NU, N, false>(x, 0.0);
-      double q = chain_prod<N, 0, NU - 1, true>(x, ppass);
-      if constexpr (CHK) q = drop_extreme<R, V>(q, g, ec);
-      add_term<(V & 1) != 0>(q, x[NU - 1], acc);
-    } else if constexpr (CHK && MODE == kFast && N > 1) {
-      const double q = drop_extreme<R, V>(chain_prod<N, 0, N - 1, false>(x, 0.0), g, ec);
-      add_term<(V & 1) != 0>(q, x[N - 1], acc);
+      add_term<(V & 1) != 0>(chain_prod<N, 0, NU - 1, true>(x, ppass), x[NU - 1], acc);
     } else {
       accumulate_term<N, (V & 1) != 0, MODE>(x, xl, acc);
     }
     __builtin_amdgcn_sched_barrier(0);
-    group_steps<N, R, MODE, NU, CHK, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass,
-                                            ec);
+    group_steps<N, R, MODE, NU, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
   }
 }
 
 // The Gray walk of one lane over its groups.  Every 32 groups (a run of
 // 2^(R+5) terms) the FP64 lane sum is folded into the double-double s and
 // the square of the run's sum of |terms| is added to q.
-template <int N, int R, int MODE, int NU, bool CHK>
+template <int N, int R, int MODE, int NU>
 __device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], ColPtr cols,
                                             uint32_t groups, int zero_blk, DD& s, LaneAcc& acc,
-                                            double& q, const ExtCut& ec) {
+                                            double& q) {
   double ppass = 1.0;
   double a_run = 0.0;  // acc.a at the start of the current run
   double cv[N];
   load_col<N, N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
   for (uint32_t g = 0; g < groups; ++g) {
-    group_steps<N, R, MODE, NU, CHK, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass, ec);
+    group_steps<N, R, MODE, NU, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
     if ((g & 31u) == 31u) {
       if constexpr (MODE == kFast) {  // fold into double-double
         s = dd_add_d(s, acc.p);
@@ -308,17 +279,13 @@ __device__ __forceinline__ void wave_min_max(int v, int& mn, int& mx) {
 // squared run sums (see walk_groups).
 // colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
 // Gray columns k < n-1, block 2(n-1) = zeros.  With NU < N, column 0 is zero
-// in rows [NU, N).  CHK (FP64 mode): the subsets with |S| <= ext_k or
-// |S| >= n-1-ext_k are left out (summed elsewhere); a wave tests its lanes
-// only at steps where one of them can be extreme.  Every lane of the wave
-// must take part (wave-wide min/max).
-template <int N, int R, int MODE, int NU, bool CHK>
+// in rows [NU, N).
+template <int N, int R, int MODE, int NU>
 __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0, uint64_t pattern,
-                                          int n_walk_bits, int n_high_bits, int ext_k, DD& s,
-                                          double& a, double& q) {
+                                          int n_walk_bits, int n_high_bits, DD& s, double& a,
+                                          double& q) {
   static_assert(NU == N || (MODE == kFast && R >= 1 && NU >= 1), "pair variant: FP64 mode only");
-  static_assert(!CHK || MODE == kFast, "extreme-subset cut: FP64 mode only");
   const ColPtr cols = (ColPtr)colpm;
   const int zero_blk = 2 * (N - 1);
 
@@ -347,15 +314,7 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
   q = 0.0;
   LaneAcc acc;
   const uint32_t groups = 1u << (n_walk_bits - R);
-  ExtCut ec{-1, 1 << 30, -1, 1 << 30};
-  if constexpr (CHK) {
-    const int pl = __builtin_popcountll(pattern);
-    const int hi = n_walk_bits + n_high_bits - ext_k;
-    int pmin, pmax;
-    wave_min_max(pl, pmin, pmax);
-    ec = {ext_k - pmin, hi - pmax, ext_k - pl, hi - pl};
-  }
-  walk_groups<N, R, MODE, NU, CHK>(x, xl, cols, groups, zero_blk, s, acc, q, ec);
+  walk_groups<N, R, MODE, NU>(x, xl, cols, groups, zero_blk, s, acc, q);
   if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
   else s = two_sum(acc.p, acc.plo);
   a = acc.a;
@@ -365,29 +324,20 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
   }
 }
 
-// Pattern of lane `lane` in a pass (see LanePass).
-__device__ __forceinline__ uint64_t lane_pattern(uint64_t lane, int n_high_bits, int free_bits,
-                                                 uint64_t class_mask) {
-  if (class_mask == ~0ull) return lane;
-  const uint64_t top = unrank_pattern(lane >> free_bits, n_high_bits - free_bits, class_mask);
-  return (top << free_bits) | (lane & ((1ull << free_bits) - 1ull));
-}
-
-// One pass of one permanent: lane = global thread id, one partial per
-// workgroup.
-template <int N, int R, int MODE, int NU, bool CHK>
+// One pass (or a block range of it) of one permanent: lane = lane_base +
+// global thread id, one partial per workgroup.
+template <int N, int R, int MODE, int NU>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
-                        Partial* __restrict__ partials, uint64_t lanes, int n_walk_bits,
-                        int n_high_bits, int free_bits, uint64_t class_mask, int ext_k) {
+                        Partial* __restrict__ partials, uint64_t lane_base, uint64_t lanes,
+                        int n_walk_bits, int n_high_bits) {
   __shared__ Partial scratch[256 / kWave];
-  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t pattern =
-      lane < lanes ? lane_pattern(lane, n_high_bits, free_bits, class_mask) : 0ull;
+  const uint64_t lane = lane_base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t pattern = lane < lanes ? lane : 0ull;  // high columns fixed by the lane id
   DD s;
   double a, q;
-  lane_walk<N, R, MODE, NU, CHK>(colpm, x0, pattern, n_walk_bits, n_high_bits, ext_k, s, a, q);
+  lane_walk<N, R, MODE, NU>(colpm, x0, pattern, n_walk_bits, n_high_bits, s, a, q);
   if (lane >= lanes) {
     s = {0.0, 0.0};
     a = 0.0;
@@ -401,25 +351,25 @@ constexpr int unroll_bits_for() {
   return N - 1 < 3 ? N - 1 : 3;
 }
 
-template <int N, int MODE, int NU, bool CHK>
+// Workgroups [block_begin, block_end) of pass P (all of them when
+// block_end == 0); partial of workgroup b goes to P.partials[b].
+template <int N, int MODE, int NU>
 hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
-  if (P.lanes == 0) return hipSuccess;
-  if (P.walk_bits < R) return hipErrorInvalidValue;
-  ryser_dense_kernel<N, R, MODE, NU, CHK><<<dim3((unsigned)P.blocks), dim3(256), 0, stream>>>(
-      L.colpm, L.x0, P.partials, P.lanes, P.walk_bits, P.high_bits, P.free_bits, P.class_mask,
-      L.ext_k);
+  const uint64_t b0 = P.block_begin, b1 = P.block_end ? P.block_end : P.blocks;
+  if (P.lanes == 0 || b1 <= b0) return hipSuccess;
+  if (P.walk_bits < R || b1 > P.blocks) return hipErrorInvalidValue;
+  ryser_dense_kernel<N, R, MODE, NU><<<dim3((unsigned)(b1 - b0)), dim3(256), 0, stream>>>(
+      L.colpm, L.x0, P.partials + b0, b0 * 256, P.lanes, P.walk_bits, P.high_bits);
   return hipGetLastError();
 }
 
-// FP64 passes: the main lanes, then (extreme-subset cut) the side lanes.
+// FP64 run: the FP64 kernel, pair variant NU.
 template <int N, int NU>
 hipError_t launch_fast(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
-  hipError_t e = launch_pass<N, kFast, NU, false>(L, L.main, stream);
+  hipError_t e = launch_pass<N, kFast, NU>(L, L.main, stream);
   if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
-  if (e != hipSuccess || L.ext_k < 0 || L.side.lanes == 0) return e;
-  if constexpr (N >= 2) return launch_pass<N, kFast, NU, true>(L, L.side, stream);
-  return hipErrorInvalidValue;
+  return e;
 }
 
 // Pair variants exist for NU = 4, 8, ..., N-4 when N >= kMinPairN.
@@ -432,15 +382,10 @@ hipError_t launch_pair(const DenseLaunch& L, hipStream_t stream, hipEvent_t main
 }
 
 template <int N>
-hipError_t launch_mode(const DenseLaunch& L, hipStream_t stream) {
-  if (L.mode == kDDProd) return launch_pass<N, kDDProd, N, false>(L, L.main, stream);
-  return launch_pass<N, kDDAll, N, false>(L, L.main, stream);
-}
-
-template <int N>
 hipError_t launch_n(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   if (L.mode == kDDProd || L.mode == kDDAll) {
-    hipError_t e = launch_mode<N>(L, stream);
+    hipError_t e = L.mode == kDDProd ? launch_pass<N, kDDProd, N>(L, L.main, stream)
+                                     : launch_pass<N, kDDAll, N>(L, L.main, stream);
     if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
     return e;
   }

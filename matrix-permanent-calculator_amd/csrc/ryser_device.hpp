@@ -26,44 +26,6 @@ struct DD {
   double hi, lo;
 };
 
-// Binomial coefficients C(c, p), 0 <= c < kBinomRows, in constant memory
-// (one copy per code object).
-constexpr int kBinomRows = 48;
-struct BinomTable {
-  uint64_t v[kBinomRows][kBinomRows + 1];
-};
-constexpr BinomTable make_binom() {
-  BinomTable t{};
-  for (int c = 0; c < kBinomRows; ++c)
-    for (int p = 0; p <= kBinomRows; ++p)
-      t.v[c][p] = p == 0 ? 1 : (c == 0 ? 0 : t.v[c - 1][p - 1] + t.v[c - 1][p]);
-  return t;
-}
-static __constant__ BinomTable kBinom = make_binom();
-
-// The r-th h-bit pattern among those whose popcount c has bit c of
-// class_mask set, ordered by c, then colex (bit positions as a c-subset).
-// Requires r < sum of C(h, c) over the selected classes, h < kBinomRows.
-__device__ __forceinline__ uint64_t unrank_pattern(uint64_t r, int h, uint64_t class_mask) {
-  int c = 0;
-  for (; c < h; ++c) {
-    if ((class_mask >> c) & 1ull) {
-      const uint64_t cnt = kBinom.v[h][c];
-      if (r < cnt) break;
-      r -= cnt;
-    }
-  }
-  uint64_t pattern = 0;
-  int pos = h - 1;
-  for (int i = c; i >= 1; --i) {
-    while (kBinom.v[pos][i] > r) --pos;  // largest pos with C(pos, i) <= r
-    pattern |= 1ull << pos;
-    r -= kBinom.v[pos][i];
-    --pos;
-  }
-  return pattern;
-}
-
 __device__ __forceinline__ DD two_sum(double a, double b) {
   double s = a + b;
   double bb = s - a;

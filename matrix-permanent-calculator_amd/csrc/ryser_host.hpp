@@ -4,6 +4,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -55,16 +56,47 @@ inline void build_column_store(int n, ColFn col, std::vector<double>& out) {
 }
 
 // Work split of the 2^(n-1) Gray subsets: every lane walks 2^walk_bits
-// subsets; 2^high_bits lanes.  Default: lanes = 2^min(21, n-11) (2^21
-// lanes fill the 256 CUs of an MI355X eight times over at 4 waves/SIMD),
-// with at least 2^10 steps per lane so the closed-form lane start stays a
-// small fraction of the walk.
+// subsets; 2^high_bits lanes.  The FP64 error of the walk is its X drift
+// (each x_j carries the roundings of the additions since the lane's start),
+// and the drift of all lanes moves the result the same way, scaled by the
+// lane sums -- so shorter walks give a more accurate sum.  Measured at
+// n = 40 (FP64 against the double-double mode): 2^18-step walks 2.3e-10,
+// 2^12-step walks 6e-12 relative error, in the same time.  Default:
+//   n <= 32: lanes = 2^min(21, n-11), at least 2^10 steps per lane;
+//   n >= 33: 2^12 steps per lane, at most 2^28 lanes (2^20 workgroup
+//            partials), longer walks beyond n = 41.
 struct Split {
   int walk_bits;
   int high_bits;
 };
 
 inline Split choose_split(int n, int unroll_bits, int walk_override) {
+  const int B = n - 1;
+  int h;
+  if (B <= 31) {
+    h = B - 10;
+    if (h > 21) h = 21;
+    if (h < 0) h = 0;
+  } else {
+    h = B - 12;
+    if (h > 28) h = 28;
+  }
+  int w = B - h;
+  if (walk_override > 0) {
+    w = walk_override > B ? B : walk_override;
+    h = B - w;
+  }
+  if (w < unroll_bits) {
+    w = unroll_bits;
+    h = B - w;
+  }
+  return {w, h};
+}
+
+// Split of the skipping kernel: long walks (the skips are aligned runs
+// inside a lane's walk) -- lanes = 2^min(21, n-11).  Integer matrices only,
+// where X is exact and there is no drift.
+inline Split choose_skip_split(int n, int unroll_bits, int walk_override) {
   const int B = n - 1;
   int h = B - 10;
   if (h > 21) h = 21;
@@ -79,92 +111,6 @@ inline Split choose_split(int n, int unroll_bits, int walk_override) {
     h = B - w;
   }
   return {w, h};
-}
-
-// Extreme-subset cut of the FP64 walk (summed in double-double instead, see
-// ryser_extreme.hip): the largest k with 2 sum_{i<=k} C(n-1, i) subsets
-// within min(2^21, 2^(n-7)) and 2k < n-1, for n >= 24; -1 (no cut) below.
-// `override_k` >= -1 forces a value (-1: off); -2 means automatic.
-constexpr int kMinExtN = 24;
-
-inline uint64_t binom_u64(int c, int p) {
-  if (p < 0 || p > c) return 0;
-  uint64_t r = 1;
-  for (int i = 1; i <= p; ++i) r = r * (uint64_t)(c - p + i) / (uint64_t)i;
-  return r;
-}
-
-inline uint64_t extreme_per_side(int n, int k) {
-  uint64_t m = 0;
-  for (int i = 0; i <= k; ++i) m += binom_u64(n - 1, i);
-  return m;
-}
-
-inline int choose_ext_k(int n, int max_k, int override_k) {
-  const int B = n - 1;
-  if (override_k >= -1) {
-    int k = override_k > max_k ? max_k : override_k;
-    while (k >= 0 && 2 * k >= B) --k;
-    return k;
-  }
-  if (n < kMinExtN) return -1;
-  const uint64_t budget = 1ull << (B - 6 < 21 ? B - 6 : 21);
-  int k = -1;
-  while (k + 1 <= max_k && 2 * (k + 1) < B && 2 * extreme_per_side(n, k + 1) <= budget) ++k;
-  return k;
-}
-
-// Lanes of the two FP64 passes under an extreme-subset cut k, for the split
-// (w, h) of n-1 = w + h columns.  A lane whose pattern has popcount c walks
-// subsets of sizes c .. c+w, so it can meet an extreme subset iff c <= k or
-// c + w >= n-1-k ("side" classes); the other classes form the main pass.
-// The side pass splits each side lane's walk into 2^free_bits lanes (free
-// bits taken from the top of the walk) so that its few lanes still fill the
-// GPU.
-struct PassGeometry {
-  uint64_t lanes, class_mask;
-  int walk_bits, high_bits, free_bits;
-};
-
-inline void cut_passes(int n, int w, int h, int k, int unroll_bits, PassGeometry& main,
-                       PassGeometry& side) {
-  const int B = n - 1;
-  uint64_t mid_mask = 0, side_mask = 0, mid = 0, sd = 0;
-  for (int c = 0; c <= h; ++c) {
-    if (c <= k || c + w >= B - k) {
-      side_mask |= 1ull << c;
-      sd += binom_u64(h, c);
-    } else {
-      mid_mask |= 1ull << c;
-      mid += binom_u64(h, c);
-    }
-  }
-  int f = w - unroll_bits;
-  if (f > 6) f = 6;
-  if (f < 0) f = 0;
-  main = {mid, mid_mask, w, h, 0};
-  side = {sd << f, side_mask, w - f, h + f, f};
-}
-
-// x_all = x0 + sum of all walk columns (block 2c of the column store), per
-// row as an unevaluated pair hi + lo formed with TwoSum: out[j] = hi,
-// out[n + j] = lo.
-inline void extreme_total_vector(int n, const std::vector<double>& cs,
-                                 const std::vector<double>& x0, std::vector<double>& out) {
-  out.assign(2 * (size_t)n, 0.0);
-  for (int j = 0; j < n; ++j) {
-    double hi = x0[j], lo = 0.0;
-    for (int c = 0; c < n - 1; ++c) {
-      const double v = cs[(size_t)(2 * c) * n + j];
-      const double s = hi + v;
-      const double bb = s - hi;
-      lo += (hi - (s - bb)) + (v - bb);
-      hi = s;
-    }
-    const double s = hi + lo;
-    out[j] = s;
-    out[n + j] = lo - (s - hi);
-  }
 }
 
 // Stated error of a result, with u = 2^-53 and A = |scale| * sum of |terms|
@@ -196,24 +142,52 @@ inline double error_statement(double abs_sum_scaled, double value, int n, int wa
   return abs_sum_scaled * u * ((double)n + 1.0 + drift + fold) + start;
 }
 
-// Expected error when the roundings of different terms behave as
-// independent random errors: per term about (n + sqrt(n)) u |T| (n product
-// roundings and the X drift), in total (n + sqrt(n)) u sqrt(sum T^2).  The
-// kernels report Q = sum over runs of 2^(R+5) consecutive terms of
-// (sum of |T| in the run)^2 >= sum T^2 (equal when one term dominates each
-// run), so the estimate uses sqrt(Q): it can exceed the typical error when
-// all terms are alike, never fall short of it by the run structure.
-// sq_scaled = scale^2 Q.  Double-double products (mode 1) leave only the X
+// Expected error (about three standard deviations) when roundings behave as
+// independent random errors.  Per term: the n product roundings give about
+// (n + sqrt(n)) u |T|; the drift of X (non-integer matrices) gives
+// sqrt(n) sqrt(2^w + h) u |T| -- each x_j carries a random walk of the
+// roundings of up to h + 2^w additions, and the drift is shared by
+// neighbouring terms of a lane, so it does not average out within a run.
+// Over terms: the kernels report Q = sum over runs of consecutive terms of
+// (sum of |T| in the run)^2 >= sum T^2, so sqrt(Q) stands for the root sum of
+// squares (it can exceed it when all terms are alike, never fall short).
+// sq_scaled = scale^2 Q.  Double-double products (mode 1) leave only the
 // drift of non-integer matrices; mode 2 has neither.  The rounding of x0
 // (non-integer matrices) adds about 2 sqrt(n) u |value| (rows independent).
-inline double error_estimate(double sq_scaled, double value, int n, int mode, bool integral) {
+// Calibrated against the exact oracle (n <= 26) and the double-double mode.
+inline double error_estimate(double sq_scaled, double value, int n, int walk_bits,
+                             int high_bits, int mode, bool integral) {
   const double u = std::ldexp(1.0, -53);
   const double dd = ((double)n + 8.0) * std::ldexp(1.0, -104);
   const double rms = std::sqrt(sq_scaled);
-  const double start = integral ? 0.0 : 2.0 * std::sqrt((double)n) * u * std::fabs(value);
-  if (mode == 2 || (mode == 1 && integral)) return dd * rms + start;
-  if (mode == 1) return (std::sqrt((double)n) * u + dd) * rms + start;
-  return ((double)n + std::sqrt((double)n)) * u * rms + start;
+  const double rn = std::sqrt((double)n);
+  const double start = integral ? 0.0 : 2.0 * rn * u * std::fabs(value);
+  const double drift =
+      integral ? 0.0 : rn * std::sqrt(std::ldexp(1.0, walk_bits) + (double)high_bits) * u;
+  if (mode == 2) return 3.0 * dd * rms + start;
+  if (mode == 1) return 3.0 * (drift + dd) * rms + start;
+  return 3.0 * (((double)n + rn) * u + drift) * rms + start;
+}
+
+// Double-double addition on the host (same steps as the device dd_add).
+struct HostDD {
+  double hi, lo;
+};
+inline HostDD host_two_sum(double a, double b) {
+  const double s = a + b;
+  const double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+inline HostDD host_dd_add(HostDD a, HostDD b) {
+  HostDD s = host_two_sum(a.hi, b.hi);
+  const HostDD t = host_two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  double h = s.hi + s.lo;
+  s.lo = s.lo - (h - s.hi);
+  s.hi = h;
+  s.lo += t.lo;
+  h = s.hi + s.lo;
+  return {h, s.lo - (h - s.hi)};
 }
 
 // Integer matrices: every x_j is an exact half-integer and the permanent an

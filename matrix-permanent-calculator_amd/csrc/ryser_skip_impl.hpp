@@ -1,0 +1,200 @@
+// ryser_skip_impl.hpp -- the Ryser walk with zero-run skipping (the
+// reference's sparse skipper, gpu_exact_sparse.cu:555-670 and :1123-1190,
+// CLI sparse ID 7), designed for wave-uniform lanes.
+//
+// For an integer matrix every x_j is exact, so a term is zero exactly when
+// some x_j(S) is zero.  The reference's skipper lets each thread jump past
+// the Gray steps that cannot change a zero row.  Lanes here must keep
+// flipping the same column, so the skip is taken per wave and per aligned
+// block of the walk:
+//
+//  * Within an aligned block of 2^m steps [t, t + 2^m) only the walk columns
+//    [0, m) change.  A row r whose first entry among the walk columns is at
+//    column lev[r] >= m keeps its x_r throughout the block; if x_r is zero at
+//    the block's first subset, all 2^m terms of the block are zero.
+//  * The host orders the rows by lev, ascending.  Every 2^S steps each lane
+//    finds its highest zero row j (so lev[j] is the largest level it could
+//    skip), the wave takes the minimum j over its lanes, and skips the block
+//    of level m = min(lev[j], alignment of t) if m >= S: two column
+//    additions replace 2^m steps (in Gray order the subset before the next
+//    block differs from the block's first subset only in column m-1; the
+//    next block's first step flips column m + ctz(t/2^m + 1)).
+//  * Otherwise the wave walks 2^S steps with the plain kernel's unrolled
+//    groups (ryser_dense_impl.hpp) and checks again.
+//
+// The host orders the walk columns so the low ones touch few rows (greedy:
+// next the column adding the fewest new rows), which makes lev large for
+// most rows.  Skipped terms are exactly zero, so the sum, its rounding and
+// the stated error are those of the plain walk over the same column order.
+#pragma once
+#include "ryser_dense_impl.hpp"
+
+namespace perm {
+
+typedef const int __attribute__((address_space(4)))* LevPtr;
+
+template <int N>
+__device__ __forceinline__ void load_col_n(double (&cv)[N], ColPtr base, int blk) {
+  load_col<N, N>(cv, base, blk);
+}
+
+// Column block of step t (t >= 1) of the Gray walk: column ctz(t), added
+// when bit ctz(t)+1 of t is clear, else subtracted.
+__device__ __forceinline__ int step_block(uint32_t t) {
+  const int c = __builtin_ctz(t);
+  return 2 * c + (int)((t >> (c + 1)) & 1u);
+}
+
+template <int N, int R, int MODE>
+__device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
+                                          const double* __restrict__ x0,
+                                          const int* __restrict__ row_level,
+                                          uint64_t pattern, bool valid, int n_walk_bits,
+                                          int n_high_bits, int check_bits, DD& s, double& a,
+                                          double& q, uint32_t& walked_groups) {
+  const ColPtr cols = (ColPtr)colpm;
+  const LevPtr lev = (LevPtr)row_level;
+  const int zero_blk = 2 * (N - 1);
+  double x[N], xl[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    x[j] = x0[j];
+    xl[j] = 0.0;
+  }
+  int parity = 0;
+  for (int b = 0; b < n_high_bits; ++b) {
+    ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
+    const int bit = (int)((pattern >> b) & 1);
+    parity ^= bit;
+    const double sel = bit ? 1.0 : 0.0;
+    double cv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
+    add_column<N, MODE, N>(x, xl, cv);
+  }
+
+  s = {0.0, 0.0};
+  q = 0.0;
+  walked_groups = 0;
+  LaneAcc acc;
+  const uint32_t steps = 1u << n_walk_bits;
+  const uint32_t groups = steps >> R;
+  const uint32_t chunk_groups = 1u << (check_bits - R);
+  double ppass = 1.0, a_run = 0.0;
+  double cv[N];
+  load_col_n<N>(cv, cols, zero_blk);  // step 0 changes nothing
+  uint32_t t = 0;
+  while (t < steps) {
+    // x = x(gray(t-1)), cv = the change of step t: apply it, x = x(gray(t)).
+    add_column<N, MODE, N>(x, xl, cv);
+    // highest zero row of this lane (rows are in ascending lev order)
+    int jz = -1;
+#pragma unroll
+    for (int j = 0; j < N; ++j) jz = x[j] == 0.0 ? j : jz;
+    if (!valid) jz = N - 1;
+    int jmin, jmax;
+    wave_min_max(jz, jmin, jmax);
+    const int align = t == 0 ? n_walk_bits : __builtin_ctz(t);
+    int m = jmin < 0 ? -1 : lev[jmin];
+    m = m < align ? m : align;
+    if (m >= check_bits) {
+      // skip [t, t + 2^m): x <- x(gray(t + 2^m - 1)), cv <- step t + 2^m
+      const uint32_t b = t >> m;
+      load_col_n<N>(cv, cols, 2 * (m - 1) + (int)(b & 1u));  // bit m-1 becomes !b_0
+      add_column<N, MODE, N>(x, xl, cv);
+      t += 1u << m;
+      load_col_n<N>(cv, cols, t < steps ? step_block(t) : zero_blk);
+      continue;
+    }
+    // walk 2^check_bits steps; the first adds nothing (x is already at t)
+    const uint32_t g0 = t >> R;
+    group_steps<N, R, MODE, N, 0, true>(x, xl, cv, cols, g0, groups, zero_blk, acc, ppass);
+    for (uint32_t g = g0 + 1; g < g0 + chunk_groups; ++g) {
+      group_steps<N, R, MODE, N, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+      if constexpr (MODE == kFast) {
+        if ((g & 31u) == 31u) {  // at least every 32 groups
+          s = dd_add_d(s, acc.p);
+          acc.p = 0.0;
+        }
+      }
+    }
+    if constexpr (MODE == kFast) {  // and at the end of every chunk
+      s = dd_add_d(s, acc.p);
+      acc.p = 0.0;
+    }
+    walked_groups += chunk_groups;
+    const double d = acc.a - a_run;
+    q = __builtin_fma(d, d, q);
+    a_run = acc.a;
+    t += 1u << check_bits;
+  }
+  if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
+  else s = dd_add(s, two_sum(acc.p, acc.plo));
+  a = acc.a;
+  if (parity) {
+    s.hi = -s.hi;
+    s.lo = -s.lo;
+  }
+}
+
+// One skipping walk per lane over the plain split (pattern = lane id).
+// partials: one record per workgroup; walked: groups of 2^R steps walked per
+// wave (the rest was skipped), one slot per wave, for the statistics.
+template <int N, int R, int MODE>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
+void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
+                       const int* __restrict__ row_level, Partial* __restrict__ partials,
+                       uint32_t* __restrict__ walked, uint64_t lanes, int n_walk_bits,
+                       int n_high_bits, int check_bits) {
+  __shared__ Partial scratch[256 / kWave];
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = lane < lanes;
+  DD s;
+  double a, q;
+  uint32_t wg;
+  skip_walk<N, R, MODE>(colpm, x0, row_level, valid ? lane : 0ull, valid, n_walk_bits,
+                        n_high_bits, check_bits, s, a, q, wg);
+  if (!valid) {
+    s = {0.0, 0.0};
+    a = 0.0;
+    q = 0.0;
+  }
+  block_reduce_store(s, a, q, scratch, partials + blockIdx.x);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    walked[blockIdx.x * (256 / kWave) + threadIdx.x / kWave] = wg;
+}
+
+template <int N>
+hipError_t launch_skip_n(const SkipLaunch& L, hipStream_t stream) {
+  constexpr int R = unroll_bits_for<N>();
+  if (L.walk_bits < R || L.check_bits < R || L.check_bits > L.walk_bits)
+    return hipErrorInvalidValue;
+  if (L.mode == kFast)
+    ryser_skip_kernel<N, R, kFast><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
+        L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
+        L.check_bits);
+  else if (L.mode == kDDProd)
+    ryser_skip_kernel<N, R, kDDProd><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
+        L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
+        L.check_bits);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace perm
+
+#define PERM_SKIP_RANGE(TAG, LO, HI)                                                          \
+  namespace perm {                                                                            \
+  template <int... I>                                                                         \
+  static hipError_t skip_dispatch_##TAG(const SkipLaunch& L, hipStream_t s,                   \
+                                        std::integer_sequence<int, I...>) {                   \
+    hipError_t e = hipErrorInvalidValue;                                                      \
+    ((L.n == LO + I ? (e = launch_skip_n<LO + I>(L, s), 0) : 0), ...);                        \
+    return e;                                                                                 \
+  }                                                                                           \
+  hipError_t launch_skip_range_##TAG(const SkipLaunch& L, hipStream_t s) {                    \
+    return skip_dispatch_##TAG(L, s, std::make_integer_sequence<int, HI - LO + 1>{});         \
+  }                                                                                           \
+  }

@@ -7,6 +7,7 @@
 //   GPU dense  -p 0..4   gpu_perman64_{xglobal,xlocal,xshared,
 //                        xshared_coalescing,xshared_coalescing_mshared}
 //   GPU sparse -s -p 1..4  the same names with the _sparse suffix
+//   GPU sparse -s -p 7     gpu_perman64_xshared_coalescing_mshared_skipper
 //   CPU dense  -c        parallel_perman64 (any -p, main.cu:188-192)
 //   CPU sparse -c -s -p1 parallel_perman64_sparse
 // The result line keeps the reference's shape, `Result: <name> <perm> in
@@ -81,11 +82,15 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
       }
       name = kDenseNames[o.algo];
       st = perm_dense(dt, mat.data(), n, o.algo, &r);
+    } else if (o.algo == 7) {  // the skipper (main.cu:138-142)
+      name = "gpu_perman64_xshared_coalescing_mshared_skipper";
+      st = perm_skip(dt, mat.data(), c.rptrs.data(), c.cols.data(), c.cptrs.data(),
+                     c.rows.data(), c.cvals.data(), n, &r);
     } else {
       if (o.algo < 1 || o.algo > 4) {
-        std::printf(o.algo >= 5 && o.algo != 7 ? "Algorithm ID %d (multi-GPU / hybrid) is not "
-                                                 "part of this single-GPU build\n"
-                                               : "Unknown Algorithm ID\n",
+        std::printf(o.algo >= 5 ? "Algorithm ID %d (multi-GPU / hybrid) is not "
+                                  "part of this single-GPU build\n"
+                                : "Unknown Algorithm ID\n",
                     o.algo);
         return 2;
       }

@@ -39,7 +39,7 @@ class PermResultC(ctypes.Structure):
                 ("err_bound", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
                 ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
-                ("nu", ctypes.c_int), ("ext_k", ctypes.c_int), ("main_ms", ctypes.c_double),
+                ("nu", ctypes.c_int), ("main_ms", ctypes.c_double),
                 ("main_subsets", ctypes.c_double)]
 
 
@@ -57,14 +57,13 @@ class PermResult:
     integral: int
     err_est: float
     nu: int
-    ext_k: int
     main_ms: float
     main_subsets: float
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
-                   r.n, r.mode, r.integral, r.err_est, r.nu, r.ext_k, r.main_ms,
+                   r.n, r.mode, r.integral, r.err_est, r.nu, r.main_ms,
                    r.main_subsets)
 
 
@@ -148,6 +147,12 @@ def lib():
         L.perm_server_destroy.argtypes = [vp]
         L.perm_server_destroy.restype = None
         L.perm_connect.restype = None
+        L.perm_skip.argtypes = [ctypes.c_int, vp, ip, ip, ip, ip, vp, ctypes.c_int,
+                                ctypes.POINTER(PermResultC)]
+        for sfx in _SFX.values():
+            f = getattr(L, f"perm_xshared_coalescing_mshared_skipper_{sfx}")
+            f.restype = ctypes.c_double
+            f.argtypes = [vp, ip, ip, ip, ip, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.perm_read_calculate_return.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5
         L.perm_read_calculate_return.restype = ctypes.c_double
         L.perm_matlab_calculate_return.argtypes = [ip] + [ctypes.c_int] * 7
@@ -341,6 +346,41 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _crs(a: np.ndarray):
+    """CRS (rptrs, cols, rvals) with the reference's > 0 filter."""
+    rptrs, cols, rvals = [0], [], []
+    for i in range(a.shape[0]):
+        for j in range(a.shape[1]):
+            if a[i, j] > 0:
+                cols.append(j)
+                rvals.append(a[i, j])
+        rptrs.append(len(cols))
+    return (np.asarray(rptrs, dtype=np.int32), np.asarray(cols, dtype=np.int32),
+            np.asarray(rvals, dtype=a.dtype))
+
+
+def skip(mat) -> PermResult:
+    """Sparse skipper (ID 7, gpu_exact_sparse.cu:1124): zero-run skipping for
+    integer matrices (n >= 16), the sparse path otherwise.  main_subsets of
+    the result = subsets actually walked."""
+    a, dt = _mat(mat)
+    cptrs, rows, cvals = compress(a)
+    rptrs, cols, _ = _crs(a)
+    r = PermResultC()
+    _check(lib().perm_skip(dt, a.ctypes.data, _ip(rptrs), _ip(cols), _ip(cptrs), _ip(rows),
+                           cvals.ctypes.data, a.shape[0], ctypes.byref(r)))
+    return PermResult.from_c(r)
+
+
+def dropin_skipper(mat) -> float:
+    a, dt = _mat(mat)
+    cptrs, rows, cvals = compress(a)
+    rptrs, cols, _ = _crs(a)
+    f = getattr(lib(), f"perm_xshared_coalescing_mshared_skipper_{_SFX[dt]}")
+    return f(a.ctypes.data, _ip(rptrs), _ip(cols), _ip(cptrs), _ip(rows), cvals.ctypes.data,
+             a.shape[0], 2048, 256)
 
 
 def read_calculate_return(filename: str, algorithm: int, nt: int = 16, x: int = 0, y: int = 0,

@@ -1,0 +1,60 @@
+"""Accuracy of the FP64 walk and its stated error.
+
+* the statistical error estimate covers the observed error against the
+  exact oracle, and never exceeds the stated bound;
+* matrices with mixed signs against the exact oracle;
+* every work split (walk length per lane) gives the permanent within the
+  stated error, and the default split at n = 36 stays within 1e-11 of the
+  double-double mode (the walk's error is its X drift, which shrinks with
+  the walk length)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import permanent_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+@pytest.mark.parametrize("n,density", [(18, 0.5), (20, 0.9), (22, 0.3), (24, 0.6), (25, 0.5)])
+def test_error_estimate_covers_observed(n, density):
+    a = P.synthetic_matrix(n, density, seed=900 + n)
+    ex = float(O.exact_permanent(a))
+    r = P.dense(a, 4)
+    assert abs(r.value - ex) <= r.err_est, (n, r.value, ex, r.err_est)
+    assert r.err_est <= r.err_bound
+
+
+@pytest.mark.parametrize("n", [13, 17, 21])
+def test_mixed_signs_vs_exact(n):
+    rng = np.random.default_rng(40 + n)
+    a = rng.uniform(-1.0, 1.0, size=(n, n))
+    ex = float(O.exact_permanent(a))
+    r = P.dense(a, 4)
+    assert abs(r.value - ex) <= max(1e-11 * abs(ex), r.err_bound), (r.value, ex)
+
+
+@pytest.mark.parametrize("walk_bits", [0, 4, 8, 12, 16, 20])
+def test_every_split_within_stated_error(monkeypatch, walk_bits):
+    a = P.synthetic_matrix(24, 0.5, seed=24)
+    ex = float(O.exact_permanent(a))
+    if walk_bits:
+        monkeypatch.setenv("PERMAN_WALK_BITS", str(walk_bits))
+    r = P.dense(a, 4)
+    assert abs(r.value - ex) <= r.err_bound
+    assert rel(r.value, ex) < 1e-12
+
+
+def test_default_split_n36_vs_double_double():
+    a = P.synthetic_matrix(36, 0.5, seed=2)
+    plan = P.Plan(a)
+    fast = plan.run(0)
+    ref = plan.run(2)
+    plan.close()
+    print(f"n=36 fast={fast.value!r} dd={ref.value!r} rel={rel(fast.value, ref.value):.3e}")
+    assert rel(fast.value, ref.value) < 1e-11
+    assert abs(fast.value - ref.value) <= fast.err_est

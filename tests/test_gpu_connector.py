@@ -24,7 +24,7 @@ def write_matrix(path, a, type_name):
 def test_read_calculate_return(tmp_path, type_name, algo):
     a = P.synthetic_matrix(14, 0.6, seed=61)
     if type_name == "int":
-        a = np.round(a * 7).astype(np.int32)
+        a = np.round(a * 3).astype(np.int32)
     elif type_name == "float":
         a = a.astype(np.float32)
     f = tmp_path / f"m_{type_name}.txt"
@@ -33,7 +33,7 @@ def test_read_calculate_return(tmp_path, type_name, algo):
     ex = float(O.exact_permanent(np.asarray(parsed, dtype=np.float64)))
     v = P.read_calculate_return(str(f), algo)
     assert v == pytest.approx(ex, rel=1e-12), (type_name, algo, v, ex)
-    if type_name == "int":
+    if type_name == "int" and abs(ex) < 2 ** 53:
         assert v == ex
 
 
