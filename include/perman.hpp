@@ -6,7 +6,13 @@
 //   gpu_exact_dense.cu:402,467,525,583,641   gpu_perman64_<variant>(T*, int, int, int)
 //   gpu_exact_sparse.cu:674,734,794,854      gpu_perman64_<variant>_sparse(...)
 //   gpu_exact_sparse.cu:1124                 gpu_perman64_xshared_coalescing_mshared_skipper
+//   gpu_exact_dense.cu:702,777,914           ..._multigpu, _multigpucpu_chunks,
+//                                            _multigpu_manual_distribution
+//   gpu_exact_sparse.cu:917,996,1193,1328    ..._multigpu_sparse, _multigpucpu_chunks_sparse,
+//                                            _multigpucpu_chunks_skipper,
+//                                            _multigpu_sparse_manual_distribution
 //   algo.h:663, algo.h:569                   parallel_perman64 / _sparse
+//   algo.h:749, algo.h:886                   parallel_skip_perman64_w / _w_balanced
 // for T in {int, float, double}.  A caller that included the reference's .cu
 // files replaces those includes with this header and links
 // -lperman_hip.  Results are returned by value as double (NaN on error; see
@@ -72,6 +78,48 @@ inline double skipper(double* m, int* rp, int* c, int* cp, int* r, double* cv, i
                       int b) {
   return perm_xshared_coalescing_mshared_skipper_f64(m, rp, c, cp, r, cv, n, g, b);
 }
+
+#define PERMAN_TYPED_OVERLOADS(T, SFX)                                                        \
+  inline double multigpu(T* m, int n, int g, int gd, int bd) {                                \
+    return perm_xshared_coalescing_mshared_multigpu_dense_##SFX(m, n, g, gd, bd);             \
+  }                                                                                           \
+  inline double multigpu_manual(T* m, int n, int g, int gd, int bd) {                         \
+    return perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_##SFX(m, n, g,  \
+                                                                                   gd, bd);   \
+  }                                                                                           \
+  inline double chunks(T* m, int n, int g, bool cpu, int t, int gd, int bd) {                 \
+    return perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_##SFX(m, n, g, cpu, t,    \
+                                                                         gd, bd);             \
+  }                                                                                           \
+  inline double multigpu_sparse(T* m, int* cp, int* r, T* cv, int n, int g, int gd, int bd) { \
+    return perm_xshared_coalescing_mshared_multigpu_sparse_##SFX(m, cp, r, cv, n, g, gd, bd); \
+  }                                                                                           \
+  inline double multigpu_sparse_manual(T* m, int* cp, int* r, T* cv, int n, int g, int gd,    \
+                                       int bd) {                                              \
+    return perm_xshared_coalescing_mshared_multigpu_sparse_manual_distribution_##SFX(         \
+        m, cp, r, cv, n, g, gd, bd);                                                          \
+  }                                                                                           \
+  inline double chunks_sparse(T* m, int* cp, int* r, T* cv, int n, int g, bool cpu, int t,    \
+                              int gd, int bd) {                                               \
+    return perm_xshared_coalescing_mshared_multigpucpu_chunks_sparse_##SFX(m, cp, r, cv, n,   \
+                                                                          g, cpu, t, gd, bd); \
+  }                                                                                           \
+  inline double chunks_skipper(T* m, int* rp, int* c, int* cp, int* r, T* cv, int n, int g,  \
+                               bool cpu, int t, int gd, int bd) {                             \
+    return perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_##SFX(                  \
+        m, rp, c, cp, r, cv, n, g, cpu, t, gd, bd);                                           \
+  }                                                                                           \
+  inline double cpu_skip(int* rp, int* c, T* rv, int* cp, int* r, T* cv, int n, int t) {      \
+    return perm_parallel_skip_perman64_w_##SFX(rp, c, rv, cp, r, cv, n, t);                   \
+  }                                                                                           \
+  inline double cpu_skip_balanced(int* rp, int* c, T* rv, int* cp, int* r, T* cv, int n,      \
+                                  int t) {                                                    \
+    return perm_parallel_skip_perman64_w_balanced_##SFX(rp, c, rv, cp, r, cv, n, t);          \
+  }
+PERMAN_TYPED_OVERLOADS(int, i32)
+PERMAN_TYPED_OVERLOADS(float, f32)
+PERMAN_TYPED_OVERLOADS(double, f64)
+#undef PERMAN_TYPED_OVERLOADS
 }  // namespace perman_detail
 
 // ---- dense, algorithm IDs 0-4 -------------------------------------------
@@ -128,6 +176,55 @@ double gpu_perman64_xshared_coalescing_mshared_skipper(T* mat, int* rptrs, int* 
   return perman_detail::skipper(mat, rptrs, cols, cptrs, rows, cvals, nov, grid_dim, block_dim);
 }
 
+// ---- multi-GPU / hybrid IDs 5, 6, 66 (dense) and 5, 6, 8, 66 (sparse) -----
+// One GPU computes the whole range; `cpu` adds the CPU worker of the chunked
+// scheduler (IDs 6).
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpu(T* mat, int nov, int gpu_num,
+                                                        int grid_dim, int block_dim) {
+  return perman_detail::multigpu(mat, nov, gpu_num, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks(T* mat, int nov, int gpu_num,
+                                                                  bool cpu, int threads,
+                                                                  int grid_dim, int block_dim) {
+  return perman_detail::chunks(mat, nov, gpu_num, cpu, threads, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpu_manual_distribution(T* mat, int nov,
+                                                                            int gpu_num,
+                                                                            int grid_dim,
+                                                                            int block_dim) {
+  return perman_detail::multigpu_manual(mat, nov, gpu_num, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpu_sparse(T* mat, int* cptrs, int* rows,
+                                                               T* cvals, int nov, int gpu_num,
+                                                               int grid_dim, int block_dim) {
+  return perman_detail::multigpu_sparse(mat, cptrs, rows, cvals, nov, gpu_num, grid_dim,
+                                        block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_sparse(
+    T* mat, int* cptrs, int* rows, T* cvals, int nov, int gpu_num, bool cpu, int threads,
+    int grid_dim, int block_dim) {
+  return perman_detail::chunks_sparse(mat, cptrs, rows, cvals, nov, gpu_num, cpu, threads,
+                                      grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_skipper(
+    T* mat, int* rptrs, int* cols, int* cptrs, int* rows, T* cvals, int nov, int gpu_num,
+    bool cpu, int threads, int grid_dim, int block_dim) {
+  return perman_detail::chunks_skipper(mat, rptrs, cols, cptrs, rows, cvals, nov, gpu_num, cpu,
+                                       threads, grid_dim, block_dim);
+}
+template <class T>
+double gpu_perman64_xshared_coalescing_mshared_multigpu_sparse_manual_distribution(
+    T* mat, int* cptrs, int* rows, T* cvals, int nov, int gpu_num, int grid_dim, int block_dim) {
+  return perman_detail::multigpu_sparse_manual(mat, cptrs, rows, cvals, nov, gpu_num, grid_dim,
+                                               block_dim);
+}
+
 // ---- CPU (`-c`) ------------------------------------------------------------
 template <class T>
 double parallel_perman64(T* mat, int nov, int threads) {
@@ -136,6 +233,16 @@ double parallel_perman64(T* mat, int nov, int threads) {
 template <class T>
 double parallel_perman64_sparse(T* mat, int* cptrs, int* rows, T* cvals, int nov, int threads) {
   return perman_detail::cpu_sparse(mat, cptrs, rows, cvals, nov, threads);
+}
+template <class T>
+double parallel_skip_perman64_w(int* rptrs, int* cols, T* rvals, int* cptrs, int* rows,
+                                T* cvals, int nov, int threads) {
+  return perman_detail::cpu_skip(rptrs, cols, rvals, cptrs, rows, cvals, nov, threads);
+}
+template <class T>
+double parallel_skip_perman64_w_balanced(int* rptrs, int* cols, T* rvals, int* cptrs, int* rows,
+                                         T* cvals, int nov, int threads) {
+  return perman_detail::cpu_skip_balanced(rptrs, cols, rvals, cptrs, rows, cvals, nov, threads);
 }
 
 #endif  // PERMAN_HPP

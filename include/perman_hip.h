@@ -221,7 +221,8 @@ double perm_xshared_coalescing_mshared_skipper_f64(const double* mat, const int*
  * whole Gray range.  The chunked forms (ID 6) cut the FP64 walk into
  * 2^(n-29) chunks (2^(n-28) with cpu != 0, as the reference) handed out by a
  * shared counter to the GPU and, with cpu != 0, to a CPU worker using
- * `threads` OpenMP threads; partial sums are combined in chunk order.
+ * `threads` OpenMP threads; the partial sums are combined by the same grid
+ * reduction as the one-launch path.
  * Without the CPU worker the value equals the one-launch value bitwise; with
  * it, the chunks the CPU took differ in rounding only.  ID 8 runs the skipper
  * on the GPU alone.  perm_chunked is the full-result form (cptrs NULL:
@@ -355,6 +356,39 @@ double perm_parallel_perman64_sparse_f32(const float* mat, const int* cptrs, con
                                          const float* cvals, int nov, int threads);
 double perm_parallel_perman64_sparse_f64(const double* mat, const int* cptrs, const int* rows,
                                          const double* cvals, int nov, int threads);
+
+/* ---- CPU skip variants of `perman -c -s -p2/-p3` (parallel_skip_perman64_w,
+ * algo.h:749; parallel_skip_perman64_w_balanced, algo.h:886).  The Gray walk
+ * jumps over runs of subsets in which some row sum x_j is exactly zero (x_j
+ * stays zero until a column touching row j flips); both names run the same
+ * dynamically scheduled walk over aligned blocks, combined in block order.
+ * The CRS (rptrs/cols) must list the same entries as the CCS; rvals is
+ * accepted for the signature.  perm_cpu_skip is the full-result form:
+ * main_subsets = the terms actually evaluated, kernel_ms = host wall time. */
+int perm_cpu_skip(int dtype, const int* rptrs, const int* cols, const void* rvals,
+                  const int* cptrs, const int* rows, const void* cvals, int nov, int threads,
+                  perm_result* out);
+double perm_parallel_skip_perman64_w_i32(const int* rptrs, const int* cols, const int* rvals,
+                                         const int* cptrs, const int* rows, const int* cvals,
+                                         int nov, int threads);
+double perm_parallel_skip_perman64_w_f32(const int* rptrs, const int* cols, const float* rvals,
+                                         const int* cptrs, const int* rows, const float* cvals,
+                                         int nov, int threads);
+double perm_parallel_skip_perman64_w_f64(const int* rptrs, const int* cols, const double* rvals,
+                                         const int* cptrs, const int* rows, const double* cvals,
+                                         int nov, int threads);
+double perm_parallel_skip_perman64_w_balanced_i32(const int* rptrs, const int* cols,
+                                                  const int* rvals, const int* cptrs,
+                                                  const int* rows, const int* cvals, int nov,
+                                                  int threads);
+double perm_parallel_skip_perman64_w_balanced_f32(const int* rptrs, const int* cols,
+                                                  const float* rvals, const int* cptrs,
+                                                  const int* rows, const float* cvals, int nov,
+                                                  int threads);
+double perm_parallel_skip_perman64_w_balanced_f64(const int* rptrs, const int* cols,
+                                                  const double* rvals, const int* cptrs,
+                                                  const int* rows, const double* cvals, int nov,
+                                                  int threads);
 
 #ifdef __cplusplus
 }

@@ -291,9 +291,9 @@ int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
 // the GPU worker: it takes runs of chunks large enough to fill the GPU and
 // launches the walk over their block range.  With `use_cpu` a second
 // thread takes one chunk at a time and computes its lanes on the host with
-// `threads` OpenMP threads.  When the counter is exhausted all partial
-// records come back and are summed on the host in block order, CPU chunks in
-// place of theirs.
+// `threads` OpenMP threads.  When the counter is exhausted the CPU chunks'
+// sums are written over their blocks' partial records and the device grid
+// reduction sums all records, in the one-launch order.
 // Without the CPU worker the result equals the one-launch result bitwise;
 // with it, chunks the CPU took differ in rounding only.
 int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_result* out,
@@ -331,7 +331,6 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
       }
     });
   }
-  int st = PERM_OK;
   hipError_t e = hipEventRecord(P.ev0, P.stream);
   for (;;) {
     if (e != hipSuccess) break;
@@ -344,35 +343,30 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
     e = perm::launch_dense_pass_only(Lc, P.stream);
   }
   if (cpu.joinable()) cpu.join();
-  std::vector<perm::Partial> parts(L.total_blocks);
   if (e == hipSuccess) e = hipEventRecord(P.ev1, P.stream);
+  // the CPU's chunks: its sum in the chunk's first record, zeros in the rest,
+  // then the device reduction over all records, in the one-launch order
+  std::vector<perm::Partial> staged(cpu_ids.size() * chunk_blocks, perm::Partial{0.0, 0.0, 0.0, 0.0});
+  for (size_t i = 0; i < cpu_ids.size() && e == hipSuccess; ++i) {
+    const uint64_t b0 = cpu_ids[i] * chunk_blocks, b1 = std::min(mb, b0 + chunk_blocks);
+    perm::Partial* h = staged.data() + i * chunk_blocks;
+    h[0] = {cpu_part[i][0], cpu_part[i][1], cpu_part[i][2], cpu_part[i][3]};
+    e = hipMemcpyAsync(L.partials + b0, h, (b1 - b0) * sizeof(perm::Partial),
+                       hipMemcpyHostToDevice, P.stream);
+  }
+  perm::Partial r{0.0, 0.0, 0.0, 0.0};
+  if (e == hipSuccess) e = perm::launch_reduce(L.partials, L.total_blocks, L.result, P.stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(parts.data(), L.partials, parts.size() * sizeof(perm::Partial),
-                       hipMemcpyDeviceToHost, P.stream);
+    e = hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, P.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(P.stream);
   if (e != hipSuccess)
     return fail(PERM_ERR_HIP, std::string("chunked run: ") + hipGetErrorString(e));
-  for (size_t i = 0; i < cpu_ids.size(); ++i) {
-    const uint64_t b0 = cpu_ids[i] * chunk_blocks, b1 = std::min(mb, b0 + chunk_blocks);
-    for (uint64_t b = b0; b < b1; ++b) parts[b] = {0.0, 0.0, 0.0, 0.0};
-    parts[b0] = {cpu_part[i][0], cpu_part[i][1], cpu_part[i][2], cpu_part[i][3]};
-  }
-  perm::HostDD s{0.0, 0.0};
-  perm::Partial r{0.0, 0.0, 0.0, 0.0};
-  for (const perm::Partial& p : parts) {
-    s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
-    r.abs_sum += p.abs_sum;
-    r.sq_sum += p.sq_sum;
-  }
-  r.hi = s.hi;
-  r.lo = s.lo;
   fill_result(&P, r, 0, out);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, P.ev0, P.ev1);
   out->kernel_ms = ms;
   if (cpu_chunks_out) *cpu_chunks_out = (int)cpu_ids.size();
   if (perm::needs_dd_pass(*out)) return plan_run_impl(&P, 1, out);
-  (void)st;
   return ok();
 }
 

@@ -53,6 +53,11 @@ hipError_t launch_dense_pass_only(const DenseLaunch& L, hipStream_t stream) {
   return launch_passes(L, stream, nullptr);
 }
 
+hipError_t launch_reduce(const Partial* in, uint64_t count, Partial* out, hipStream_t stream) {
+  reduce_partials_kernel<<<1, 256, 0, stream>>>(in, count, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   hipError_t e = launch_passes(L, stream, main_done);
   if (e != hipSuccess) return e;

@@ -63,6 +63,9 @@ hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t mai
 // Only the walk of L (its block range), no reduction (the caller sums the
 // partial records).
 hipError_t launch_dense_pass_only(const DenseLaunch& L, hipStream_t stream);
+// The grid reduction alone: the DD sum of in[0, count) into *out (the same
+// order as launch_dense's).
+hipError_t launch_reduce(const Partial* in, uint64_t count, Partial* out, hipStream_t stream);
 // Unrolled inner steps per group (log2) used for a given n.
 int dense_unroll_bits(int n);
 

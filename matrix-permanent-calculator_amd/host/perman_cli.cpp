@@ -8,12 +8,18 @@
 //                        xshared_coalescing,xshared_coalescing_mshared}
 //   GPU sparse -s -p 1..4  the same names with the _sparse suffix
 //   GPU sparse -s -p 7     gpu_perman64_xshared_coalescing_mshared_skipper
+//   GPU dense  -p 5, 66 / sparse -s -p 5, 66   the multi-GPU IDs: this build
+//                        drives one GPU, which computes the whole range
+//   GPU dense  -p 6 / sparse -s -p 6   the chunk scheduler; -g -c adds the
+//                        CPU worker with -t threads (main.cu:62-66)
+//   GPU sparse -s -p 8   the chunked skipper: the skipper on the GPU
 //   CPU dense  -c        parallel_perman64 (any -p, main.cu:188-192)
 //   CPU sparse -c -s -p1 parallel_perman64_sparse
+//   CPU sparse -c -s -p2, -p3  parallel_skip_perman64_w / _w_balanced
 // The result line keeps the reference's shape, `Result: <name> <perm> in
 // <seconds>`, printed with 17 significant digits, followed by the stated
-// error of the value.  Approximation (-a), multi-GPU/hybrid IDs (5, 6, 66)
-// and grid graphs (-i) are not part of this build and exit with status 2.
+// error of the value.  Approximation (-a) and grid graphs (-i) are not part
+// of this build and exit with status 2.
 #include <getopt.h>
 #include <omp.h>
 
@@ -71,32 +77,44 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
     int st;
     const char* name;
     const double t0 = omp_get_wtime();
+    int cpu_chunks = 0;
     if (o.dense) {
-      if (o.algo < 0 || o.algo > 4) {
-        std::printf(o.algo == 5 || o.algo == 6 || o.algo == 66
-                        ? "Algorithm ID %d (multi-GPU / hybrid) is not part of this single-GPU "
-                          "build\n"
-                        : "Unknown Algorithm ID\n",
-                    o.algo);
+      if (o.algo >= 0 && o.algo <= 4) {
+        name = kDenseNames[o.algo];
+        st = perm_dense(dt, mat.data(), n, o.algo, &r);
+      } else if (o.algo == 5 || o.algo == 66) {  // one GPU computes the whole range
+        name = o.algo == 5 ? "gpu_perman64_xshared_coalescing_mshared_multigpu"
+                           : "gpu_perman64_xshared_coalescing_mshared_multigpu_manual_distribution";
+        st = perm_dense(dt, mat.data(), n, 4, &r);
+      } else if (o.algo == 6) {  // chunk scheduler, CPU worker with -c (main.cu:62-66)
+        name = "gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks";
+        st = perm_chunked(dt, mat.data(), nullptr, nullptr, nullptr, n, o.cpu, o.threads, &r,
+                          &cpu_chunks);
+      } else {
+        std::printf("Unknown Algorithm ID\n");
         return 2;
       }
-      name = kDenseNames[o.algo];
-      st = perm_dense(dt, mat.data(), n, o.algo, &r);
-    } else if (o.algo == 7) {  // the skipper (main.cu:138-142)
-      name = "gpu_perman64_xshared_coalescing_mshared_skipper";
+    } else if (o.algo == 7 || o.algo == 8) {  // the skipper (main.cu:138-147)
+      name = o.algo == 7 ? "gpu_perman64_xshared_coalescing_mshared_skipper"
+                         : "gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_skipper";
       st = perm_skip(dt, mat.data(), c.rptrs.data(), c.cols.data(), c.cptrs.data(),
                      c.rows.data(), c.cvals.data(), n, &r);
-    } else {
-      if (o.algo < 1 || o.algo > 4) {
-        std::printf(o.algo >= 5 ? "Algorithm ID %d (multi-GPU / hybrid) is not "
-                                  "part of this single-GPU build\n"
-                                : "Unknown Algorithm ID\n",
-                    o.algo);
-        return 2;
-      }
+    } else if (o.algo >= 1 && o.algo <= 4) {
       name = kSparseNames[o.algo];
       st = perm_sparse(dt, mat.data(), c.cptrs.data(), c.rows.data(), c.cvals.data(), n, o.algo,
                        &r);
+    } else if (o.algo == 5 || o.algo == 66) {
+      name = o.algo == 5 ? "gpu_perman64_xshared_coalescing_mshared_multigpu_sparse"
+                         : "gpu_perman64_xshared_coalescing_mshared_multigpu_sparse_manual_"
+                           "distribution";
+      st = perm_sparse(dt, mat.data(), c.cptrs.data(), c.rows.data(), c.cvals.data(), n, 4, &r);
+    } else if (o.algo == 6) {
+      name = "gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_sparse";
+      st = perm_chunked(dt, mat.data(), c.cptrs.data(), c.rows.data(), c.cvals.data(), n, o.cpu,
+                        o.threads, &r, &cpu_chunks);
+    } else {
+      std::printf("Unknown Algorithm ID\n");
+      return 2;
     }
     const double t1 = omp_get_wtime();
     if (st != PERM_OK) {
@@ -104,6 +122,7 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
       return 1;
     }
     std::printf("kernel in %.6f\n", r.kernel_ms / 1e3);
+    if (cpu_chunks > 0) std::printf("CPU worker: %d chunk(s)\n", cpu_chunks);
     print_result(name, r.value, t1 - t0);
     std::printf("Error bound: %.3e (sum of |terms| %.6e)\n", r.err_bound, r.abs_sum);
     return 0;
@@ -119,8 +138,24 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
       v = perm_parallel_perman64_f32(mat.data(), n, o.threads);
     else v = perm_parallel_perman64_f64(mat.data(), n, o.threads);
   } else {
+    if (o.algo == 2 || o.algo == 3) {  // the skip walks (main.cu:215-224)
+      name = o.algo == 2 ? "parallel_skip_perman64_w" : "parallel_skip_perman64_w_balanced";
+      perm_result r;
+      const int st = perm_cpu_skip(dt, c.rptrs.data(), c.cols.data(), c.rvals.data(),
+                                   c.cptrs.data(), c.rows.data(), c.cvals.data(), n, o.threads,
+                                   &r);
+      const double t1 = omp_get_wtime();
+      if (st != PERM_OK) {
+        std::fprintf(stderr, "perman: %s\n", perm_last_error());
+        return 1;
+      }
+      print_result(name, r.value, t1 - t0);
+      std::printf("Error bound: %.3e (terms evaluated %.6g of %.6g)\n", r.err_bound,
+                  r.main_subsets, r.subsets);
+      return 0;
+    }
     if (o.algo != 1) {
-      std::printf("CPU sparse algorithm %d (skip variants) is not part of this build\n", o.algo);
+      std::printf("Unknown Algorithm ID\n");
       return 2;
     }
     name = "parallel_perman64_sparse";
@@ -194,7 +229,7 @@ int main(int argc, char** argv) {
   }
   for (int i = optind; i < argc; ++i) std::printf("Non-option argument %s\n", argv[i]);
   if (!o.cpu && !o.gpu) o.gpu = true;
-  if (o.gpu) o.cpu = false;  // -g wins over -c (main.cu:31, :186)
+  // with -g, -c only enables the CPU worker of the chunked IDs (main.cu:31, :62)
   if (o.grid || o.approximation) {
     std::fprintf(stderr, "perman: %s is not part of this build (exact Ryser paths only)\n",
                  o.grid ? "grid-graph mode (-i)" : "approximation (-a)");

@@ -147,8 +147,13 @@ def lib():
         L.perm_server_destroy.argtypes = [vp]
         L.perm_server_destroy.restype = None
         L.perm_connect.restype = None
+        L.perm_chunked.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(PermResultC),
+                                   ctypes.POINTER(ctypes.c_int)]
         L.perm_skip.argtypes = [ctypes.c_int, vp, ip, ip, ip, ip, vp, ctypes.c_int,
                                 ctypes.POINTER(PermResultC)]
+        L.perm_cpu_skip.argtypes = [ctypes.c_int, ip, ip, vp, ip, ip, vp, ctypes.c_int,
+                                    ctypes.c_int, ctypes.POINTER(PermResultC)]
         for sfx in _SFX.values():
             f = getattr(L, f"perm_xshared_coalescing_mshared_skipper_{sfx}")
             f.restype = ctypes.c_double
@@ -381,6 +386,36 @@ def dropin_skipper(mat) -> float:
     f = getattr(lib(), f"perm_xshared_coalescing_mshared_skipper_{_SFX[dt]}")
     return f(a.ctypes.data, _ip(rptrs), _ip(cols), _ip(cptrs), _ip(rows), cvals.ctypes.data,
              a.shape[0], 2048, 256)
+
+
+def cpu_skip(mat, threads: int = 8) -> PermResult:
+    """CPU skip walk of `perman -c -s -p2/-p3` (parallel_skip_perman64_w and
+    _w_balanced, algo.h:749, :886).  main_subsets = terms evaluated."""
+    a, dt = _mat(mat)
+    cptrs, rows, cvals = compress(a)
+    rptrs, cols, rvals = _crs(a)
+    r = PermResultC()
+    _check(lib().perm_cpu_skip(dt, _ip(rptrs), _ip(cols), rvals.ctypes.data, _ip(cptrs),
+                               _ip(rows), cvals.ctypes.data, a.shape[0], threads,
+                               ctypes.byref(r)))
+    return PermResult.from_c(r)
+
+
+def chunked(mat, use_cpu: bool = False, threads: int = 8, sparse_path: bool = False):
+    """The chunked GPU(+CPU) scheduler (reference IDs 6, gpu_exact_dense.cu:777):
+    returns (PermResult, number of chunks the CPU computed)."""
+    a, dt = _mat(mat)
+    r, nc = PermResultC(), ctypes.c_int()
+    if sparse_path:
+        cptrs, rows, cvals = compress(a)
+        st = lib().perm_chunked(dt, a.ctypes.data, _ip(cptrs), _ip(rows), cvals.ctypes.data,
+                                a.shape[0], int(use_cpu), threads, ctypes.byref(r),
+                                ctypes.byref(nc))
+    else:
+        st = lib().perm_chunked(dt, a.ctypes.data, None, None, None, a.shape[0], int(use_cpu),
+                                threads, ctypes.byref(r), ctypes.byref(nc))
+    _check(st)
+    return PermResult.from_c(r), nc.value
 
 
 def read_calculate_return(filename: str, algorithm: int, nt: int = 16, x: int = 0, y: int = 0,

@@ -151,7 +151,10 @@ def test_cli_flag_errors(tmp_path):
     f = tmp_path / "m"
     write_matrix(f, np.eye(3))
     assert run_cli("-f", str(f), "-a").returncode == 2   # approximations not in this build
-    assert run_cli("-f", str(f), "-c", "-s", "-p", "2").returncode == 2
+    assert run_cli("-f", str(f), "-c", "-s", "-p", "4").returncode == 2
+    for args in (("-p", "9"), ("-p", "7"), ("-s", "-p", "0"), ("-s", "-p", "9")):
+        r = run_cli("-f", str(f), "-g", *args)  # rejected before any device call
+        assert r.returncode == 2 and "Unknown Algorithm ID" in r.stdout, args
     r = run_cli("-f", str(f), "-t", "-3")
     assert r.returncode == 1 and "requires an argument" in r.stderr
 

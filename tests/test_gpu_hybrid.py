@@ -1,0 +1,95 @@
+"""Multi-GPU / hybrid algorithm IDs on one GPU (dense 5, 6, 66; sparse 5, 6,
+8, 66): the chunk scheduler over the Gray range with and without the CPU
+worker, and every drop-in name, against the one-launch path and the exact
+oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import re
+
+import oracle as O
+import permanent_amd as P
+from test_capi import run_cli, write_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def test_chunked_gpu_only_equals_one_launch():
+    a = P.synthetic_matrix(32, 0.5, seed=32)   # 2^3 chunks
+    r, cpu = P.chunked(a, use_cpu=False)
+    one = P.dense(a)
+    assert cpu == 0
+    assert (r.value, r.sum_hi, r.sum_lo) == (one.value, one.sum_hi, one.sum_lo)
+
+
+def test_chunked_with_cpu_worker():
+    a = P.synthetic_matrix(31, 0.5, seed=31)   # 2^3 chunks with the CPU worker
+    r, cpu = P.chunked(a, use_cpu=True, threads=8)
+    one = P.dense(a)
+    print(f"CPU worker computed {cpu} chunk(s)")
+    assert abs(r.value - one.value) <= r.err_bound + one.err_bound
+    assert abs(r.value - one.value) <= 1e-11 * abs(one.value)
+
+
+def test_chunked_small_and_sparse_vs_exact():
+    a = P.synthetic_matrix(18, 0.4, seed=18)
+    ex = float(O.exact_permanent(a))
+    for cpu in (False, True):
+        r, _ = P.chunked(a, use_cpu=cpu, threads=4, sparse_path=True)
+        assert r.value == pytest.approx(ex, rel=1e-12)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def test_hybrid_dropins_every_id_and_type():
+    L = P.lib()
+    base = P.synthetic_matrix(16, 0.5, seed=16, binary=True)
+    ex = float(O.exact_permanent(base))
+    for dtype, sfx in ((np.int32, "i32"), (np.float32, "f32"), (np.float64, "f64")):
+        a = np.ascontiguousarray(base.astype(dtype))
+        cptrs, rows, cvals = P.compress(a)
+        rptrs, cols, _ = P._crs(a)
+        vals = []
+        f = getattr(L, f"perm_xshared_coalescing_mshared_multigpu_dense_{sfx}")
+        f.restype = ctypes.c_double
+        vals.append(f(ctypes.c_void_p(a.ctypes.data), 16, 2, 2048, 256))
+        f = getattr(L, f"perm_xshared_coalescing_mshared_multigpu_manual_distribution_dense_{sfx}")
+        f.restype = ctypes.c_double
+        vals.append(f(ctypes.c_void_p(a.ctypes.data), 16, 4, 2048, 256))
+        f = getattr(L, f"perm_xshared_coalescing_mshared_multigpucpu_chunks_dense_{sfx}")
+        f.restype = ctypes.c_double
+        vals.append(f(ctypes.c_void_p(a.ctypes.data), 16, 2, 1, 4, 2048, 256))
+        for name, extra in (("multigpu_sparse", (2, 2048, 256)),
+                            ("multigpu_sparse_manual_distribution", (4, 2048, 256)),
+                            ("multigpucpu_chunks_sparse", (2, 1, 4, 2048, 256))):
+            f = getattr(L, f"perm_xshared_coalescing_mshared_{name}_{sfx}")
+            f.restype = ctypes.c_double
+            vals.append(f(ctypes.c_void_p(a.ctypes.data), _ip(cptrs), _ip(rows),
+                          ctypes.c_void_p(cvals.ctypes.data), 16, *extra))
+        f = getattr(L, f"perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_{sfx}")
+        f.restype = ctypes.c_double
+        vals.append(f(ctypes.c_void_p(a.ctypes.data), _ip(rptrs), _ip(cols), _ip(cptrs),
+                      _ip(rows), ctypes.c_void_p(cvals.ctypes.data), 16, 2, 1, 4, 2048, 256))
+        assert vals == [ex] * len(vals), (sfx, vals, ex)
+
+
+def test_cli_every_gpu_id(tmp_path):
+    """`perman -g` for every exact GPU ID of the reference's RunAlgo
+    (main.cu:31-148): dense 0-6 and 66, sparse 1-8 and 66; -g -c gives the
+    chunked IDs their CPU worker."""
+    a = P.synthetic_matrix(17, 0.45, seed=17, binary=True) * 2
+    f = tmp_path / "m17"
+    write_matrix(f, a, "int")
+    ex = float(O.exact_permanent(a))
+    runs = [("-p", str(p)) for p in (0, 1, 2, 3, 4, 5, 6, 66)]
+    runs += [("-s", "-p", str(p)) for p in (1, 2, 3, 4, 5, 6, 7, 8, 66)]
+    runs += [("-c", "-p", "6", "-t", "4"), ("-c", "-s", "-p", "6", "-t", "4")]
+    for args in runs:
+        r = run_cli("-f", str(f), "-g", *args)
+        assert r.returncode == 0, (args, r.stdout, r.stderr)
+        m = re.search(r"^Result: (\S+) (\S+) in \S+$", r.stdout, re.M)
+        assert m and float(m.group(2)) == ex, (args, r.stdout)
