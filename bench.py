@@ -21,6 +21,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
 
+# HBM bytes per launch of the n=40 kernel on this workload: rocprofv3 FETCH_SIZE
+# + WRITE_SIZE, each scaled by its reading for a copy kernel of known size
+# (profiles/ryser_dense_n40.md): 734.6 KiB read + 16098.7 KiB written.
+TRAFFIC_BYTES_N40 = (734.6 + 16098.7) * 1024
 FP64_PEAK_TFLOPS = 78.6
 N = 40
 DENSITY = 0.5
@@ -289,12 +293,15 @@ def main():
         "result": {"value": r.value, "err_est": r.err_est, "err_bound": r.err_bound,
                    "abs_sum": r.abs_sum, "nu": r.nu},
         "roofline": {
-            "bound": "valu",
+            "bound": "mfma",
+            "bound_detail": "FP64 compute: the kernel issues FP64 VALU instructions, whose MI355X "
+                            "peak (78.6 TFLOP/s) equals the FP64 matrix-core peak; HBM does not "
+                            "bound it (traffic below, ~21 MB/s)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_N40,
             "kernel": f"perm::ryser_dense_kernel<40,3,0,{r.nu}>",
             "kernel_ms": main_ms,
             "all_kernels_ms": k_ms,

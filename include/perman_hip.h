@@ -289,7 +289,7 @@ double perm_xshared_coalescing_mshared_multigpucpu_chunks_skipper_f64(
  * perm_read_calculate_return(filename, algorithm, nt, x, y, z) replaces
  * read_calculate_return (py_connector.c:63) and
  * perm_matlab_calculate_return(mat, algorithm, nt, x, y, z, nov, nnz) replaces
- * matlab_calculate_return (py_connector.c:146); perm_connect replaces
+ * matlab_calculate_return (py_connector.c:152); perm_connect replaces
  * connect (:59).  Algorithms 4-8 (the exact ones: 4/6/7 sparse, 5/8 dense)
  * run on the GPU; 0-3 (approximations) return NaN with PERM_ERR_ARG; nt, x,
  * y, z are unused.  Unlike the reference's connector, the file's values are

@@ -182,8 +182,23 @@ int main() {
   int mi[9] = {1, 2, 0, 0, 3, 4, 5, 0, 6};
   double (*g)(double*, int, int, int) = &gpu_perman64_xshared_coalescing_mshared<double>;
   double (*s)(float*, int*, int*, float*, int, int, int) = &gpu_perman64_xlocal_sparse<float>;
-  std::printf("%.17g %.17g %.17g %d\n", parallel_perman64(m, 3, 2), parallel_perman64(mi, 3, 1),
-              parallel_perman64_sparse(m, cp, rows, cv, 3, 1), g != nullptr && s != nullptr);
+  int rp[4] = {0, 2, 4, 6}, cols[6] = {0, 1, 1, 2, 0, 2};
+  double rv[6] = {1, 2, 3, 4, 5, 6};
+  // every multi-GPU / hybrid name instantiates for the three element types
+  auto h1 = &gpu_perman64_xshared_coalescing_mshared_multigpu<int>;
+  auto h2 = &gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks<float>;
+  auto h3 = &gpu_perman64_xshared_coalescing_mshared_multigpu_manual_distribution<double>;
+  auto h4 = &gpu_perman64_xshared_coalescing_mshared_multigpu_sparse<int>;
+  auto h5 = &gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_sparse<float>;
+  auto h6 = &gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_skipper<double>;
+  auto h7 = &gpu_perman64_xshared_coalescing_mshared_multigpu_sparse_manual_distribution<int>;
+  bool h = h1 != nullptr && h2 != nullptr && h3 != nullptr && h4 != nullptr && h5 != nullptr &&
+           h6 != nullptr && h7 != nullptr;
+  std::printf("%.17g %.17g %.17g %.17g %.17g %d\n", parallel_perman64(m, 3, 2),
+              parallel_perman64(mi, 3, 1), parallel_perman64_sparse(m, cp, rows, cv, 3, 1),
+              parallel_skip_perman64_w(rp, cols, rv, cp, rows, cv, 3, 2),
+              parallel_skip_perman64_w_balanced(rp, cols, rv, cp, rows, cv, 3, 1),
+              g != nullptr && s != nullptr && h);
   return 0;
 }
 """)
@@ -195,7 +210,7 @@ int main() {
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     m = np.array([[1, 2, 0], [0, 3, 4], [5, 0, 6]], dtype=float)
     want = float(O.exact_permanent(m))
-    assert [float(v) for v in out[:3]] == [want, want, want] and out[3] == "1"
+    assert [float(v) for v in out[:5]] == [want] * 5 and out[5] == "1"
 
 
 def test_connector_entry_points_reject_without_touching_the_gpu(tmp_path):
