@@ -72,7 +72,7 @@ def sparse_leg(P, reps: int):
         "subsets_per_s": subsets / wall,
         "ms_per_run": wall * 1e3,
         "kernel_ms": r.kernel_ms,
-        "nu": r.nu,
+        "shape": [r.nu, r.nu1, r.nu2],
         "pct_fp64_peak_dense_equivalent": 100.0 * 72.0 * subsets / wall / 1e12 / FP64_PEAK_TFLOPS,
         "pct_fp64_peak_sparse_minimal": 100.0 * (36.0 + nnz / 36.0) * subsets / wall / 1e12
                                         / FP64_PEAK_TFLOPS,
@@ -291,7 +291,7 @@ def main():
         "pct_fp64_peak_whole_step": 100.0 * 2.0 * N * subsets / (ms_per_step / 1e3) / 1e12
                                     / FP64_PEAK_TFLOPS,
         "result": {"value": r.value, "err_est": r.err_est, "err_bound": r.err_bound,
-                   "abs_sum": r.abs_sum, "nu": r.nu},
+                   "abs_sum": r.abs_sum, "shape": [r.nu, r.nu1, r.nu2]},
         "roofline": {
             "bound": "mfma",
             "bound_detail": "FP64 compute: the kernel issues FP64 VALU instructions, whose MI355X "
@@ -302,16 +302,17 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": TRAFFIC_BYTES_N40,
-            "kernel": f"perm::ryser_dense_kernel<40,3,0,{r.nu}>",
+            "kernel": f"perm::ryser_dense_kernel<40,3,0,perm::Nest<{r.nu},{r.nu1},{r.nu2}>>",
             "kernel_ms": main_ms,
             "all_kernels_ms": k_ms,
             "subsets_per_launch": r.main_subsets,
             "algorithmic_flops_per_launch": flops,
             "note": "FP64 VALU-bound. Algorithmic work is 2n flops per subset (n adds + n "
-                    "multiplies); the pair kernel issues n + nu + 1/2 FP64 instructions per "
-                    "subset (adds of zero skipped, the passive-row product shared by the two "
-                    "subsets of a pair), so the rate can exceed the 50% of the FMA-counted peak "
-                    "that 2n non-fusable instructions would allow",
+                    "multiplies). The nested kernel restricts Gray columns 0, 1, 2 to the row "
+                    "prefixes [0,K0), [0,K1), [0,K2) and reuses the products of the untouched "
+                    "tails: (4 K0 + 2 K1 + K2 + n)/4 + 2 FP64 instructions per subset instead of "
+                    "2n, so the algorithmic rate can exceed the 50% of the FMA-counted peak that "
+                    "2n non-fusable instructions would allow",
         },
     }
     out["sparse"] = sparse_leg(P, 3)

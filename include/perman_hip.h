@@ -60,9 +60,11 @@ typedef struct perm_result {
   int mode;
   int integral;         /* 1 if every entry is an integer (X is then exact) */
   double err_est;       /* statistical error estimate (independent roundings) */
-  int nu;               /* FP64 kernel: rows touched by Gray column 0 (pair variant), or n */
+  int nu;               /* FP64 kernel: rows touched by Gray column 0 (pair/nested variant), or n */
   double main_ms;       /* single-matrix path: device time of the main walk kernel */
   double main_subsets;  /* subsets that kernel evaluated */
+  int nu1;              /* FP64 kernel: rows touched by Gray columns 0-1 (nested variant), or n */
+  int nu2;              /* FP64 kernel: rows touched by Gray columns 0-2 (nested variant), or n */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */

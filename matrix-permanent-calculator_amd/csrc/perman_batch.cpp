@@ -306,7 +306,7 @@ int perm_batch_set(perm_batch* B, int item_log2, int streams, int max_grid) {
 int perm_batch_add_dense(perm_batch* B, int dtype, const void* mat, int nov, int* index) {
   if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
   perm::Prepared p;
-  if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p, false)) return s;
   return add_prepared(B, nov, p, index);
 }
 
@@ -314,7 +314,7 @@ int perm_batch_add_sparse(perm_batch* B, int dtype, const void* mat, const int* 
                           const int* rows, const void* cvals, int nov, int* index) {
   if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
   perm::Prepared p;
-  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, false)) return s;
   return add_prepared(B, nov, p, index);
 }
 

@@ -63,7 +63,8 @@ struct perm_plan {
   perm::DenseLaunch L{};
   int unroll_bits = 0;
   int walk_bits = 0, high_bits = 0;  // split of the n-1 Gray columns
-  int nu = 0;             // active rows of Gray column 0 for the FP64 kernel (n: none)
+  int nu = 0;             // rows touched by Gray column 0 for the FP64 kernel (n: none)
+  int nu1 = 0, nu2 = 0;   // rows touched by Gray columns 0-1 and 0-2 (n: not nested)
   bool integral = false;  // every entry an integer of magnitude < 2^31
   bool borrowed = false;  // stream/events/memory belong to the device cache
 };
@@ -128,11 +129,20 @@ void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
   P->L.result = (perm::Partial*)(b + l.col + l.x + l.part);
 }
 
+// The FP64 kernel's walk shape from the host preparation (0: not restricted).
+void set_shape(perm_plan* P, int n, const Prepared& prep) {
+  P->nu = prep.nu > 0 ? prep.nu : n;
+  P->nu1 = prep.nu1 > 0 ? prep.nu1 : n;
+  P->nu2 = prep.nu2 > 0 ? prep.nu2 : n;
+}
+
 // The walk of one run in `mode`: all 2^h lanes.
 void configure_passes(perm_plan* P, int mode) {
   perm::DenseLaunch& L = P->L;
   L.mode = mode;
   L.nu = mode == 0 ? P->nu : P->n;
+  L.nu1 = mode == 0 ? P->nu1 : P->n;
+  L.nu2 = mode == 0 ? P->nu2 : P->n;
   L.main.lanes = 1ull << P->high_bits;
   L.main.blocks = blocks_of(L.main.lanes);
   L.main.walk_bits = P->walk_bits;
@@ -154,12 +164,14 @@ int upload_plan(perm_plan* P, const std::vector<double>& colstore, const std::ve
 }
 
 // Create an owning plan (its own stream, events and device memory).
-int make_plan(int n, const std::vector<double>& colstore, const std::vector<double>& x0, int nu,
-              bool integral, perm_plan** out) {
+int make_plan(int n, const Prepared& prep, perm_plan** out) {
+  const std::vector<double>& colstore = prep.cs;
+  const std::vector<double>& x0 = prep.x0;
+  const bool integral = prep.integral;
   if (int s = select_device()) return s;
   auto* P = new perm_plan();
   P->device = perm::current_device();
-  P->nu = nu;
+  set_shape(P, n, prep);
   P->integral = integral;
   P->mem.bytes = layout_plan(P, n, colstore.size());
   auto cleanup = [&](int code, const std::string& msg) {
@@ -216,6 +228,8 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->status = PERM_OK;
   out->mode = mode;
   out->nu = mode == 0 ? P->nu : P->n;
+  out->nu1 = mode == 0 ? P->nu1 : P->n;
+  out->nu2 = mode == 0 ? P->nu2 : P->n;
   out->integral = P->integral ? 1 : 0;
 }
 
@@ -261,7 +275,7 @@ int bind_oneshot(int n, const Prepared& prep, DeviceCache& C, perm_plan& P) {
   P.ev1 = C.ev1;
   P.evm = C.evm;
   P.integral = prep.integral;
-  P.nu = prep.nu;
+  set_shape(&P, n, prep);
   const size_t need = layout_plan(&P, n, cs.size());
   if (need > C.bytes) {
     if (C.base) HIP_TRY(hipFree(C.base));
@@ -408,14 +422,14 @@ int perm_plan_dense(int dtype, const void* mat, int nov, perm_plan** plan) {
   if (!plan) return fail(PERM_ERR_ARG, "null plan");
   Prepared p;
   if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
-  return make_plan(nov, p.cs, p.x0, p.nu, p.integral, plan);
+  return make_plan(nov, p, plan);
 }
 int perm_plan_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                      const void* cvals, int nov, perm_plan** plan) {
   if (!plan) return fail(PERM_ERR_ARG, "null plan");
   Prepared p;
   if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
-  return make_plan(nov, p.cs, p.x0, p.nu, p.integral, plan);
+  return make_plan(nov, p, plan);
 }
 int perm_plan_run(perm_plan* plan, perm_result* out) { return plan_run_impl(plan, -1, out); }
 int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out) {

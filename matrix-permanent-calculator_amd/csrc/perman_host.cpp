@@ -98,40 +98,106 @@ int select_device() {
 // k < n-1), then chooses the walk order (pair_order) and builds the signed
 // column store.
 
-// Zero-skipping order for the pair kernels: the walked column with the
-// fewest nonzeros goes to Gray position 0 and the rows it touches move to
-// the front, padded to a multiple of 4 (nu).  Row and column permutations
-// leave the permanent unchanged; PERMAN_PAIR=0 keeps the natural order.
-void pair_order(int n, std::vector<double>& x0, std::vector<double>& cols, int& nu) {
-  nu = n;
+// Walk order for the pair and nested kernels (ryser_dense_impl.hpp).  Gray
+// columns 0, 1, 2 become the columns whose row supports admit the compiled
+// shape (nest_menu_shape) of least work (nest_cost): candidates for column 0
+// are the 8 sparsest columns, for columns 1 and 2 the 12 sparsest.  Rows are
+// ordered [column 0's rows | column 1's new rows | column 2's new rows |
+// the rest] (ascending index inside each part), so column l touches only
+// rows [0, K_l).  Row and column permutations leave the permanent unchanged.
+// `nested` false: only pair shapes (Gray column 0 restricted).
+// PERMAN_PAIR=0 keeps the natural order; PERMAN_NEST=0 only pair shapes.
+void nest_order(int n, bool nested, std::vector<double>& x0, std::vector<double>& cols,
+                int& nu, int& nu1, int& nu2) {
+  nu = nu1 = nu2 = n;
   if (n < perm::kMinPairN || env_int("PERMAN_PAIR", 1) == 0) return;
+  if (env_int("PERMAN_NEST", 1) == 0) nested = false;
   const int B = n - 1;
-  int best = 0, best_nnz = n + 1;
-  for (int k = 0; k < B; ++k) {
-    int c = 0;
-    for (int j = 0; j < n; ++j) c += cols[(size_t)k * n + j] != 0.0;
-    if (c < best_nnz) {
-      best_nnz = c;
-      best = k;
+  std::vector<uint64_t> sup(B, 0);
+  for (int k = 0; k < B; ++k)
+    for (int j = 0; j < n; ++j)
+      if (cols[(size_t)k * n + j] != 0.0) sup[k] |= 1ull << j;
+  std::vector<int> by_count(B);
+  for (int k = 0; k < B; ++k) by_count[k] = k;
+  std::stable_sort(by_count.begin(), by_count.end(), [&](int a, int b) {
+    return __builtin_popcountll(sup[a]) < __builtin_popcountll(sup[b]);
+  });
+  const int n0 = std::min(B, 8), n12 = nested ? std::min(B, 12) : 0;
+  int best_cost = perm::nest_cost(n, {n, n, n});
+  perm::NestShape best{n, n, n};
+  int bc[3] = {-1, -1, -1};
+  // cheapest compiled shape covering supports of sizes (c0, c01, c012)
+  auto cover = [&](int c0, int c01, int c012, perm::NestShape& out) {
+    int cost = -1;
+    for (int k0 = std::max(4, (c0 + 3) / 4 * 4); k0 <= n - 4; k0 += 4)
+      for (int j = 0; j < perm::kNestShapesPerK0; ++j) {
+        const perm::NestShape s = perm::nest_menu_shape(n, k0, j);
+        if (s.k0 == 0 || s.k1 < c01 || s.k2 < c012) continue;
+        const int c = perm::nest_cost(n, s);
+        if (cost < 0 || c < cost) {
+          cost = c;
+          out = s;
+        }
+      }
+    return cost;
+  };
+  for (int i0 = 0; i0 < n0; ++i0) {
+    const int a = by_count[i0];
+    const int c0 = __builtin_popcountll(sup[a]);
+    perm::NestShape s;
+    int cost = cover(c0, n, n, s);  // pair shape: columns 1, 2 unrestricted
+    if (cost >= 0 && cost < best_cost) {
+      best_cost = cost;
+      best = s;
+      bc[0] = a;
+      bc[1] = bc[2] = -1;
+    }
+    for (int i1 = 0; i1 < n12; ++i1) {
+      const int b = by_count[i1];
+      if (b == a) continue;
+      const uint64_t u01 = sup[a] | sup[b];
+      for (int i2 = 0; i2 < n12; ++i2) {
+        const int c = by_count[i2];
+        if (c == a || c == b) continue;
+        const uint64_t u012 = u01 | sup[c];
+        cost = cover(c0, __builtin_popcountll(u01), __builtin_popcountll(u012), s);
+        if (cost >= 0 && cost < best_cost) {
+          best_cost = cost;
+          best = s;
+          bc[0] = a;
+          bc[1] = b;
+          bc[2] = c;
+        }
+      }
     }
   }
-  const int padded = std::max(4, (best_nnz + 3) / 4 * 4);
-  if (padded > n - 4) return;
+  if (bc[0] < 0) return;
+  // rows: column 0's, then column 1's new ones, then column 2's, then the rest
   std::vector<int> rp, cp;
+  uint64_t taken = 0;
+  for (int l = 0; l < 3; ++l) {
+    if (bc[l] < 0) continue;
+    for (int j = 0; j < n; ++j)
+      if (((sup[bc[l]] >> j) & 1) && !((taken >> j) & 1)) {
+        rp.push_back(j);
+        taken |= 1ull << j;
+      }
+  }
   for (int j = 0; j < n; ++j)
-    if (cols[(size_t)best * n + j] != 0.0) rp.push_back(j);
-  for (int j = 0; j < n; ++j)
-    if (cols[(size_t)best * n + j] == 0.0) rp.push_back(j);
-  cp.push_back(best);
+    if (!((taken >> j) & 1)) rp.push_back(j);
+  for (int l = 0; l < 3; ++l)
+    if (bc[l] >= 0) cp.push_back(bc[l]);
   for (int k = 0; k < B; ++k)
-    if (k != best) cp.push_back(k);
+    if (k != bc[0] && k != bc[1] && k != bc[2]) cp.push_back(k);
   std::vector<double> x2(n), c2(cols.size());
   for (int j = 0; j < n; ++j) x2[j] = x0[rp[j]];
   for (int k = 0; k < B; ++k)
     for (int j = 0; j < n; ++j) c2[(size_t)k * n + j] = cols[(size_t)cp[k] * n + rp[j]];
   x0.swap(x2);
   cols.swap(c2);
-  nu = padded;
+  nu = best.k0;
+  nu1 = best.k1;
+  nu2 = best.k2;
 }
 
 bool integral_matrix(const std::vector<double>& a) {
@@ -140,12 +206,12 @@ bool integral_matrix(const std::vector<double>& a) {
   return true;
 }
 
-void finish_prepare(int n, std::vector<double>& cols, Prepared& p) {
-  pair_order(n, p.x0, cols, p.nu);
+void finish_prepare(int n, bool nested, std::vector<double>& cols, Prepared& p) {
+  nest_order(n, nested, p.x0, cols, p.nu, p.nu1, p.nu2);
   perm::build_column_store(n, [&](int k, int j) { return cols[(size_t)k * n + j]; }, p.cs);
 }
 
-int prepare_dense(int dtype, const void* mat, int n, Prepared& p) {
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p, bool nested) {
   if (!mat) return fail(PERM_ERR_ARG, "null pointer");
   if (int s = check_n(n)) return s;
   std::vector<double> a;
@@ -156,7 +222,7 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p) {
   std::vector<double> cols((size_t)(n - 1) * n);
   for (int k = 0; k < n - 1; ++k)
     for (int j = 0; j < n; ++j) cols[(size_t)k * n + j] = a[(size_t)j * n + k];
-  finish_prepare(n, cols, p);
+  finish_prepare(n, nested, cols, p);
   return PERM_OK;
 }
 
@@ -189,10 +255,10 @@ int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
 }
 
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, Prepared& p) {
+                   const void* cvals, int n, Prepared& p, bool nested) {
   std::vector<double> cols;
   if (int s = gather_sparse(dtype, mat, cptrs, rows, cvals, n, p.x0, cols, p.integral)) return s;
-  finish_prepare(n, cols, p);
+  finish_prepare(n, nested, cols, p);
   return PERM_OK;
 }
 

@@ -21,16 +21,19 @@ int env_int(const char* name, int dflt);
 int check_n(int n);
 
 // A matrix prepared for the Ryser kernels: start vector, signed column
-// store (walk order applied), active rows of Gray column 0 (nu; n when the
-// pair kernels do not apply), whether every entry is an integer.
+// store (walk order applied), the walk shape (Gray column 0 touches rows
+// [0, nu), columns 0-1 rows [0, nu1), columns 0-2 rows [0, nu2); n or 0
+// where not restricted), whether every entry is an integer.
 struct Prepared {
   std::vector<double> x0, cs;
-  int nu = 0;
+  int nu = 0, nu1 = 0, nu2 = 0;
   bool integral = false;
 };
-int prepare_dense(int dtype, const void* mat, int n, Prepared& p);
+// nested: allow nested walk shapes (single-matrix paths); false: pair
+// shapes only (the batch and server kernels).
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p, bool nested = true);
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, Prepared& p);
+                   const void* cvals, int n, Prepared& p, bool nested = true);
 // The sparse path's inputs before any reordering: x0 from the dense matrix,
 // cols[k*n + j] = CCS value of (j, k) for the walked columns k < n-1.
 int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,

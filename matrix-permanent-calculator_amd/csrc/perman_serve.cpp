@@ -582,7 +582,7 @@ int perm_server_submit_dense(perm_server* S, int dtype, const void* mat, int nov
     return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
                                          std::to_string(perm::kMaxBatchN));
   perm::Prepared p;
-  if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p, false)) return s;
   return submit(S, nov, p, id);
 }
 
@@ -594,7 +594,7 @@ int perm_server_submit_sparse(perm_server* S, int dtype, const void* mat, const 
     return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
                                          std::to_string(perm::kMaxBatchN));
   perm::Prepared p;
-  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, false)) return s;
   return submit(S, nov, p, id);
 }
 

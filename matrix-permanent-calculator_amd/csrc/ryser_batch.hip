@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void ryser_batch_variant_kernel(
     const uint64_t lane = d.lane_base + (threadIdx.x & (kWave - 1));
     DD s;
     double a, q;
-    lane_walk<N, R, kFast, NU>(m.colpm, m.x0, lane, m.walk_bits, m.high_bits, s, a, q);
+    lane_walk<N, R, kFast, Nest<NU, N, N>>(m.colpm, m.x0, lane, m.walk_bits, m.high_bits, s, a, q);
     if (lane >= m.lanes) {
       s = {0.0, 0.0};
       a = 0.0;

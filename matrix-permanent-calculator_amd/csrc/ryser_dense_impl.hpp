@@ -22,13 +22,19 @@
 //  * Per step: n adds, n-2 multiplies, one FMA that multiplies in the last x
 //    and accumulates the signed term, one FMA for sum |term|: 2n FP64 VALU
 //    instructions per subset.
-//  * Pair variant (NU < N): Gray column 0 flips on every odd step.  The host
-//    places the sparsest column there and moves the rows it touches to the
-//    front (rows [0, NU)).  The two subsets of each pair then differ only in
-//    x[0..NU), so the product of the passive rows x[NU..N) is formed once per
-//    pair and the odd step adds and multiplies only NU rows:
-//    n + NU + 1/2 instructions per subset instead of 2n.  Adds of zero are
-//    exact, so the skipped work changes no bits of any x.
+//  * Nested variants (shape Nest<K0, K1, K2>, K0 <= K1 <= K2 <= N): Gray
+//    column 0 flips on every odd step, column 1 on steps 2 mod 4, column 2
+//    on steps 4 mod 8.  The host places sparse columns there and orders the
+//    rows so that column l touches only rows [0, K_l).  A step of level l
+//    (the flipped column's position; 3 for any higher column) then adds
+//    K_l rows and the products of the untouched tails are reused:
+//    T3 = prod x[K2..N) changes only at level 3, T2 = prod x[K1..K2) T3 at
+//    levels >= 2, T1 = prod x[K0..K1) T2 at levels >= 1, and every term is
+//    prod x[0..K0) T1.  Per 8 steps: 4 K0 + 2 K1 + K2 + N adds and as many
+//    multiplies instead of 8 N each.  Nest<K0, N, N> is the pair variant
+//    (only the passive product of column 0's untouched rows is shared);
+//    Nest<N, N, N> is the plain walk.  Adds of zero are exact, so the
+//    skipped adds change no bits of any x.
 //  * Lane sums are folded into double-double every 2^(R+5) steps; lanes,
 //    waves and workgroups are reduced in double-double in a fixed order.
 #pragma once
@@ -175,25 +181,52 @@ __device__ __forceinline__ void add_column(double (&x)[N], double (&xl)[N], cons
   }
 }
 
+// Walk shape: column l < 3 of the Gray walk touches rows [0, K_l) only.
+template <int K0_, int K1_, int K2_>
+struct Nest {
+  static constexpr int K0 = K0_, K1 = K1_, K2 = K2_;
+};
+
+// Cached tail products of the nested variants (see the file comment).
+struct Tails {
+  double t1 = 1.0, t2 = 1.0, t3 = 1.0;
+};
+
+// Rows touched by a column at Gray level `lv` (3: any column >= 3).
+template <int N, class S>
+constexpr int level_rows(int lv) {
+  return lv == 0 ? S::K0 : lv == 1 ? S::K1 : lv == 2 ? S::K2 : N;
+}
+
+// Product x[LO..HI) times `seed` (SEED) -- `seed` itself for an empty range.
+template <int N, int LO, int HI, bool SEED>
+__device__ __forceinline__ double tail_prod(const double (&x)[N], double seed) {
+  if constexpr (LO >= HI) {
+    static_assert(SEED, "empty tail without a seed");
+    return seed;
+  } else {
+    return chain_prod<N, LO, HI, SEED>(x, seed);
+  }
+}
+
 // Step v of group g (step index t = g*2^R + v).  Phase A adds the column
-// that the previous step prefetched into SGPRs; phase B issues the scalar
-// loads of the next step's column and, while they are in flight, forms the
-// product and accumulates the term.  sched_barrier keeps the phases apart so
-// the loads always overlap a product.  With the pair variant (NU < N) odd
-// steps touch only x[0..NU) and reuse the passive product `ppass` formed on
-// the even step before them.  NOADD0: step 0 of the group adds nothing (x
-// already holds its subset; cv is not read).
-template <int N, int R, int MODE, int NU, int V, bool NOADD0 = false>
+// that the previous step prefetched into SGPRs (the rows its level touches);
+// phase B issues the scalar loads of the next step's column and, while they
+// are in flight, refreshes the tail products the step's level invalidated,
+// forms the product and accumulates the term.  sched_barrier keeps the
+// phases apart so the loads always overlap a product.  NOADD0: step 0 of the
+// group adds nothing (x already holds its subset; cv is not read).
+template <int N, int R, int MODE, class S, int V, bool NOADD0 = false>
 __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], double (&cv)[N],
                                             ColPtr cols, uint32_t g, uint32_t groups,
-                                            int zero_blk, LaneAcc& acc, double& ppass) {
-  constexpr bool kPair = NU < N;
+                                            int zero_blk, LaneAcc& acc, Tails& tl) {
+  constexpr int K0 = S::K0, K1 = S::K1, K2 = S::K2;
+  constexpr bool kNested = K0 < N;
   if constexpr (V < (1 << R)) {
+    constexpr int lv = V == 0 ? 3 : __builtin_ctz(V);
     if constexpr (NOADD0 && V == 0) {
-    } else if constexpr (kPair && (V & 1)) {
-      add_column<N, MODE, NU>(x, xl, cv);
     } else {
-      add_column<N, MODE, N>(x, xl, cv);
+      add_column<N, MODE, level_rows<N, S>(lv)>(x, xl, cv);
     }
     __builtin_amdgcn_sched_barrier(0);
     int blk;
@@ -206,6 +239,7 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
       if constexpr (k + 1 < R) neg = ((V + 1) >> (k + 1)) & 1;
       else neg = (int)(g & 1u);
       blk = 2 * k + neg;
+      load_col<N, level_rows<N, S>(k)>(cv, cols, blk);
     } else {
       // Next step opens group g+1: column R + ctz(g+1).  Past the last group
       // the (unused) prefetch reads the zero column to stay in bounds.
@@ -213,33 +247,34 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
       const int c = __builtin_ctz(gn);
       const int neg = (int)((gn >> (c + 1)) & 1u);
       blk = gn < groups ? 2 * (R + c) + neg : zero_blk;
+      load_col<N, N>(cv, cols, blk);
     }
-    if constexpr (kPair && ((V + 1) & 1) && V + 1 < (1 << R)) load_col<N, NU>(cv, cols, blk);
-    else load_col<N, N>(cv, cols, blk);
-    if constexpr (kPair) {
-      if constexpr ((V & 1) == 0) ppass = chain_prod<N, NU, N, false>(x, 0.0);
-      add_term<(V & 1) != 0>(chain_prod<N, 0, NU - 1, true>(x, ppass), x[NU - 1], acc);
+    if constexpr (kNested) {
+      if constexpr (lv >= 3 && K2 < N) tl.t3 = tail_prod<N, K2, N, false>(x, 0.0);
+      if constexpr (lv >= 2 && K1 < N) tl.t2 = tail_prod<N, K1, K2, (K2 < N)>(x, tl.t3);
+      if constexpr (lv >= 1) tl.t1 = tail_prod<N, K0, K1, (K1 < N)>(x, tl.t2);
+      add_term<(V & 1) != 0>(chain_prod<N, 0, K0 - 1, true>(x, tl.t1), x[K0 - 1], acc);
     } else {
       accumulate_term<N, (V & 1) != 0, MODE>(x, xl, acc);
     }
     __builtin_amdgcn_sched_barrier(0);
-    group_steps<N, R, MODE, NU, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+    group_steps<N, R, MODE, S, V + 1>(x, xl, cv, cols, g, groups, zero_blk, acc, tl);
   }
 }
 
 // The Gray walk of one lane over its groups.  Every 32 groups (a run of
 // 2^(R+5) terms) the FP64 lane sum is folded into the double-double s and
 // the square of the run's sum of |terms| is added to q.
-template <int N, int R, int MODE, int NU>
+template <int N, int R, int MODE, class S>
 __device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], ColPtr cols,
                                             uint32_t groups, int zero_blk, DD& s, LaneAcc& acc,
                                             double& q) {
-  double ppass = 1.0;
+  Tails tl;
   double a_run = 0.0;  // acc.a at the start of the current run
   double cv[N];
   load_col<N, N>(cv, cols, zero_blk);  // step 0 of the walk changes nothing
   for (uint32_t g = 0; g < groups; ++g) {
-    group_steps<N, R, MODE, NU, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+    group_steps<N, R, MODE, S, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, tl);
     if ((g & 31u) == 31u) {
       if constexpr (MODE == kFast) {  // fold into double-double
         s = dd_add_d(s, acc.p);
@@ -278,14 +313,17 @@ __device__ __forceinline__ void wave_min_max(int v, int& mn, int& mx) {
 // (sign of the high pattern applied), its sum of |terms| and its sum of
 // squared run sums (see walk_groups).
 // colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
-// Gray columns k < n-1, block 2(n-1) = zeros.  With NU < N, column 0 is zero
-// in rows [NU, N).
-template <int N, int R, int MODE, int NU>
+// Gray columns k < n-1, block 2(n-1) = zeros.  Shape S: column l < 3 is zero
+// in rows [K_l, N).
+template <int N, int R, int MODE, class S>
 __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0, uint64_t pattern,
                                           int n_walk_bits, int n_high_bits, DD& s, double& a,
                                           double& q) {
-  static_assert(NU == N || (MODE == kFast && R >= 1 && NU >= 1), "pair variant: FP64 mode only");
+  static_assert(S::K0 <= S::K1 && S::K1 <= S::K2 && S::K2 <= N, "nested shape");
+  static_assert(S::K0 == N || (MODE == kFast && R >= 1 && S::K0 >= 1), "nested: FP64 mode only");
+  static_assert(S::K1 == N || R >= 2, "level-1 tail needs a 4-step group");
+  static_assert(S::K2 == N || R >= 3, "level-2 tail needs an 8-step group");
   const ColPtr cols = (ColPtr)colpm;
   const int zero_blk = 2 * (N - 1);
 
@@ -314,7 +352,7 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
   q = 0.0;
   LaneAcc acc;
   const uint32_t groups = 1u << (n_walk_bits - R);
-  walk_groups<N, R, MODE, NU>(x, xl, cols, groups, zero_blk, s, acc, q);
+  walk_groups<N, R, MODE, S>(x, xl, cols, groups, zero_blk, s, acc, q);
   if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
   else s = two_sum(acc.p, acc.plo);
   a = acc.a;
@@ -326,7 +364,7 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
 
 // One pass (or a block range of it) of one permanent: lane = lane_base +
 // global thread id, one partial per workgroup.
-template <int N, int R, int MODE, int NU>
+template <int N, int R, int MODE, class S>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
@@ -337,7 +375,7 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
   const uint64_t pattern = lane < lanes ? lane : 0ull;  // high columns fixed by the lane id
   DD s;
   double a, q;
-  lane_walk<N, R, MODE, NU>(colpm, x0, pattern, n_walk_bits, n_high_bits, s, a, q);
+  lane_walk<N, R, MODE, S>(colpm, x0, pattern, n_walk_bits, n_high_bits, s, a, q);
   if (lane >= lanes) {
     s = {0.0, 0.0};
     a = 0.0;
@@ -353,47 +391,75 @@ constexpr int unroll_bits_for() {
 
 // Workgroups [block_begin, block_end) of pass P (all of them when
 // block_end == 0); partial of workgroup b goes to P.partials[b].
-template <int N, int MODE, int NU>
+template <int N, int MODE, class S>
 hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
   const uint64_t b0 = P.block_begin, b1 = P.block_end ? P.block_end : P.blocks;
   if (P.lanes == 0 || b1 <= b0) return hipSuccess;
   if (P.walk_bits < R || b1 > P.blocks) return hipErrorInvalidValue;
-  ryser_dense_kernel<N, R, MODE, NU><<<dim3((unsigned)(b1 - b0)), dim3(256), 0, stream>>>(
+  ryser_dense_kernel<N, R, MODE, S><<<dim3((unsigned)(b1 - b0)), dim3(256), 0, stream>>>(
       L.colpm, L.x0, P.partials + b0, b0 * 256, P.lanes, P.walk_bits, P.high_bits);
   return hipGetLastError();
 }
 
-// FP64 run: the FP64 kernel, pair variant NU.
-template <int N, int NU>
+// FP64 run: the FP64 kernel of shape S.
+template <int N, class S>
 hipError_t launch_fast(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
-  hipError_t e = launch_pass<N, kFast, NU>(L, L.main, stream);
+  hipError_t e = launch_pass<N, kFast, S>(L, L.main, stream);
   if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
   return e;
 }
 
-// Pair variants exist for NU = 4, 8, ..., N-4 when N >= kMinPairN.
-template <int N, int... Q>
-hipError_t launch_pair(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
-                       std::integer_sequence<int, Q...>) {
+// Shape j of K0 = 4(q+1) in the compiled menu (nest_menu_shape).
+template <int N, int Q, int J>
+hipError_t launch_shape(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
+                        bool& hit) {
+  constexpr int K0 = 4 * (Q + 1);
+  constexpr NestShape sh = nest_menu_shape(N, K0, J);
+  if constexpr (sh.k0 == 0) {
+    return hipErrorInvalidValue;
+  } else {
+    if (L.nu != sh.k0 || L.nu1 != sh.k1 || L.nu2 != sh.k2) return hipErrorInvalidValue;
+    hit = true;
+    return launch_fast<N, Nest<sh.k0, sh.k1, sh.k2>>(L, stream, main_done);
+  }
+}
+
+template <int N, int Q, int... J>
+hipError_t launch_k0(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done, bool& hit,
+                     std::integer_sequence<int, J...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((L.nu == 4 * (Q + 1) ? (e = launch_fast<N, 4 * (Q + 1)>(L, stream, main_done), 0) : 0), ...);
+  ((hit ? 0 : (e = launch_shape<N, Q, J>(L, stream, main_done, hit), 0)), ...);
   return e;
+}
+
+template <int N, int... Q>
+hipError_t launch_nested(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
+                         std::integer_sequence<int, Q...>) {
+  hipError_t e = hipErrorInvalidValue;
+  bool hit = false;
+  ((L.nu == 4 * (Q + 1) && !hit
+        ? (e = launch_k0<N, Q>(L, stream, main_done, hit,
+                               std::make_integer_sequence<int, kNestShapesPerK0>{}),
+           0)
+        : 0),
+   ...);
+  return hit ? e : hipErrorInvalidValue;
 }
 
 template <int N>
 hipError_t launch_n(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   if (L.mode == kDDProd || L.mode == kDDAll) {
-    hipError_t e = L.mode == kDDProd ? launch_pass<N, kDDProd, N>(L, L.main, stream)
-                                     : launch_pass<N, kDDAll, N>(L, L.main, stream);
+    hipError_t e = L.mode == kDDProd ? launch_pass<N, kDDProd, Nest<N, N, N>>(L, L.main, stream)
+                                     : launch_pass<N, kDDAll, Nest<N, N, N>>(L, L.main, stream);
     if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
     return e;
   }
-  if (L.nu == N || L.nu <= 0) return launch_fast<N, N>(L, stream, main_done);
+  if (L.nu == N || L.nu <= 0) return launch_fast<N, Nest<N, N, N>>(L, stream, main_done);
   if constexpr (N >= kMinPairN) {
     if (L.nu % 4 == 0 && L.nu < N)
-      return launch_pair<N>(L, stream, main_done,
-                            std::make_integer_sequence<int, (N - 1) / 4>{});
+      return launch_nested<N>(L, stream, main_done,
+                              std::make_integer_sequence<int, (N - 1) / 4>{});
   }
   return hipErrorInvalidValue;
 }

@@ -80,7 +80,8 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
   const uint32_t steps = 1u << n_walk_bits;
   const uint32_t groups = steps >> R;
   const uint32_t chunk_groups = 1u << (check_bits - R);
-  double ppass = 1.0, a_run = 0.0;
+  Tails tl;
+  double a_run = 0.0;
   double cv[N];
   load_col_n<N>(cv, cols, zero_blk);  // step 0 changes nothing
   uint32_t t = 0;
@@ -108,9 +109,10 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
     }
     // walk 2^check_bits steps; the first adds nothing (x is already at t)
     const uint32_t g0 = t >> R;
-    group_steps<N, R, MODE, N, 0, true>(x, xl, cv, cols, g0, groups, zero_blk, acc, ppass);
+    group_steps<N, R, MODE, Nest<N, N, N>, 0, true>(x, xl, cv, cols, g0, groups, zero_blk, acc,
+                                                   tl);
     for (uint32_t g = g0 + 1; g < g0 + chunk_groups; ++g) {
-      group_steps<N, R, MODE, N, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, ppass);
+      group_steps<N, R, MODE, Nest<N, N, N>, 0>(x, xl, cv, cols, g, groups, zero_blk, acc, tl);
       if constexpr (MODE == kFast) {
         if ((g & 31u) == 31u) {  // at least every 32 groups
           s = dd_add_d(s, acc.p);

@@ -40,7 +40,7 @@ class PermResultC(ctypes.Structure):
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
                 ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
                 ("nu", ctypes.c_int), ("main_ms", ctypes.c_double),
-                ("main_subsets", ctypes.c_double)]
+                ("main_subsets", ctypes.c_double), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int)]
 
 
 @dataclass
@@ -59,12 +59,14 @@ class PermResult:
     nu: int
     main_ms: float
     main_subsets: float
+    nu1: int = 0
+    nu2: int = 0
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
                    r.n, r.mode, r.integral, r.err_est, r.nu, r.main_ms,
-                   r.main_subsets)
+                   r.main_subsets, r.nu1, r.nu2)
 
 
 class BatchTuningC(ctypes.Structure):
