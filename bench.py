@@ -145,24 +145,26 @@ def batched_leg(P, count: int, seed: int):
     }
 
 
-def served_leg(P, count: int, load: float, seed: int):
-    """Served stream: `count` requests, n in [16, 36] with P(n) ~ 2^(-(n-16)/4),
-    density U[0.1, 0.9], half 0/1 and half FP64, a third through the sparse
-    path (seed 6).  Capacity: every request submitted at once (closed loop).
-    Then an open loop of Poisson arrivals at `load` x capacity; p50/p99 of
-    submit-to-result latency.  Plain: one drop-in call per request."""
+def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
+    """Served stream: `count` requests, n in [16, 36] with
+    P(n) ~ 2^(-(n-16)/decay), density U[0.1, 0.9], half 0/1 and half FP64, a
+    third through the sparse path.  Capacity: every request submitted at once
+    (closed loop).  Then an open loop of Poisson arrivals at `load` x
+    capacity; p50/p99 of submit-to-result latency.  Plain: one drop-in call
+    per request."""
     import threading
 
     import numpy as np
 
-    reqs = P.served_workload(count, seed)
-    work = sum(2.0 * m.shape[0] * 2.0 ** (m.shape[0] - 1) for m, _ in reqs)
+    # requests arrive as arrays (CCS built for the sparse ones) before any timing
+    reqs = [P.Request(m, sp) for m, sp in P.served_workload(count, seed, decay=decay)]
+    work = sum(2.0 * q.a.shape[0] * 2.0 ** (q.a.shape[0] - 1) for q in reqs)
     with P.Server() as srv:
-        for m, sp in reqs[:32]:  # warm the kernels of the common sizes
-            srv.wait(srv.submit(m, sparse_path=sp))
+        for q in reqs[:32]:  # warm the kernels of the common sizes
+            srv.wait(srv.submit_request(q))
         t0 = time.perf_counter()
-        tickets = [srv.submit(m, sparse_path=sp) for m, sp in reqs]
-        closed = [srv.wait(t) for t in tickets]
+        tickets = [srv.submit_request(q) for q in reqs]
+        closed = srv.wait_many(tickets)
         t_closed = time.perf_counter() - t0
     capacity = count / t_closed
     rate = load * capacity
@@ -188,11 +190,11 @@ def served_leg(P, count: int, load: float, seed: int):
         th = threading.Thread(target=collector)
         th.start()
         t0 = time.perf_counter()
-        for i, (m, sp) in enumerate(reqs):
+        for i, q in enumerate(reqs):
             delay = arrivals[i] - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            t = srv.submit(m, sparse_path=sp)
+            t = srv.submit_request(q)
             with cv:
                 queue.append(t)
                 cv.notify()
@@ -201,14 +203,12 @@ def served_leg(P, count: int, load: float, seed: int):
         t_open = time.perf_counter() - t0
     lat_ms = np.array(lat)
     t0 = time.perf_counter()
-    plain = []
-    for m, sp in reqs:
-        plain.append(P.sparse(m, algo_id=4).value if sp else P.dense(m).value)
+    plain = [P.solve(q).value for q in reqs]
     t_plain = time.perf_counter() - t0
     worst = max(abs(r.value - v) / max(abs(v), r.err_bound, 1e-300)
                 for (r, _), v in zip(closed, plain))
     return {
-        "workload": f"{count} requests, n in [16,36] with P(n)~2^(-(n-16)/4), density "
+        "workload": f"{count} requests, n in [16,36] with P(n)~2^(-(n-16)/{decay:g}), density "
                     f"U[0.1,0.9], half 0/1 half FP64, 1/3 sparse path, seed {seed}",
         "algorithmic_flops": work,
         "capacity_req_per_s": capacity,
@@ -321,6 +321,10 @@ def main():
         out["batched"] = batched_leg(P, args.batch_count, 5)
         if args.serve_count > 0:
             out["served"] = served_leg(P, args.serve_count, args.serve_load, 6)
+            # the other reading of the configuration's size law, P(n) ~ 2^-(n-16):
+            # mostly n = 16..18, per-request overhead dominates
+            out["served_small_n"] = served_leg(P, args.serve_count, args.serve_load, 7,
+                                               decay=1.0)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mat, args.cpu_sample_log2, args.cpu_threads)
     print(json.dumps(out), flush=True)

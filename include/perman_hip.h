@@ -323,12 +323,19 @@ typedef struct perm_server_config {
   int streams;            /* compute streams (default 4, at most 8) */
   int inflight;           /* iterations queued on the GPU (default 2, at most 4) */
   int max_items;          /* items per iteration (default 32768) */
-  double iteration_us;    /* work per iteration at full FP64 issue rate, microseconds (4000) */
+  double iteration_us;    /* work per iteration at full FP64 issue rate, microseconds (8000) */
   int max_resident;       /* matrices resident on the device (default 2048) */
   double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */
 } perm_server_config;
 typedef struct perm_server_stats {
   long long submitted, admitted, completed, iterations, items, pending;
+  long long uploads;      /* host-to-device copies of admitted matrices */
+  double admit_ms;        /* scheduler time admitting and uploading requests */
+  double issue_ms;        /* scheduler time building and launching iterations */
+  double retire_ms;       /* scheduler time summing and publishing results */
+  double gpu_wait_ms;     /* scheduler time blocked on the GPU */
+  double gpu_ms;          /* summed GPU span of the iterations (upload to result copy) */
+  long long dd_passes;    /* integer requests recomputed with double-double products */
 } perm_server_stats;
 void perm_server_default_config(perm_server_config* cfg);
 int perm_server_create(const perm_server_config* cfg, perm_server** server);
@@ -339,6 +346,11 @@ int perm_server_submit_sparse(perm_server* server, int dtype, const void* mat, c
                               unsigned long long* id);
 int perm_server_wait(perm_server* server, unsigned long long id, perm_result* out,
                      double* latency_ms);
+/* Waits for `count` requests in order; out[i] / latency_ms[i] (latency_ms
+ * may be NULL) as perm_server_wait.  Every id must be known; returns the
+ * first failing request's status after collecting all of them. */
+int perm_server_wait_many(perm_server* server, const unsigned long long* ids, int count,
+                          perm_result* out, double* latency_ms);
 int perm_server_poll(perm_server* server, unsigned long long id, perm_result* out,
                      double* latency_ms);
 int perm_server_stats_get(perm_server* server, perm_server_stats* out);

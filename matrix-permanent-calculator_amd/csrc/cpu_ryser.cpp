@@ -262,8 +262,12 @@ int skip_cpu(int dtype_unused, const int* rptrs, const int* cols_idx, const T* r
   out->kernel_ms = (omp_get_wtime() - t0) * 1e3;
   out->main_ms = out->kernel_ms;
   out->integral = integral ? 1 : 0;
-  out->nu = n;
+  out->nu = out->nu1 = out->nu2 = n;
   out->status = PERM_OK;
+  if (integral &&
+      perm::integer_sum_exact(true, perm::integer_product_bits(x0.data(), cs.data(), n),
+                              out->abs_sum, scale, n))
+    out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(integral, out->value, out->err_bound, out->err_est);
   return perm::ok();
 }

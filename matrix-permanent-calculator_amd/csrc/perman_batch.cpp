@@ -5,7 +5,8 @@
 // into items of one wave (64 lanes) walking 2^w Gray subsets per lane, with w
 // chosen from the batch's item size (2^item_log2 subsets per item) and n
 // alone -- so a matrix's items, and therefore its result, do not depend on
-// what else is in the batch.  Items are grouped by kernel variant (n, nu);
+// what else is in the batch.  Items are grouped by kernel variant (n and
+// walk shape);
 // one persistent kernel per variant drains its group, the groups running
 // concurrently on several streams.  The whole launch sequence (counter reset,
 // variant kernels, per-matrix reduction, result copy) is captured once in a
@@ -34,8 +35,10 @@ using perm::Partial;
 constexpr int kMaxStreams = 8;
 
 struct Entry {
-  int n = 0, nu = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  perm::NestShape shape{0, 0, 0};
   bool integral = false;
+  double product_bits = 1e9;
   size_t col_off = 0, x0_off = 0;  // offsets (doubles) into the data buffer
   uint32_t first_item = 0, item_count = 0;
 };
@@ -92,10 +95,12 @@ int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
   Entry e;
   e.n = n;
   e.integral = p.integral;
-  // the batch kernels carry a subset of the pair widths: widen nu to one
-  // of them (extra rows of zeros in column 0 are exact)
-  e.nu = perm::batch_nu(n, p.nu);
-  e.variant = perm::batch_variant(n, e.nu);
+  e.product_bits = p.product_bits;
+  // the cheapest batch kernel shape covering the prepared one (extra rows
+  // add zeros, exactly)
+  e.shape = perm::batch_shape(n, {p.nu > 0 ? p.nu : n, p.nu1 > 0 ? p.nu1 : n,
+                                  p.nu2 > 0 ? p.nu2 : n});
+  e.variant = perm::batch_variant(n, e.shape);
   if (e.variant < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel for this n");
   e.col_off = B->data.size();
   B->data.insert(B->data.end(), p.cs.begin(), p.cs.end());
@@ -145,7 +150,7 @@ int build(perm_batch* B) {
     order[v] = v;
     for (uint32_t m : by_var[v]) {
       const Entry& e = B->ents[m];
-      work[v] += std::ldexp((double)(e.n + e.nu), e.n - 1);
+      work[v] += std::ldexp((double)perm::nest_cost(e.n, e.shape), e.n - 4);
     }
   }
   std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
@@ -230,8 +235,8 @@ int build(perm_batch* B) {
 
 namespace perm {
 
-void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial& r,
-                      perm_result& o) {
+void fill_item_result(int n, int walk_bits, bool integral, double product_bits, NestShape shape,
+                      int mode, const Partial& r, perm_result& o) {
   const int high_bits = n - 1 - walk_bits;
   std::memset(&o, 0, sizeof(o));
   const double scale = (double)(4 * (n & 1) - 2);
@@ -241,14 +246,18 @@ void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial
   o.value = scale * (r.hi + r.lo);
   o.abs_sum = std::fabs(scale) * r.abs_sum;
   o.err_bound =
-      error_statement(o.abs_sum, o.value, n, walk_bits, dense_unroll_bits(n), integral, 0);
+      error_statement(o.abs_sum, o.value, n, walk_bits, dense_unroll_bits(n), integral, mode);
   o.err_est =
-      error_estimate(scale * scale * r.sq_sum, o.value, n, walk_bits, high_bits, 0, integral);
+      error_estimate(scale * scale * r.sq_sum, o.value, n, walk_bits, high_bits, mode, integral);
   o.err_est = std::min(o.err_est, o.err_bound);
   o.subsets = std::ldexp(1.0, n - 1);
   o.integral = integral ? 1 : 0;
-  o.nu = nu;
-  o.mode = 0;
+  o.nu = shape.k0;
+  o.nu1 = shape.k1;
+  o.nu2 = shape.k2;
+  o.mode = mode;
+  if (integer_sum_exact(integral, product_bits, o.abs_sum, scale, n))
+    o.err_bound = o.err_est = 0.0;  // exact arithmetic throughout
   round_integral(integral, o.value, o.err_bound, o.err_est);
   o.status = PERM_OK;
 }
@@ -306,7 +315,7 @@ int perm_batch_set(perm_batch* B, int item_log2, int streams, int max_grid) {
 int perm_batch_add_dense(perm_batch* B, int dtype, const void* mat, int nov, int* index) {
   if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
   perm::Prepared p;
-  if (int s = perm::prepare_dense(dtype, mat, nov, p, false)) return s;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p, perm::kMenuBatch)) return s;
   return add_prepared(B, nov, p, index);
 }
 
@@ -314,7 +323,7 @@ int perm_batch_add_sparse(perm_batch* B, int dtype, const void* mat, const int* 
                           const int* rows, const void* cvals, int nov, int* index) {
   if (!B) return perm::fail(PERM_ERR_ARG, "null batch");
   perm::Prepared p;
-  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, false)) return s;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, perm::kMenuBatch)) return s;
   return add_prepared(B, nov, p, index);
 }
 
@@ -334,7 +343,8 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   if (device_ms) *device_ms = ms;
   for (size_t m = 0; m < B->ents.size(); ++m) {
     const Entry& e = B->ents[m];
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.nu, B->h_results[m], results[m]);
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 0,
+                           B->h_results[m], results[m]);
     results[m].kernel_ms = ms;
     if (perm::needs_dd_pass(results[m])) {
       // integer matrix whose FP64 error could reach 1/2: double-double
@@ -343,8 +353,9 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
       const size_t cs_len = (size_t)(2 * (e.n - 1) + 1) * e.n;
       p.cs.assign(B->data.begin() + e.col_off, B->data.begin() + e.col_off + cs_len);
       p.x0.assign(B->data.begin() + e.x0_off, B->data.begin() + e.x0_off + e.n);
-      p.nu = e.n;
+      p.nu = p.nu1 = p.nu2 = e.n;
       p.integral = true;
+      p.product_bits = e.product_bits;
       if (int s = perm::run_prepared(e.n, p, 1, &results[m])) return s;
     }
   }

@@ -66,6 +66,7 @@ struct perm_plan {
   int nu = 0;             // rows touched by Gray column 0 for the FP64 kernel (n: none)
   int nu1 = 0, nu2 = 0;   // rows touched by Gray columns 0-1 and 0-2 (n: not nested)
   bool integral = false;  // every entry an integer of magnitude < 2^31
+  double product_bits = 1e9;  // integer matrices: log2 bound of the walk's product numerators
   bool borrowed = false;  // stream/events/memory belong to the device cache
 };
 
@@ -131,6 +132,7 @@ void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
 
 // The FP64 kernel's walk shape from the host preparation (0: not restricted).
 void set_shape(perm_plan* P, int n, const Prepared& prep) {
+  P->product_bits = prep.product_bits;
   P->nu = prep.nu > 0 ? prep.nu : n;
   P->nu1 = prep.nu1 > 0 ? prep.nu1 : n;
   P->nu2 = prep.nu2 > 0 ? prep.nu2 : n;
@@ -224,6 +226,8 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->err_est = perm::error_estimate(scale * scale * r.sq_sum, out->value, n, P->walk_bits,
                                       P->high_bits, mode, P->integral);
   out->err_est = std::min(out->err_est, out->err_bound);
+  if (perm::integer_sum_exact(P->integral, P->product_bits, out->abs_sum, scale, n))
+    out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(P->integral, out->value, out->err_bound, out->err_est);
   out->status = PERM_OK;
   out->mode = mode;

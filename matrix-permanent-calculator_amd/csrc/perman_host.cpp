@@ -105,13 +105,14 @@ int select_device() {
 // ordered [column 0's rows | column 1's new rows | column 2's new rows |
 // the rest] (ascending index inside each part), so column l touches only
 // rows [0, K_l).  Row and column permutations leave the permanent unchanged.
-// `nested` false: only pair shapes (Gray column 0 restricted).
+// `menu`: the shapes the kernels that will run the matrix provide.
 // PERMAN_PAIR=0 keeps the natural order; PERMAN_NEST=0 only pair shapes.
-void nest_order(int n, bool nested, std::vector<double>& x0, std::vector<double>& cols,
+void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector<double>& cols,
                 int& nu, int& nu1, int& nu2) {
   nu = nu1 = nu2 = n;
   if (n < perm::kMinPairN || env_int("PERMAN_PAIR", 1) == 0) return;
-  if (env_int("PERMAN_NEST", 1) == 0) nested = false;
+  if (env_int("PERMAN_NEST", 1) == 0) menu.mask &= 1u;
+  const bool nested = (menu.mask & ~1u) != 0 && n >= perm::kMinNestN;
   const int B = n - 1;
   std::vector<uint64_t> sup(B, 0);
   for (int k = 0; k < B; ++k)
@@ -131,8 +132,9 @@ void nest_order(int n, bool nested, std::vector<double>& x0, std::vector<double>
     int cost = -1;
     for (int k0 = std::max(4, (c0 + 3) / 4 * 4); k0 <= n - 4; k0 += 4)
       for (int j = 0; j < perm::kNestShapesPerK0; ++j) {
+        if (!perm::nest_menu_has(menu, n, k0, j)) continue;
         const perm::NestShape s = perm::nest_menu_shape(n, k0, j);
-        if (s.k0 == 0 || s.k1 < c01 || s.k2 < c012) continue;
+        if (s.k1 < c01 || s.k2 < c012) continue;
         const int c = perm::nest_cost(n, s);
         if (cost < 0 || c < cost) {
           cost = c;
@@ -206,12 +208,13 @@ bool integral_matrix(const std::vector<double>& a) {
   return true;
 }
 
-void finish_prepare(int n, bool nested, std::vector<double>& cols, Prepared& p) {
-  nest_order(n, nested, p.x0, cols, p.nu, p.nu1, p.nu2);
+void finish_prepare(int n, const NestMenu& menu, std::vector<double>& cols, Prepared& p) {
+  nest_order(n, menu, p.x0, cols, p.nu, p.nu1, p.nu2);
+  if (p.integral) p.product_bits = perm::integer_product_bits(p.x0.data(), cols.data(), n);
   perm::build_column_store(n, [&](int k, int j) { return cols[(size_t)k * n + j]; }, p.cs);
 }
 
-int prepare_dense(int dtype, const void* mat, int n, Prepared& p, bool nested) {
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p, const NestMenu& menu) {
   if (!mat) return fail(PERM_ERR_ARG, "null pointer");
   if (int s = check_n(n)) return s;
   std::vector<double> a;
@@ -222,7 +225,7 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p, bool nested) {
   std::vector<double> cols((size_t)(n - 1) * n);
   for (int k = 0; k < n - 1; ++k)
     for (int j = 0; j < n; ++j) cols[(size_t)k * n + j] = a[(size_t)j * n + k];
-  finish_prepare(n, nested, cols, p);
+  finish_prepare(n, menu, cols, p);
   return PERM_OK;
 }
 
@@ -255,10 +258,10 @@ int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
 }
 
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, Prepared& p, bool nested) {
+                   const void* cvals, int n, Prepared& p, const NestMenu& menu) {
   std::vector<double> cols;
   if (int s = gather_sparse(dtype, mat, cptrs, rows, cvals, n, p.x0, cols, p.integral)) return s;
-  finish_prepare(n, nested, cols, p);
+  finish_prepare(n, menu, cols, p);
   return PERM_OK;
 }
 

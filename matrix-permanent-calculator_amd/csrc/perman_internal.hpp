@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "nest_shape.hpp"
+
 namespace perm {
 
 // Per-thread status (perm_last_status / perm_last_error) and device.
@@ -28,12 +30,14 @@ struct Prepared {
   std::vector<double> x0, cs;
   int nu = 0, nu1 = 0, nu2 = 0;
   bool integral = false;
+  double product_bits = 1e9;  // integer_product_bits (integer matrices)
 };
-// nested: allow nested walk shapes (single-matrix paths); false: pair
-// shapes only (the batch and server kernels).
-int prepare_dense(int dtype, const void* mat, int n, Prepared& p, bool nested = true);
+// menu: the walk shapes the kernels that will run the matrix are compiled
+// for (kMenuSingle: single-matrix paths; kMenuBatch: batch and server).
+int prepare_dense(int dtype, const void* mat, int n, Prepared& p,
+                  const NestMenu& menu = kMenuSingle);
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
-                   const void* cvals, int n, Prepared& p, bool nested = true);
+                   const void* cvals, int n, Prepared& p, const NestMenu& menu = kMenuSingle);
 // The sparse path's inputs before any reordering: x0 from the dense matrix,
 // cols[k*n + j] = CCS value of (j, k) for the walked columns k < n-1.
 int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
@@ -56,8 +60,8 @@ struct Partial;
 // Result of a matrix computed as 64-lane items (batch and server paths):
 // value, error statement and estimate from the fixed-order sum of its item
 // partials; integer results with error below 1/2 rounded (round_integral).
-void fill_item_result(int n, int walk_bits, bool integral, int nu, const Partial& r,
-                      perm_result& o);
+void fill_item_result(int n, int walk_bits, bool integral, double product_bits, NestShape shape,
+                      int mode, const Partial& r, perm_result& o);
 // An integer matrix whose FP64 result could be off by 1/2 or more while
 // below 2^53: run again with double-double products (mode 1).
 bool needs_dd_pass(const perm_result& o);

@@ -10,7 +10,8 @@
 //  * iterations: each iteration takes 64-lane items (the batch kernels'
 //    items, ryser_batch.hip) from the admitted requests, shortest remaining
 //    work first, up to a work budget of a few milliseconds, and launches one
-//    persistent kernel per (n, nu) variant present over several streams;
+//    persistent kernel per variant (n and walk shape) present over several
+//    streams;
 //    up to `inflight` iterations are queued on the GPU at once, so the next
 //    one is ready when the previous drains;
 //  * completion: an iteration's item partials come back to pinned memory
@@ -19,18 +20,22 @@
 //    result depends only on the matrix, bitwise the same alone or in any mix
 //    and whatever iterations its items ran in;
 //  * integer matrices whose FP64 error bound reaches 1/2 get a
-//    double-double product pass through the single-matrix path on a helper
+//    double-double product pass: the request goes back to the active list
+//    with the batch kernels' double-double variant, its matrix still in its
+//    arena slot
 //    thread, so the scheduler never blocks on it.
 //
 // Iterations change every time, so they are launched directly (no graph).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <queue>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -64,7 +69,9 @@ constexpr size_t kSlotDoubles =
 
 struct Request {
   uint64_t id = 0;
-  int n = 0, nu = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
+  int mode = 0;  // 0 FP64 items; 1 the double-double pass of an integer matrix
+  perm::NestShape shape{0, 0, 0};
   bool integral = false;
   perm::Prepared prep;
   int slot = -1;
@@ -86,15 +93,13 @@ struct Segment {
 
 struct Iteration {
   BatchMatrix* h_mats = nullptr;
-  BatchItem* h_items = nullptr;
   Partial* h_parts = nullptr;
   BatchMatrix* d_mats = nullptr;
-  BatchItem* d_items = nullptr;
   Partial* d_parts = nullptr;
   uint32_t* d_counters = nullptr;
   void* dmem = nullptr;
   void* hmem = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t start = nullptr, done = nullptr;  // timed: the iteration's GPU span
   std::vector<Segment> segs;
   bool busy = false;
 };
@@ -106,22 +111,27 @@ struct perm_server {
   int device = 0;
   size_t max_items = 0;
   double iter_cost = 0.0;
-  std::thread worker, helper;
+  std::thread worker;
   std::mutex mu;
-  std::condition_variable cv_submit, cv_done, cv_helper;
+  std::condition_variable cv_submit, cv_done;
   std::deque<Request*> pending;                  // submitted, not admitted (mu)
-  std::deque<Request*> dd_queue;                 // waiting for a double-double pass (mu)
   std::unordered_map<uint64_t, Request*> table;  // every uncollected request (mu)
   uint64_t next_id = 1;
   bool stop = false;
-  bool worker_done = false;  // the scheduler has returned (mu)
   int fatal = PERM_OK;  // scheduler error (mu): every later call fails with it
   std::string fatal_msg;
   perm_server_stats stats{};  // (mu)
   // scheduler-owned
   std::vector<Request*> active;
-  std::vector<int> free_slots;
+  // free arena slots, smallest first (requests admitted together get
+  // neighbouring slots, uploaded by one copy)
+  std::priority_queue<int, std::vector<int>, std::greater<int>> free_slots;
   double queued_cost = 0.0;
+  // scheduler time per phase (ms) and upload copies: written by the
+  // scheduler only, read by perm_server_stats_get
+  std::atomic<double> t_admit{0.0}, t_issue{0.0}, t_retire{0.0}, t_wait{0.0}, t_gpu{0.0};
+  std::atomic<long long> n_uploads{0};
+  std::atomic<long long> n_dd{0};  // requests given a double-double pass
   double* d_arena = nullptr;
   double* h_arena = nullptr;  // pinned mirror of the arena (upload staging)
   Iteration it[kMaxInflight];
@@ -142,26 +152,30 @@ namespace {
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
 
-void fail_request(perm_server* S, Request* r, int code, const std::string& msg) {
-  std::lock_guard<std::mutex> lk(S->mu);
-  r->status = code;
-  r->error = msg;
-  r->finished = true;
-  r->t_done = Clock::now();
+// Publish a retired iteration's finished requests under one lock, one wake-up.
+void publish_all(perm_server* S, const std::vector<Request*>& rs) {
+  if (rs.empty()) return;
+  const Clock::time_point now = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Request* r : rs) {
+      r->finished = true;
+      r->t_done = now;
+    }
+    S->stats.completed += (long long)rs.size();
+  }
   S->cv_done.notify_all();
 }
 
-void publish(perm_server* S, Request* r) {
-  std::lock_guard<std::mutex> lk(S->mu);
-  r->finished = true;
-  r->t_done = Clock::now();
-  S->stats.completed += 1;
-  S->cv_done.notify_all();
-}
+// FP64 VALU lane-instructions per subset of a double-double product pass
+// (TwoProd chains and TwoSum accumulation; scheduling estimate only).
+double dd_subset_cost(int n) { return 4.0 * n + 12.0; }
 
-// All items of r are done: sum its partials in item order, state the
-// error, hand integer matrices that need it to the double-double helper.
-void finalize(perm_server* S, Request* r) {
+// All items of r are done: sum its partials in item order and state the
+// error.  An integer matrix whose FP64 error could reach 1/2 goes back to the
+// active list for a double-double product pass over the same slot (returns
+// false); otherwise the slot is freed and r is ready to publish (true).
+bool finalize(perm_server* S, Request* r) {
   perm::HostDD s = {0.0, 0.0};
   Partial tot{0.0, 0.0, 0.0, 0.0};
   for (const Partial& p : r->parts) {
@@ -171,53 +185,81 @@ void finalize(perm_server* S, Request* r) {
   }
   tot.hi = s.hi;
   tot.lo = s.lo;
-  perm::fill_item_result(r->n, r->walk_bits, r->integral, r->nu, tot, r->result);
+  perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
+                         r->mode, tot, r->result);
+  if (r->mode == 0 && perm::needs_dd_pass(r->result)) {
+    r->mode = 1;
+    r->shape = {r->n, r->n, r->n};
+    r->variant = perm::batch_variant(r->n, r->shape, 1);
+    r->issued = r->done = 0;
+    std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0});
+    r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n);
+    S->queued_cost += (double)r->items * r->item_cost;
+    S->active.push_back(r);
+    S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
+    return false;
+  }
   r->parts.clear();
   r->parts.shrink_to_fit();
-  S->free_slots.push_back(r->slot);
+  S->free_slots.push(r->slot);
   r->slot = -1;
-  if (perm::needs_dd_pass(r->result)) {
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->dd_queue.push_back(r);
-    S->cv_helper.notify_one();
-    return;
-  }
   r->prep = perm::Prepared();
-  publish(S, r);
+  return true;
 }
 
-// Copy newly admitted requests into their arena slots (pinned mirror, then
-// one asynchronous copy each on s0, ahead of any kernel that reads them).
-int upload(perm_server* S, Request* r) {
-  double* h = S->h_arena + (size_t)r->slot * kSlotDoubles;
-  std::memcpy(h, r->prep.cs.data(), r->prep.cs.size() * sizeof(double));
-  std::memcpy(h + r->prep.cs.size(), r->prep.x0.data(), r->prep.x0.size() * sizeof(double));
-  const size_t bytes = (r->prep.cs.size() + r->prep.x0.size()) * sizeof(double);
-  HIP_TRYS(hipMemcpyAsync(S->d_arena + (size_t)r->slot * kSlotDoubles, h, bytes,
-                          hipMemcpyHostToDevice, S->s0));
+// Copy newly admitted requests into their arena slots: each into the pinned
+// mirror, then one asynchronous copy per run of consecutive slots on s0,
+// ahead of any kernel that reads them.
+int upload(perm_server* S, std::vector<Request*>& fresh) {
+  std::sort(fresh.begin(), fresh.end(),
+            [](const Request* a, const Request* b) { return a->slot < b->slot; });
+  for (size_t i = 0; i < fresh.size();) {
+    size_t j = i;
+    for (;;) {
+      Request* r = fresh[j];
+      double* h = S->h_arena + (size_t)r->slot * kSlotDoubles;
+      std::memcpy(h, r->prep.cs.data(), r->prep.cs.size() * sizeof(double));
+      std::memcpy(h + r->prep.cs.size(), r->prep.x0.data(), r->prep.x0.size() * sizeof(double));
+      if (j + 1 == fresh.size() || fresh[j + 1]->slot != r->slot + 1) break;
+      ++j;
+    }
+    const Request* last = fresh[j];
+    const size_t first = (size_t)fresh[i]->slot * kSlotDoubles;
+    const size_t end = (size_t)last->slot * kSlotDoubles + last->prep.cs.size() + last->prep.x0.size();
+    HIP_TRYS(hipMemcpyAsync(S->d_arena + first, S->h_arena + first, (end - first) * sizeof(double),
+                            hipMemcpyHostToDevice, S->s0));
+    S->n_uploads.store(S->n_uploads.load(std::memory_order_relaxed) + 1,
+                       std::memory_order_relaxed);
+    i = j + 1;
+  }
   return PERM_OK;
 }
 
 int admit(perm_server* S) {
+  std::vector<Request*> fresh;
   for (;;) {
     Request* r = nullptr;
     {
       std::lock_guard<std::mutex> lk(S->mu);
-      if (S->pending.empty() || S->free_slots.empty()) return PERM_OK;
+      if (S->pending.empty() || S->free_slots.empty()) break;
       Request* c = S->pending.front();
       const double cost = (double)c->items * c->item_cost;
-      if (!S->active.empty() && S->queued_cost + cost > S->cfg.max_queued_work) return PERM_OK;
+      if (!(S->active.empty() && fresh.empty()) && S->queued_cost + cost > S->cfg.max_queued_work)
+        break;
       S->pending.pop_front();
       r = c;
       S->stats.admitted += 1;
     }
-    r->slot = S->free_slots.back();
-    S->free_slots.pop_back();
+    r->slot = S->free_slots.top();
+    S->free_slots.pop();
     r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0});
     S->queued_cost += (double)r->items * r->item_cost;
-    if (int st = upload(S, r)) return st;
-    S->active.push_back(r);
+    fresh.push_back(r);
   }
+  if (fresh.empty()) return PERM_OK;
+  if (int st = upload(S, fresh)) return st;
+  for (Request* r : fresh) S->active.push_back(r);
+  return PERM_OK;
 }
 
 // Choose the next iteration's items and launch it on a free context.
@@ -225,9 +267,16 @@ int issue(perm_server* S, bool& issued) {
   issued = false;
   if (S->n_busy >= S->cfg.inflight) return PERM_OK;
   std::vector<Request*> cand;
+  double work = 0.0;
   for (Request* r : S->active)
-    if (r->issued < r->items) cand.push_back(r);
+    if (r->issued < r->items) {
+      cand.push_back(r);
+      work += r->remaining_cost();
+    }
   if (cand.empty()) return PERM_OK;
+  // continuous batching: while an iteration runs, let small work accumulate
+  // into the next one instead of paying an iteration's launch cost for it
+  if (S->n_busy > 0 && work < S->iter_cost / 16.0) return PERM_OK;
   std::sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
     const double ca = a->remaining_cost(), cb = b->remaining_cost();
     return ca != cb ? ca < cb : a->id < b->id;
@@ -257,6 +306,9 @@ int issue(perm_server* S, bool& issued) {
   };
   std::vector<Group> groups;
   uint32_t pos = 0;
+  // staging: matrix records, then (from the next 256-byte boundary) items
+  const size_t mats_bytes = align256(I.segs.size() * sizeof(BatchMatrix));
+  BatchItem* h_items = (BatchItem*)((char*)I.h_mats + mats_bytes);
   for (size_t m = 0; m < I.segs.size(); ++m) {
     Segment& g = I.segs[m];
     Request* r = g.r;
@@ -265,7 +317,7 @@ int issue(perm_server* S, bool& issued) {
                    S->d_arena + (size_t)r->slot * kSlotDoubles + r->prep.cs.size(),
                    1ull << r->high_bits, r->walk_bits, r->high_bits, r->variant, 0u, 0u, 0u};
     for (uint32_t i = 0; i < g.count; ++i)
-      I.h_items[pos + i] = {(uint32_t)m, 0u, (uint64_t)(g.first + i) * perm::kItemLanes};
+      h_items[pos + i] = {(uint32_t)m, 0u, (uint64_t)(g.first + i) * perm::kItemLanes};
     if (groups.empty() || groups.back().vid != r->variant)
       groups.push_back({r->variant, pos, pos + g.count});
     else
@@ -274,20 +326,26 @@ int issue(perm_server* S, bool& issued) {
   }
   const uint32_t n_items = pos;
   hipStream_t s0 = S->s0;
-  HIP_TRYS(hipMemcpyAsync(I.d_mats, I.h_mats, I.segs.size() * sizeof(BatchMatrix),
-                          hipMemcpyHostToDevice, s0));
-  HIP_TRYS(hipMemcpyAsync(I.d_items, I.h_items, n_items * sizeof(BatchItem),
+  // one upload: the matrix records, then the items
+  BatchItem* d_items = (BatchItem*)((char*)I.d_mats + mats_bytes);
+  HIP_TRYS(hipEventRecord(I.start, s0));
+  HIP_TRYS(hipMemcpyAsync(I.d_mats, I.h_mats, mats_bytes + n_items * sizeof(BatchItem),
                           hipMemcpyHostToDevice, s0));
   HIP_TRYS(perm::launch_batch_reset(I.d_counters, (uint32_t)groups.size(), s0));
-  const int ns = std::max(1, std::min(S->cfg.streams, kMaxStreams));
-  HIP_TRYS(hipEventRecord(S->ev_fork, s0));
-  for (int i = 0; i < ns; ++i) HIP_TRYS(hipStreamWaitEvent(S->s[i], S->ev_fork, 0));
+  // variants run concurrently on the streams they need (one group: on s0)
+  const int ns = (int)std::min<size_t>(groups.size() > 1 ? groups.size() : 0,
+                                       (size_t)std::max(1, std::min(S->cfg.streams, kMaxStreams)));
+  if (ns > 0) {
+    HIP_TRYS(hipEventRecord(S->ev_fork, s0));
+    for (int i = 0; i < ns; ++i) HIP_TRYS(hipStreamWaitEvent(S->s[i], S->ev_fork, 0));
+  }
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     const Group& G = groups[gi];
     const uint32_t waves = G.end - G.begin;
     const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, 2048u));
-    HIP_TRYS(perm::launch_batch_variant(G.vid, I.d_mats, I.d_items, G.begin, G.end,
-                                        I.d_counters + gi, I.d_parts, grid, S->s[gi % ns]));
+    HIP_TRYS(perm::launch_batch_variant(G.vid, I.d_mats, d_items, G.begin, G.end,
+                                        I.d_counters + gi, I.d_parts, grid,
+                                        ns > 0 ? S->s[gi % ns] : s0));
   }
   for (int i = 0; i < ns; ++i) {
     HIP_TRYS(hipEventRecord(S->ev_join[i], S->s[i]));
@@ -318,16 +376,24 @@ int retire(perm_server* S, bool wait, bool& retired) {
     if (q != hipSuccess)
       return perm::fail(PERM_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
   }
+  float gpu_ms = 0.f;
+  if (hipEventElapsedTime(&gpu_ms, I.start, I.done) == hipSuccess)
+    S->t_gpu.store(S->t_gpu.load(std::memory_order_relaxed) + gpu_ms, std::memory_order_relaxed);
+  std::vector<Request*> completed, finished;
   for (const Segment& g : I.segs) {
     Request* r = g.r;
     std::copy(I.h_parts + g.pos, I.h_parts + g.pos + g.count, r->parts.begin() + g.first);
     r->done += g.count;
     S->queued_cost -= (double)g.count * r->item_cost;
-    if (r->done == r->items) {
-      S->active.erase(std::find(S->active.begin(), S->active.end(), r));
-      finalize(S, r);
-    }
+    if (r->done == r->items) completed.push_back(r);
   }
+  if (!completed.empty())  // drop them from the active list in one pass, first
+    S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
+                                   [](const Request* r) { return r->done == r->items; }),
+                    S->active.end());
+  for (Request* r : completed)
+    if (finalize(S, r)) finished.push_back(r);
+  publish_all(S, finished);
   I.segs.clear();
   I.busy = false;
   S->head = (S->head + 1) % S->cfg.inflight;
@@ -336,14 +402,30 @@ int retire(perm_server* S, bool wait, bool& retired) {
   return PERM_OK;
 }
 
+void add_ms(std::atomic<double>& acc, Clock::time_point t0) {
+  const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+  acc.store(acc.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
+}
+
 int scheduler_step(perm_server* S, bool& idle) {
   idle = false;
+  Clock::time_point t0 = Clock::now();
   if (int st = admit(S)) return st;
+  add_ms(S->t_admit, t0);
+  t0 = Clock::now();
   bool issued = false, retired = false;
   if (int st = issue(S, issued)) return st;
+  add_ms(S->t_issue, t0);
+  t0 = Clock::now();
   if (int st = retire(S, false, retired)) return st;
+  add_ms(S->t_retire, t0);
   if (issued || retired) return PERM_OK;
-  if (S->n_busy >= S->cfg.inflight) return retire(S, true, retired);  // GPU queue full
+  if (S->n_busy >= S->cfg.inflight) {  // GPU queue full
+    t0 = Clock::now();
+    const int st = retire(S, true, retired);
+    add_ms(S->t_wait, t0);
+    return st;
+  }
   if (S->n_busy > 0) {
     // nothing to issue: wait briefly for the GPU or a new request
     std::unique_lock<std::mutex> lk(S->mu);
@@ -370,7 +452,6 @@ void scheduler_loop(perm_server* S) {
           kv.second->error = S->fatal_msg;
           kv.second->finished = true;
         }
-      S->dd_queue.clear();
       S->cv_done.notify_all();
       return;
     }
@@ -386,35 +467,6 @@ void scheduler_main(perm_server* S) {
   perm::set_device(S->device);
   (void)hipSetDevice(S->device);
   scheduler_loop(S);
-  std::lock_guard<std::mutex> lk(S->mu);
-  S->worker_done = true;
-  S->cv_helper.notify_all();
-}
-
-// Double-double product passes of integer matrices (single-matrix path).
-void helper_main(perm_server* S) {
-  perm::set_device(S->device);
-  for (;;) {
-    Request* r = nullptr;
-    {
-      std::unique_lock<std::mutex> lk(S->mu);
-      // runs until the scheduler has returned and the queue is empty
-      S->cv_helper.wait(lk, [&] { return !S->dd_queue.empty() || S->worker_done; });
-      if (S->dd_queue.empty()) return;
-      r = S->dd_queue.front();
-      S->dd_queue.pop_front();
-    }
-    perm_result out;
-    std::memset(&out, 0, sizeof(out));
-    const int st = perm::run_prepared(r->n, r->prep, 1, &out);
-    if (st != PERM_OK) {
-      fail_request(S, r, st, perm::last_error());
-      continue;
-    }
-    r->result = out;
-    r->prep = perm::Prepared();
-    publish(S, r);
-  }
 }
 
 int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
@@ -424,16 +476,22 @@ int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
   perm::batch_split(n, S->cfg.item_log2, r->walk_bits, r->high_bits);
   const uint64_t lanes = 1ull << r->high_bits;
   r->items = (uint32_t)((lanes + perm::kItemLanes - 1) / perm::kItemLanes);
-  // pair kernels for requests with real work; tiny ones use the plain
-  // variant (fewer kernels per iteration)
-  r->nu = r->items >= 8 ? perm::batch_nu(n, p.nu) : n;
-  r->variant = perm::batch_variant(n, r->nu);
+  // pair / nested kernels for requests with real work; tiny ones use the
+  // plain variant (fewer kernels per iteration)
+  r->shape = r->items >= 8 ? perm::batch_shape(n, {p.nu > 0 ? p.nu : n, p.nu1 > 0 ? p.nu1 : n,
+                                                   p.nu2 > 0 ? p.nu2 : n})
+                           : perm::NestShape{n, n, n};
+  r->variant = perm::batch_variant(n, r->shape);
   if (r->variant < 0) {
     delete r;
     return perm::fail(PERM_ERR_ARG, "no batch kernel for this n");
   }
   r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
-                 (double)(r->nu < n ? n + r->nu + 2 : 2 * n + 2);
+                 (double)perm::nest_cost(n, r->shape) / 8.0;
+  if (perm::batch_variant(n, {n, n, n}, 1) < 0) {
+    delete r;
+    return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
+  }
   r->prep = std::move(p);
   r->t_submit = Clock::now();
   {
@@ -477,6 +535,7 @@ void release(perm_server* S) {
     if (I.dmem) (void)hipFree(I.dmem);
     if (I.hmem) (void)hipHostFree(I.hmem);
     if (I.done) (void)hipEventDestroy(I.done);
+    if (I.start) (void)hipEventDestroy(I.start);
   }
   if (S->d_arena) (void)hipFree(S->d_arena);
   if (S->h_arena) (void)hipHostFree(S->h_arena);
@@ -496,8 +555,7 @@ int setup(perm_server* S) {
   HIP_TRYS(hipMalloc((void**)&S->d_arena, slots * kSlotDoubles * sizeof(double)));
   HIP_TRYS(hipHostMalloc((void**)&S->h_arena, slots * kSlotDoubles * sizeof(double),
                          hipHostMallocDefault));
-  S->free_slots.resize(slots);
-  for (size_t i = 0; i < slots; ++i) S->free_slots[i] = (int)(slots - 1 - i);
+  for (size_t i = 0; i < slots; ++i) S->free_slots.push((int)i);
   HIP_TRYS(hipStreamCreateWithFlags(&S->s0, hipStreamNonBlocking));
   for (int i = 0; i < kMaxStreams; ++i) {
     HIP_TRYS(hipStreamCreateWithFlags(&S->s[i], hipStreamNonBlocking));
@@ -505,24 +563,23 @@ int setup(perm_server* S) {
   }
   HIP_TRYS(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
   const size_t mi = S->max_items;
-  const size_t mats_b = align256(mi * sizeof(BatchMatrix));
-  const size_t items_b = align256(mi * sizeof(BatchItem));
+  // [matrix records | items] are staged and uploaded together
+  const size_t mats_b = align256(mi * sizeof(BatchMatrix)) + align256(mi * sizeof(BatchItem));
   const size_t parts_b = align256(mi * sizeof(Partial));
   const size_t ctr_b = align256(mi * sizeof(uint32_t));
   for (int k = 0; k < c.inflight; ++k) {
     Iteration& I = S->it[k];
-    HIP_TRYS(hipMalloc(&I.dmem, mats_b + items_b + parts_b + ctr_b));
-    HIP_TRYS(hipHostMalloc(&I.hmem, mats_b + items_b + parts_b, hipHostMallocDefault));
+    HIP_TRYS(hipMalloc(&I.dmem, mats_b + parts_b + ctr_b));
+    HIP_TRYS(hipHostMalloc(&I.hmem, mats_b + parts_b, hipHostMallocDefault));
     char* d = (char*)I.dmem;
     char* h = (char*)I.hmem;
     I.d_mats = (BatchMatrix*)d;
-    I.d_items = (BatchItem*)(d + mats_b);
-    I.d_parts = (Partial*)(d + mats_b + items_b);
-    I.d_counters = (uint32_t*)(d + mats_b + items_b + parts_b);
+    I.d_parts = (Partial*)(d + mats_b);
+    I.d_counters = (uint32_t*)(d + mats_b + parts_b);
     I.h_mats = (BatchMatrix*)h;
-    I.h_items = (BatchItem*)(h + mats_b);
-    I.h_parts = (Partial*)(h + mats_b + items_b);
-    HIP_TRYS(hipEventCreateWithFlags(&I.done, hipEventDisableTiming));
+    I.h_parts = (Partial*)(h + mats_b);
+    HIP_TRYS(hipEventCreate(&I.done));
+    HIP_TRYS(hipEventCreate(&I.start));
   }
   return PERM_OK;
 }
@@ -537,7 +594,7 @@ void perm_server_default_config(perm_server_config* c) {
   c->streams = 4;
   c->inflight = 2;
   c->max_items = 32768;
-  c->iteration_us = 4000.0;
+  c->iteration_us = 8000.0;
   c->max_resident = 2048;
   c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)
 }
@@ -570,7 +627,6 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
     return perm::fail(st, msg);
   }
   S->worker = std::thread(scheduler_main, S);
-  S->helper = std::thread(helper_main, S);
   *out = S;
   return perm::ok();
 }
@@ -582,7 +638,7 @@ int perm_server_submit_dense(perm_server* S, int dtype, const void* mat, int nov
     return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
                                          std::to_string(perm::kMaxBatchN));
   perm::Prepared p;
-  if (int s = perm::prepare_dense(dtype, mat, nov, p, false)) return s;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p, perm::kMenuBatch)) return s;
   return submit(S, nov, p, id);
 }
 
@@ -594,7 +650,7 @@ int perm_server_submit_sparse(perm_server* S, int dtype, const void* mat, const 
     return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
                                          std::to_string(perm::kMaxBatchN));
   perm::Prepared p;
-  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, false)) return s;
+  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, perm::kMenuBatch)) return s;
   return submit(S, nov, p, id);
 }
 
@@ -607,6 +663,30 @@ int perm_server_wait(perm_server* S, unsigned long long id, perm_result* out,
   Request* r = it->second;
   S->cv_done.wait(lk, [&] { return r->finished; });
   return collect(S, S->table.find(id), out, latency_ms);
+}
+
+int perm_server_wait_many(perm_server* S, const unsigned long long* ids, int count,
+                          perm_result* out, double* latency_ms) {
+  if (!S || (count > 0 && (!ids || !out))) return perm::fail(PERM_ERR_ARG, "null pointer");
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (int i = 0; i < count; ++i)
+    if (S->table.find(ids[i]) == S->table.end())
+      return perm::fail(PERM_ERR_ARG, "unknown request id");
+  int first_error = PERM_OK;
+  for (int i = 0; i < count; ++i) {
+    auto it = S->table.find(ids[i]);
+    if (it == S->table.end()) {  // listed twice: collected already
+      std::memset(out + i, 0, sizeof(perm_result));
+      out[i].status = PERM_ERR_ARG;
+      if (first_error == PERM_OK) first_error = perm::fail(PERM_ERR_ARG, "request id listed twice");
+      continue;
+    }
+    Request* r = it->second;
+    S->cv_done.wait(lk, [&] { return r->finished; });
+    const int st = collect(S, S->table.find(ids[i]), out + i, latency_ms ? latency_ms + i : nullptr);
+    if (st != PERM_OK && first_error == PERM_OK) first_error = st;
+  }
+  return first_error == PERM_OK ? perm::ok() : first_error;
 }
 
 int perm_server_poll(perm_server* S, unsigned long long id, perm_result* out,
@@ -624,6 +704,14 @@ int perm_server_stats_get(perm_server* S, perm_server_stats* out) {
   std::lock_guard<std::mutex> lk(S->mu);
   *out = S->stats;
   out->pending = (long long)S->pending.size();
+  // scheduler-owned counters, read without its lock (statistics only)
+  out->uploads = S->n_uploads.load(std::memory_order_relaxed);
+  out->admit_ms = S->t_admit.load(std::memory_order_relaxed);
+  out->issue_ms = S->t_issue.load(std::memory_order_relaxed);
+  out->retire_ms = S->t_retire.load(std::memory_order_relaxed);
+  out->gpu_wait_ms = S->t_wait.load(std::memory_order_relaxed);
+  out->gpu_ms = S->t_gpu.load(std::memory_order_relaxed);
+  out->dd_passes = S->n_dd.load(std::memory_order_relaxed);
   return perm::ok();
 }
 
@@ -635,7 +723,6 @@ void perm_server_destroy(perm_server* S) {
   }
   S->cv_submit.notify_all();
   if (S->worker.joinable()) S->worker.join();  // drains admitted and pending work
-  if (S->helper.joinable()) S->helper.join();  // then the double-double passes
   (void)hipSetDevice(S->device);
   if (S->s0) (void)hipStreamSynchronize(S->s0);
   release(S);

@@ -44,6 +44,7 @@ struct SkipPrepared {
   std::vector<double> x0, cs;
   std::vector<int> lev;
   int walk_bits = 0, high_bits = 0;
+  double product_bits = 1e9;  // integer_product_bits of the matrix
 };
 
 // Greedy column order and row levels; builds the permuted column store.
@@ -177,8 +178,10 @@ void fill(int n, const SkipPrepared& sp, const perm::Partial& r, int mode, perm_
   out->subsets = std::ldexp(1.0, n - 1);
   out->mode = mode;
   out->integral = 1;
-  out->nu = n;
-    out->status = PERM_OK;
+  out->nu = out->nu1 = out->nu2 = n;
+  out->status = PERM_OK;
+  if (perm::integer_sum_exact(true, sp.product_bits, out->abs_sum, scale, n))
+    out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(true, out->value, out->err_bound, out->err_est);
 }
 
@@ -204,6 +207,7 @@ int perm_skip(int dtype, const void* mat, const int* rptrs, const int* cols, con
   sp.walk_bits = split.walk_bits;
   sp.high_bits = split.high_bits;
   prepare_order(nov, x0, cc, sp.walk_bits, sp);
+  sp.product_bits = perm::integer_product_bits(x0.data(), cc.data(), nov);
   int check_bits = perm::env_int("PERMAN_SKIP_CHECK_BITS", 5);
   check_bits = std::max(R, std::min(check_bits, sp.walk_bits));
   perm::Partial r;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "ryser_device.hpp"
+#include "ryser_kernels.hpp"
 
 namespace perm {
 
@@ -20,7 +21,7 @@ struct BatchMatrix {
   uint64_t lanes;       // 2^high_bits
   int32_t walk_bits;    // subsets per lane = 2^walk_bits
   int32_t high_bits;
-  int32_t variant;      // index of (n, nu) in the kernel's variant table
+  int32_t variant;      // index of (n, walk shape) in the kernel's variant table
   uint32_t first_item;  // this matrix's items are [first_item, first_item + item_count)
   uint32_t item_count;
   uint32_t pad;
@@ -33,10 +34,11 @@ struct BatchItem {
   uint64_t lane_base;
 };
 
-// Variant id for (n, nu) (-1 if none) and the narrowest supported pair
-// width >= want (n if none).
-int batch_variant(int n, int nu);
-int batch_nu(int n, int want);
+// Variant id of walk shape s and kernel mode (0 FP64, 1 double-double
+// products; mode 1 only for the plain shape) for order n (-1 if none), and
+// the cheapest FP64 variant shape covering `want` ((n, n, n) if none).
+int batch_variant(int n, NestShape s, int mode = 0);
+NestShape batch_shape(int n, NestShape want);
 
 // Zero `count` work counters.
 hipError_t launch_batch_reset(uint32_t* counters, uint32_t count, hipStream_t stream);

@@ -190,6 +190,37 @@ inline HostDD host_dd_add(HostDD a, HostDD b) {
   return {h, s.lo - (h - s.hi)};
 }
 
+// Integer matrices: m_j = 2 x_j(S) = 2 x0_j + 2 sum_{k in S} c_kj is an
+// integer, and over all subsets S it stays within [2 x0_j + 2 sum_k
+// min(c_kj, 0), 2 x0_j + 2 sum_k max(c_kj, 0)]: |m_j| <= r_j, the larger end
+// in magnitude (the row's sum of |entries| for a dense row).  Every partial
+// product the walk forms is (a product of some m_j) / 2^k, its numerator at
+// most prod_j max(r_j, 1); below 2^53 every product is exact.  Returns log2
+// of that bound (cols[k*n + j], k < n-1).
+inline double integer_product_bits(const double* x0, const double* cols, int n) {
+  double bits = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double lo = 2.0 * x0[j], hi = 2.0 * x0[j];
+    for (int k = 0; k < n - 1; ++k) {
+      const double c = 2.0 * cols[(size_t)k * n + j];
+      (c < 0.0 ? lo : hi) += c;
+    }
+    const double r = std::max(std::fabs(lo), std::fabs(hi));
+    if (r > 1.0) bits += std::log2(r);
+  }
+  return bits;
+}
+
+// With exact products every term is a multiple of 2^-n; the lane sums
+// (magnitudes below A = sum |term|) are then exact while A 2^n < 2^53, and
+// the double-double folds and reductions of exact values are exact: the
+// computed sum is the permanent itself.  abs_sum_scaled = |scale| A.
+inline bool integer_sum_exact(bool integral, double product_bits, double abs_sum_scaled,
+                              double scale, int n) {
+  return integral && product_bits < 52.5 &&
+         abs_sum_scaled / std::fabs(scale) * std::ldexp(1.0, n) <= std::ldexp(1.0, 52);
+}
+
 // Integer matrices: every x_j is an exact half-integer and the permanent an
 // integer.  When the stated (worst-case) error is below 1/2 and the value is
 // below 2^53, rounding to the nearest integer gives the exact permanent;

@@ -84,7 +84,11 @@ class ServerConfigC(ctypes.Structure):
 
 class ServerStatsC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in
-                ("submitted", "admitted", "completed", "iterations", "items", "pending")]
+                ("submitted", "admitted", "completed", "iterations", "items", "pending",
+                 "uploads")] + \
+               [(k, ctypes.c_double) for k in
+                ("admit_ms", "issue_ms", "retire_ms", "gpu_wait_ms", "gpu_ms")] + \
+               [("dd_passes", ctypes.c_longlong)]
 
 
 PERM_PENDING = 5
@@ -145,6 +149,8 @@ def lib():
                                                 u64p]
         L.perm_server_wait.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
         L.perm_server_poll.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
+        L.perm_server_wait_many.argtypes = [vp, u64p, ctypes.c_int, ctypes.POINTER(PermResultC),
+                                            dp]
         L.perm_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
         L.perm_server_destroy.argtypes = [vp]
         L.perm_server_destroy.restype = None
@@ -439,6 +445,32 @@ def last_error() -> str:
     return lib().perm_last_error().decode()
 
 
+class Request:
+    """A matrix ready to hand to the library: the contiguous array, its dtype
+    code and, for the sparse path, its CCS (built once, outside any timing)."""
+    __slots__ = ("a", "dt", "sparse", "cptrs", "rows", "cvals")
+
+    def __init__(self, mat, sparse_path: bool = False):
+        self.a, self.dt = _mat(mat)
+        self.sparse = bool(sparse_path)
+        if self.sparse:
+            self.cptrs, self.rows, self.cvals = compress(self.a)
+        else:
+            self.cptrs = self.rows = self.cvals = None
+
+
+def solve(req: Request) -> PermResult:
+    """One plain call for a prepared request (perm_dense / perm_sparse, ID 4)."""
+    r = PermResultC()
+    if req.sparse:
+        st = lib().perm_sparse(req.dt, req.a.ctypes.data, _ip(req.cptrs), _ip(req.rows),
+                               req.cvals.ctypes.data, req.a.shape[0], 4, ctypes.byref(r))
+    else:
+        st = lib().perm_dense(req.dt, req.a.ctypes.data, req.a.shape[0], 4, ctypes.byref(r))
+    _check(st)
+    return PermResult.from_c(r)
+
+
 class Server:
     """Continuous-batching permanent server on one GPU (perm_server_*).
     submit() returns a ticket at once; wait() blocks for its result and
@@ -467,10 +499,32 @@ class Server:
         _check(st)  # the matrix is copied before submit returns
         return tid.value
 
+    def submit_request(self, req: Request) -> int:
+        tid = ctypes.c_ulonglong()
+        if req.sparse:
+            st = lib().perm_server_submit_sparse(self._h, req.dt, req.a.ctypes.data,
+                                                 _ip(req.cptrs), _ip(req.rows),
+                                                 req.cvals.ctypes.data, req.a.shape[0],
+                                                 ctypes.byref(tid))
+        else:
+            st = lib().perm_server_submit_dense(self._h, req.dt, req.a.ctypes.data,
+                                                req.a.shape[0], ctypes.byref(tid))
+        _check(st)
+        return tid.value
+
     def wait(self, ticket: int):
         r, lat = PermResultC(), ctypes.c_double()
         _check(lib().perm_server_wait(self._h, ticket, ctypes.byref(r), ctypes.byref(lat)))
         return PermResult.from_c(r), lat.value
+
+    def wait_many(self, tickets):
+        """Results of several tickets at once: [(PermResult, latency_ms)]."""
+        k = len(tickets)
+        ids = (ctypes.c_ulonglong * k)(*tickets)
+        res = (PermResultC * k)()
+        lat = (ctypes.c_double * k)()
+        _check(lib().perm_server_wait_many(self._h, ids, k, res, lat))
+        return [(PermResult.from_c(res[i]), lat[i]) for i in range(k)]
 
     def poll(self, ticket: int):
         r, lat = PermResultC(), ctypes.c_double()
@@ -503,14 +557,15 @@ class Server:
             pass
 
 
-def served_workload(count: int, seed: int = 6, n_lo: int = 16, n_hi: int = 36):
+def served_workload(count: int, seed: int = 6, n_lo: int = 16, n_hi: int = 36,
+                    decay: float = 4.0):
     """Request mix of the served configuration: n in [n_lo, n_hi] with
-    P(n) proportional to 2^(-(n - n_lo)/4), density U[0.1, 0.9], half of the
-    requests 0/1 (int32) and half FP64 U(0,1], every third one through the
-    sparse path.  Request i draws from stream seed*2^20 + i (sizes and kinds
-    from stream seed*2^20 + 2^19)."""
+    P(n) proportional to 2^(-(n - n_lo)/decay), density U[0.1, 0.9], half of
+    the requests 0/1 (int32) and half FP64 U(0,1], every third one through
+    the sparse path.  Request i draws from stream seed*2^20 + i (sizes and
+    kinds from stream seed*2^20 + 2^19)."""
     ns = np.arange(n_lo, n_hi + 1)
-    w = np.power(2.0, -(ns - n_lo) / 4.0)
+    w = np.power(2.0, -(ns - n_lo) / decay)
     cdf = np.cumsum(w / w.sum())
     base = np.uint64(seed) << np.uint64(20)
     u = uniforms(int(base) + (1 << 19), 3 * count)

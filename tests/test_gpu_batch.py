@@ -109,3 +109,26 @@ def test_autotune_picks_a_split_and_keeps_results():
     b.close()
     for x, y in zip(before, after):
         assert abs(x.value - y.value) <= x.err_bound + y.err_bound
+
+
+def test_batch_nested_shapes_vs_exact_and_single():
+    """Sparse-ish matrices of n >= 24 run on the three-level batch kernels:
+    the results match the exact oracle (n <= 26) and the single-matrix path."""
+    mats = [P.synthetic_matrix(n, d, seed=500 + 7 * n + int(10 * d), full_support=True)
+            for n in (24, 26, 30, 34) for d in (0.2, 0.35)]
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, _ = b.run()
+    shapes = set()
+    for m, r in zip(mats, res):
+        n = m.shape[0]
+        shapes.add((r.nu, r.nu1, r.nu2))
+        one = P.dense(m)
+        assert abs(r.value - one.value) <= r.err_bound + one.err_bound, (n, r, one)
+        if n <= 26:
+            ex = float(O.exact_permanent(m))
+            assert abs(r.value - ex) <= r.err_bound
+            assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_est)
+    print("batch shapes:", sorted(shapes))
+    assert any(r.nu2 < r.n for r in res)  # three-level shapes ran

@@ -70,3 +70,12 @@ def test_grid_graph_tilings():
     for m, k in [(4, 4), (4, 6), (6, 6)]:
         G = O.grid_graph_matrix(m, k)
         assert round(P.dense(G.astype(np.int32)).value) == O.domino_tilings(m, k)
+
+
+def test_integer_exact_in_fp64_without_second_pass():
+    """Small-entry integer matrices whose products and sums stay below 2^53
+    (the bound of integer_sum_exact) are exact in FP64: err_bound 0, mode 0."""
+    for n, d, seed in ((12, 0.4, 1), (14, 0.3, 2), (16, 0.25, 3)):
+        a = P.synthetic_matrix(n, d, seed=seed, binary=True)
+        r = P.dense(a)
+        assert r.value == float(O.exact_permanent(a)) and r.err_bound == 0.0 and r.mode == 0

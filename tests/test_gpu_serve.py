@@ -118,3 +118,37 @@ def test_server_integer_requests_exact():
         ex = O.exact_permanent(m)
         if ex < 2 ** 53:
             assert r.value == float(ex), (m.shape, r, ex)
+
+
+def test_server_wait_many_and_prepared_requests():
+    mats = [(P.synthetic_matrix(16 + i % 12, 0.4, seed=900 + i), i % 3 == 0) for i in range(24)]
+    reqs = [P.Request(m, sp) for m, sp in mats]
+    with P.Server() as srv:
+        one = [srv.wait(srv.submit_request(q))[0] for q in reqs]
+        tickets = [srv.submit_request(q) for q in reqs]
+        many = srv.wait_many(tickets)
+        for a, (b, lat) in zip(one, many):
+            assert (a.sum_hi, a.sum_lo) == (b.sum_hi, b.sum_lo) and lat > 0
+        for q, r in zip(reqs, one):  # the plain call on the same prepared request
+            v = P.solve(q)
+            assert abs(v.value - r.value) <= v.err_bound + r.err_bound
+        t = srv.submit_request(reqs[0])
+        with pytest.raises(P.PermError):
+            srv.wait_many([t, t])  # listed twice: the second is an error
+
+
+def test_server_exact_pass_runs_as_batch_items():
+    """Integer requests whose FP64 error could reach 1/2 get their
+    double-double pass as batch items (counted in dd_passes) and come out as
+    the exact integer."""
+    mats = [P.synthetic_matrix(n, 0.5, seed=1200 + n, binary=True).astype(np.int32)
+            for n in (20, 21, 22, 23, 24)]
+    exact = [O.exact_permanent(m) for m in mats]
+    with P.Server() as srv:
+        res = srv.wait_many([srv.submit(m) for m in mats])
+        st = srv.stats()
+    for m, ex, (r, _) in zip(mats, exact, res):
+        if ex < 2 ** 53:
+            assert r.value == float(ex) and r.err_bound == 0.0, (m.shape, r, ex)
+    print("dd passes:", st["dd_passes"], [r.mode for r, _ in res])
+    assert st["dd_passes"] == sum(r.mode == 1 for r, _ in res)
