@@ -23,8 +23,8 @@ sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
 
 # HBM bytes per launch of the n=40 kernel on this workload: rocprofv3 FETCH_SIZE
 # + WRITE_SIZE, each scaled by its reading for a copy kernel of known size
-# (profiles/ryser_dense_n40.md): 734.6 KiB read + 16098.7 KiB written.
-TRAFFIC_BYTES_N40 = (734.6 + 16098.7) * 1024
+# (profiles/ryser_dense_n40.md): 691.5 KiB read + 16097.3 KiB written.
+TRAFFIC_BYTES_N40 = (691.5 + 16097.3) * 1024
 FP64_PEAK_TFLOPS = 78.6
 N = 40
 DENSITY = 0.5
@@ -296,7 +296,7 @@ def main():
             "bound": "mfma",
             "bound_detail": "FP64 compute: the kernel issues FP64 VALU instructions, whose MI355X "
                             "peak (78.6 TFLOP/s) equals the FP64 matrix-core peak; HBM does not "
-                            "bound it (traffic below, ~21 MB/s)",
+                            "bound it (traffic below, ~26 MB/s)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
