@@ -159,8 +159,10 @@ void perm_plan_destroy(perm_plan* plan);
 
 /* ---- batches of mixed n (extension) --------------------------------------
  * Many matrices (n <= 40) resident on the device, computed by persistent
- * kernels (one per (n, nu) variant, concurrent on several streams) captured
- * in one hipGraph; perm_batch_run replays it.  A matrix is cut into 64-lane
+ * kernels (one per variant: n and walk shape, concurrent on several streams)
+ * captured in one hipGraph; perm_batch_run replays it.  Integer matrices
+ * whose FP64 error could reach 1/2 then get a double-double pass over the
+ * same items (the batch kernels' double-double variant).  A matrix is cut into 64-lane
  * items of about 2^item_log2 subsets that depend only on its own n, so its
  * result is identical alone and in any mix.  item_log2 (default 18),
  * streams (default 4) and max_grid (workgroups per variant kernel, default
@@ -172,7 +174,8 @@ int perm_batch_add_dense(perm_batch* batch, int dtype, const void* mat, int nov,
 int perm_batch_add_sparse(perm_batch* batch, int dtype, const void* mat, const int* cptrs,
                           const int* rows, const void* cvals, int nov, int* index);
 int perm_batch_size(const perm_batch* batch);
-/* results: one per matrix, in insertion order; device_ms: graph time. */
+/* results: one per matrix, in insertion order; device_ms: graph time plus
+ * the double-double passes, if any. */
 int perm_batch_run(perm_batch* batch, perm_result* results, double* device_ms);
 void perm_batch_destroy(perm_batch* batch);
 /* Work split of a batch and the autotuner that picks it: subsets per item

@@ -174,7 +174,8 @@ int build(perm_batch* B) {
   const size_t items_b = align256(B->items.size() * sizeof(BatchItem));
   const size_t part_b = align256(B->items.size() * sizeof(Partial));
   const size_t res_b = align256(nm * sizeof(Partial));
-  const size_t ctr_b = align256(B->groups.size() * sizeof(uint32_t));
+  // one counter per variant group, then one per matrix for double-double passes
+  const size_t ctr_b = align256((B->groups.size() + nm) * sizeof(uint32_t));
   B->dbytes = data_b + mats_b + items_b + part_b + res_b + ctr_b;
   HIP_TRYB(hipMalloc(&B->dmem, B->dbytes));
   char* p = (char*)B->dmem;
@@ -341,23 +342,53 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   float ms = 0.f;
   HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
   if (device_ms) *device_ms = ms;
+  std::vector<size_t> dd;
   for (size_t m = 0; m < B->ents.size(); ++m) {
     const Entry& e = B->ents[m];
     perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 0,
                            B->h_results[m], results[m]);
     results[m].kernel_ms = ms;
-    if (perm::needs_dd_pass(results[m])) {
-      // integer matrix whose FP64 error could reach 1/2: double-double
-      // products through the single-matrix path (same column store)
-      perm::Prepared p;
-      const size_t cs_len = (size_t)(2 * (e.n - 1) + 1) * e.n;
-      p.cs.assign(B->data.begin() + e.col_off, B->data.begin() + e.col_off + cs_len);
-      p.x0.assign(B->data.begin() + e.x0_off, B->data.begin() + e.x0_off + e.n);
-      p.nu = p.nu1 = p.nu2 = e.n;
-      p.integral = true;
-      p.product_bits = e.product_bits;
-      if (int s = perm::run_prepared(e.n, p, 1, &results[m])) return s;
-    }
+    if (perm::needs_dd_pass(results[m])) dd.push_back(m);
+  }
+  if (dd.empty()) return perm::ok();
+  // Integer matrices whose FP64 error could reach 1/2: a double-double
+  // product pass over the same items with the double-double variant (one
+  // persistent kernel per matrix, on the batch's streams), then the
+  // per-matrix reduction again.
+  hipStream_t s0 = B->s[0];
+  uint32_t* ctr = B->d_counters + B->groups.size();
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  HIP_TRYB(hipEventRecord(B->ev0, s0));
+  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)dd.size(), s0));
+  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+  for (size_t i = 0; i < dd.size(); ++i) {
+    const Entry& e = B->ents[dd[i]];
+    const int vid = perm::batch_variant(e.n, {e.n, e.n, e.n}, 1);
+    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
+    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
+                                        e.first_item + e.item_count, ctr + i, B->d_partials, grid,
+                                        B->s[i % ns]));
+  }
+  for (int i = 1; i < ns; ++i) {
+    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+  }
+  const uint32_t nm = (uint32_t)B->ents.size();
+  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
+  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
+                          s0));
+  HIP_TRYB(hipEventRecord(B->ev1, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  float dd_ms = 0.f;
+  HIP_TRYB(hipEventElapsedTime(&dd_ms, B->ev0, B->ev1));
+  if (device_ms) *device_ms = ms + dd_ms;
+  for (size_t m : dd) {
+    const Entry& e = B->ents[m];
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, {e.n, e.n, e.n}, 1,
+                           B->h_results[m], results[m]);
+    results[m].kernel_ms = ms + dd_ms;
   }
   return perm::ok();
 }

@@ -132,3 +132,28 @@ def test_batch_nested_shapes_vs_exact_and_single():
             assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_est)
     print("batch shapes:", sorted(shapes))
     assert any(r.nu2 < r.n for r in res)  # three-level shapes ran
+
+
+def test_batch_integer_exact_pass_on_batch_kernels():
+    """0/1 matrices whose FP64 error could reach 1/2 get the double-double
+    pass inside the batch (mode 1) and come out as the exact integer, the
+    same value as the single-matrix path."""
+    mats = [P.synthetic_matrix(n, 0.5, seed=1300 + n, binary=True).astype(np.int32)
+            for n in (20, 21, 22, 23, 24, 16)]
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, _ = b.run()
+    modes = [r.mode for r in res]
+    print("modes:", modes)
+    assert 1 in modes
+    for m, r in zip(mats, res):
+        ex = O.exact_permanent(m)
+        one = P.dense(m)
+        if ex < 2 ** 53:
+            assert r.value == float(ex) == one.value and r.err_bound == 0.0, (m.shape, r, ex)
+        else:  # above 2^53: the same within the stated errors
+            assert abs(r.value - one.value) <= r.err_bound + one.err_bound
+            assert abs(r.value - float(ex)) <= r.err_bound
+    res2, _ = b.run()  # a replay recomputes the same values
+    assert [r.value for r in res2] == [r.value for r in res]
