@@ -81,6 +81,29 @@ def sparse_leg(P, reps: int):
     }
 
 
+def binary_leg(P, reps: int):
+    """The 0/1 variant of the n=40 configuration (density 0.5, seed 3),
+    resident plan: subsets/s and % of FP64 peak (2n flops per subset)."""
+    m = P.synthetic_matrix(N, DENSITY, seed=3, binary=True).astype("int32")
+    plan = P.Plan(m)
+    plan.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = plan.run()
+    wall = (time.perf_counter() - t0) / reps
+    plan.close()
+    subsets = 2.0 ** (N - 1)
+    return {
+        "workload": "dense 0/1, n=40, density 0.5, splitmix64 seed 3, 2^39 subsets, matrix resident",
+        "subsets_per_s": subsets / wall,
+        "ms_per_run": wall * 1e3,
+        "pct_fp64_peak_algorithmic": 100.0 * 2.0 * N * subsets / wall / 1e12 / FP64_PEAK_TFLOPS,
+        "shape": [r.nu, r.nu1, r.nu2],
+        "value": r.value,
+        "err_bound": r.err_bound,
+    }
+
+
 def cpu_leg(P, threads: int, reps: int = 11):
     """`perman -c` (the library's OpenMP FP64 Ryser) at n=20, density 0.5,
     U(0,1], seed 1: median of `reps` runs."""
@@ -233,7 +256,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=32)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-mixed", action="store_true", help="skip the batched/served legs")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the 0/1, batched and served legs")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-count", type=int, default=20000)
     ap.add_argument("--serve-load", type=float, default=0.7)
@@ -315,6 +339,8 @@ def main():
                     "2n non-fusable instructions would allow",
         },
     }
+    if not args.no_mixed:
+        out["binary_n40"] = binary_leg(P, 2)
     out["sparse"] = sparse_leg(P, 3)
     out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
     if not args.no_mixed:

@@ -46,6 +46,22 @@ def test_c2_fp64_vs_double_double(c2):
     assert abs(fast.value - ref.value) <= fast.err_bound
 
 
+def test_c2_binary_variant_vs_double_double_products():
+    """The 0/1 variant of the benchmark configuration (n=40, density 0.5,
+    seed 3): X is exact, so double-double products (mode 1) leave only
+    ~1e-30 relative rounding per operation."""
+    b = P.synthetic_matrix(40, 0.5, seed=3, binary=True).astype(np.int32)
+    plan = P.Plan(b)
+    fast = plan.run(0)
+    ref = plan.run(1)
+    plan.close()
+    print(f"n=40 0/1 fast={fast.value!r} dd={ref.value!r} rel={rel(fast.value, ref.value):.3e} "
+          f"shape=({fast.nu},{fast.nu1},{fast.nu2})")
+    assert fast.integral == 1 and ref.value == round(ref.value)
+    assert rel(fast.value, ref.value) < 1e-10
+    assert abs(fast.value - ref.value) <= fast.err_bound
+
+
 def test_grid_8x10_n40_exact_tilings():
     G = O.grid_graph_matrix(8, 10).astype(np.int32)
     assert G.shape == (40, 40)
