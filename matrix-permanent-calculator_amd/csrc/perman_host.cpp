@@ -95,7 +95,7 @@ int select_device() {
 
 // Host preparation.  Produces the start vector x0 and the Gray columns in
 // column-major order (cols[k*n + j] = a[j][k] for the walked columns
-// k < n-1), then chooses the walk order (pair_order) and builds the signed
+// k < n-1), then chooses the walk order (nest_order) and builds the signed
 // column store.
 
 // Walk order for the pair and nested kernels (ryser_dense_impl.hpp).  Gray
