@@ -142,9 +142,11 @@ void set_shape(perm_plan* P, int n, const Prepared& prep) {
 void configure_passes(perm_plan* P, int mode) {
   perm::DenseLaunch& L = P->L;
   L.mode = mode;
-  L.nu = mode == 0 ? P->nu : P->n;
-  L.nu1 = mode == 0 ? P->nu1 : P->n;
-  L.nu2 = mode == 0 ? P->nu2 : P->n;
+  // FP64 and double-double products run the prepared walk shape; the
+  // double-double X mode the plain walk
+  L.nu = mode <= 1 ? P->nu : P->n;
+  L.nu1 = mode <= 1 ? P->nu1 : P->n;
+  L.nu2 = mode <= 1 ? P->nu2 : P->n;
   L.main.lanes = 1ull << P->high_bits;
   L.main.blocks = blocks_of(L.main.lanes);
   L.main.walk_bits = P->walk_bits;
@@ -231,9 +233,9 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   perm::round_integral(P->integral, out->value, out->err_bound, out->err_est);
   out->status = PERM_OK;
   out->mode = mode;
-  out->nu = mode == 0 ? P->nu : P->n;
-  out->nu1 = mode == 0 ? P->nu1 : P->n;
-  out->nu2 = mode == 0 ? P->nu2 : P->n;
+  out->nu = mode <= 1 ? P->nu : P->n;
+  out->nu1 = mode <= 1 ? P->nu1 : P->n;
+  out->nu2 = mode <= 1 ? P->nu2 : P->n;
   out->integral = P->integral ? 1 : 0;
 }
 

@@ -10,6 +10,12 @@ hipError_t launch_dense_range_b(const DenseLaunch& L, hipStream_t s, hipEvent_t 
 hipError_t launch_dense_range_c(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
 hipError_t launch_dense_range_d(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
 hipError_t launch_dense_range_e(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_f(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_g(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_h(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_i(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_j(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
+hipError_t launch_dense_range_k(const DenseLaunch& L, hipStream_t s, hipEvent_t ev);
 
 // Grid-level reduction of `count` partials into out[0] (one workgroup of
 // 256 threads, fixed order).
@@ -42,11 +48,19 @@ hipError_t launch_skip(const SkipLaunch& L, hipStream_t stream) {
 
 static hipError_t launch_passes(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   if (L.n < 1 || L.n > kMaxDenseN) return hipErrorInvalidValue;
-  if (L.n <= 20) return launch_dense_range_a(L, stream, main_done);
-  if (L.n <= 30) return launch_dense_range_b(L, stream, main_done);
-  if (L.n <= 36) return launch_dense_range_c(L, stream, main_done);
-  if (L.n <= 42) return launch_dense_range_d(L, stream, main_done);
-  return launch_dense_range_e(L, stream, main_done);
+  using RangeFn = hipError_t (*)(const DenseLaunch&, hipStream_t, hipEvent_t);
+  static const struct {
+    int hi;
+    RangeFn fn;
+  } kRanges[] = {{20, launch_dense_range_a}, {26, launch_dense_range_b},
+                 {30, launch_dense_range_c}, {33, launch_dense_range_d},
+                 {36, launch_dense_range_e}, {38, launch_dense_range_f},
+                 {40, launch_dense_range_g}, {42, launch_dense_range_h},
+                 {44, launch_dense_range_i}, {46, launch_dense_range_j},
+                 {48, launch_dense_range_k}};
+  for (const auto& r : kRanges)
+    if (L.n <= r.hi) return r.fn(L, stream, main_done);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_dense_pass_only(const DenseLaunch& L, hipStream_t stream) {

@@ -188,8 +188,10 @@ struct Nest {
 };
 
 // Cached tail products of the nested variants (see the file comment).
+// (kDDProd: the tails are double-double pairs, low parts in l1..l3.)
 struct Tails {
   double t1 = 1.0, t2 = 1.0, t3 = 1.0;
+  double l1 = 0.0, l2 = 0.0, l3 = 0.0;
 };
 
 // Rows touched by a column at Gray level `lv` (3: any column >= 3).
@@ -207,6 +209,45 @@ __device__ __forceinline__ double tail_prod(const double (&x)[N], double seed) {
   } else {
     return chain_prod<N, LO, HI, SEED>(x, seed);
   }
+}
+
+// Double-double product (oh, ol) of x[LO..HI) times the seed (sh, sl) when
+// SEED -- the seed itself for an empty range.  One TwoProd chain: (h, l) * x
+// = h x (rounded) + (the FMA-exact error of h x + l x).
+template <int N, int LO, int HI, bool SEED>
+__device__ __forceinline__ void dd_tail_prod(const double (&x)[N], double sh, double sl,
+                                             double& oh, double& ol) {
+  if constexpr (LO >= HI) {
+    static_assert(SEED, "empty tail without a seed");
+    oh = sh;
+    ol = sl;
+  } else {
+    double h = SEED ? sh : x[LO], l = SEED ? sl : 0.0;
+#pragma unroll
+    for (int j = SEED ? LO : LO + 1; j < HI; ++j) {
+      const double ph = h * x[j];
+      const double e = __builtin_fma(h, x[j], -ph);
+      l = __builtin_fma(l, x[j], e);
+      h = ph;
+    }
+    oh = h;
+    ol = l;
+  }
+}
+
+// (p, plo) += sign * (th, tl) through TwoSum; a += |th|.
+template <bool NEG>
+__device__ __forceinline__ void dd_add_term(double th, double tl, LaneAcc& acc) {
+  if (NEG) {
+    th = -th;
+    tl = -tl;
+  }
+  const double s = acc.p + th;
+  const double bb = s - acc.p;
+  const double err = (acc.p - (s - bb)) + (th - bb);
+  acc.p = s;
+  acc.plo += err + tl;
+  acc.a += __builtin_fabs(th);
 }
 
 // Step v of group g (step index t = g*2^R + v).  Phase A adds the column
@@ -249,7 +290,15 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
       blk = gn < groups ? 2 * (R + c) + neg : zero_blk;
       load_col<N, N>(cv, cols, blk);
     }
-    if constexpr (kNested) {
+    if constexpr (kNested && MODE == kDDProd) {
+      if constexpr (lv >= 3 && K2 < N) dd_tail_prod<N, K2, N, false>(x, 0.0, 0.0, tl.t3, tl.l3);
+      if constexpr (lv >= 2 && K1 < N)
+        dd_tail_prod<N, K1, K2, (K2 < N)>(x, tl.t3, tl.l3, tl.t2, tl.l2);
+      if constexpr (lv >= 1) dd_tail_prod<N, K0, K1, (K1 < N)>(x, tl.t2, tl.l2, tl.t1, tl.l1);
+      double th, tlo;
+      dd_tail_prod<N, 0, K0, true>(x, tl.t1, tl.l1, th, tlo);
+      dd_add_term<(V & 1) != 0>(th, tlo, acc);
+    } else if constexpr (kNested) {
       if constexpr (lv >= 3 && K2 < N) tl.t3 = tail_prod<N, K2, N, false>(x, 0.0);
       if constexpr (lv >= 2 && K1 < N) tl.t2 = tail_prod<N, K1, K2, (K2 < N)>(x, tl.t3);
       if constexpr (lv >= 1) tl.t1 = tail_prod<N, K0, K1, (K1 < N)>(x, tl.t2);
@@ -321,7 +370,8 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
                                           int n_walk_bits, int n_high_bits, DD& s, double& a,
                                           double& q) {
   static_assert(S::K0 <= S::K1 && S::K1 <= S::K2 && S::K2 <= N, "nested shape");
-  static_assert(S::K0 == N || (MODE == kFast && R >= 1 && S::K0 >= 1), "nested: FP64 mode only");
+  static_assert(S::K0 == N || (MODE != kDDAll && R >= 1 && S::K0 >= 1),
+                "nested: FP64 or double-double products");
   static_assert(S::K1 == N || R >= 2, "level-1 tail needs a 4-step group");
   static_assert(S::K2 == N || R >= 3, "level-2 tail needs an 8-step group");
   const ColPtr cols = (ColPtr)colpm;
@@ -402,16 +452,16 @@ hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stre
   return hipGetLastError();
 }
 
-// FP64 run: the FP64 kernel of shape S.
-template <int N, class S>
-hipError_t launch_fast(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
-  hipError_t e = launch_pass<N, kFast, S>(L, L.main, stream);
+// One run: the kernel of mode MODE and shape S.
+template <int N, int MODE, class S>
+hipError_t launch_mode(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  hipError_t e = launch_pass<N, MODE, S>(L, L.main, stream);
   if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
   return e;
 }
 
 // Shape j of K0 = 4(q+1) in the compiled menu (nest_menu_shape).
-template <int N, int Q, int J>
+template <int N, int MODE, int Q, int J>
 hipError_t launch_shape(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
                         bool& hit) {
   constexpr int K0 = 4 * (Q + 1);
@@ -421,46 +471,50 @@ hipError_t launch_shape(const DenseLaunch& L, hipStream_t stream, hipEvent_t mai
   } else {
     if (L.nu != sh.k0 || L.nu1 != sh.k1 || L.nu2 != sh.k2) return hipErrorInvalidValue;
     hit = true;
-    return launch_fast<N, Nest<sh.k0, sh.k1, sh.k2>>(L, stream, main_done);
+    return launch_mode<N, MODE, Nest<sh.k0, sh.k1, sh.k2>>(L, stream, main_done);
   }
 }
 
-template <int N, int Q, int... J>
+template <int N, int MODE, int Q, int... J>
 hipError_t launch_k0(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done, bool& hit,
                      std::integer_sequence<int, J...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((hit ? 0 : (e = launch_shape<N, Q, J>(L, stream, main_done, hit), 0)), ...);
+  ((hit ? 0 : (e = launch_shape<N, MODE, Q, J>(L, stream, main_done, hit), 0)), ...);
   return e;
 }
 
-template <int N, int... Q>
+template <int N, int MODE, int... Q>
 hipError_t launch_nested(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done,
                          std::integer_sequence<int, Q...>) {
   hipError_t e = hipErrorInvalidValue;
   bool hit = false;
   ((L.nu == 4 * (Q + 1) && !hit
-        ? (e = launch_k0<N, Q>(L, stream, main_done, hit,
-                               std::make_integer_sequence<int, kNestShapesPerK0>{}),
+        ? (e = launch_k0<N, MODE, Q>(L, stream, main_done, hit,
+                                     std::make_integer_sequence<int, kNestShapesPerK0>{}),
            0)
         : 0),
    ...);
   return hit ? e : hipErrorInvalidValue;
 }
 
-template <int N>
-hipError_t launch_n(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
-  if (L.mode == kDDProd || L.mode == kDDAll) {
-    hipError_t e = L.mode == kDDProd ? launch_pass<N, kDDProd, Nest<N, N, N>>(L, L.main, stream)
-                                     : launch_pass<N, kDDAll, Nest<N, N, N>>(L, L.main, stream);
-    if (e == hipSuccess && main_done) e = hipEventRecord(main_done, stream);
-    return e;
-  }
-  if (L.nu == N || L.nu <= 0) return launch_fast<N, Nest<N, N, N>>(L, stream, main_done);
+template <int N, int MODE>
+hipError_t launch_shaped(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  if (L.nu == N || L.nu <= 0) return launch_mode<N, MODE, Nest<N, N, N>>(L, stream, main_done);
   if constexpr (N >= kMinPairN) {
     if (L.nu % 4 == 0 && L.nu < N)
-      return launch_nested<N>(L, stream, main_done,
-                              std::make_integer_sequence<int, (N - 1) / 4>{});
+      return launch_nested<N, MODE>(L, stream, main_done,
+                                    std::make_integer_sequence<int, (N - 1) / 4>{});
   }
+  return hipErrorInvalidValue;
+}
+
+// kFast and kDDProd run the walk shape in L (nu, nu1, nu2); kDDAll the plain
+// walk.
+template <int N>
+hipError_t launch_n(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
+  if (L.mode == kDDAll) return launch_mode<N, kDDAll, Nest<N, N, N>>(L, stream, main_done);
+  if (L.mode == kDDProd) return launch_shaped<N, kDDProd>(L, stream, main_done);
+  if (L.mode == kFast) return launch_shaped<N, kFast>(L, stream, main_done);
   return hipErrorInvalidValue;
 }
 

@@ -74,3 +74,21 @@ def test_plan_and_chunked_follow_the_shape():
     assert (r.value, r.nu, r.nu1, r.nu2) == (one.value, one.nu, one.nu1, one.nu2)
     c, _ = P.chunked(a, use_cpu=False)
     assert (c.sum_hi, c.sum_lo) == (one.sum_hi, one.sum_lo)
+
+
+@pytest.mark.parametrize("n,density,seed", [(24, 0.3, 61), (26, 0.25, 62), (25, 0.5, 63)])
+def test_double_double_products_on_nested_shapes(n, density, seed):
+    """Mode 1 (double-double products) runs the nested shape too; for an
+    integer matrix it is exact, and for a real one it agrees with FP64."""
+    b = P.synthetic_matrix(n, density, seed, full_support=True, binary=True).astype(np.int32)
+    plan = P.Plan(b)
+    r = plan.run(1)
+    plan.close()
+    assert r.mode == 1 and r.nu < n, r
+    assert r.value == float(O.exact_permanent(b))
+    a = P.synthetic_matrix(n, density, seed, full_support=True)
+    plan = P.Plan(a)
+    f, d = plan.run(0), plan.run(1)
+    plan.close()
+    assert (d.nu, d.nu1, d.nu2) == (f.nu, f.nu1, f.nu2)
+    assert abs(d.value - f.value) <= f.err_bound + d.err_bound
