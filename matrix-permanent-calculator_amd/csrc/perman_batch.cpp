@@ -364,7 +364,7 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
   for (size_t i = 0; i < dd.size(); ++i) {
     const Entry& e = B->ents[dd[i]];
-    const int vid = perm::batch_variant(e.n, {e.n, e.n, e.n}, 1);
+    const int vid = perm::batch_variant(e.n, e.shape, 1);
     if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
     const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
     HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
@@ -386,7 +386,7 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   if (device_ms) *device_ms = ms + dd_ms;
   for (size_t m : dd) {
     const Entry& e = B->ents[m];
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, {e.n, e.n, e.n}, 1,
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 1,
                            B->h_results[m], results[m]);
     results[m].kernel_ms = ms + dd_ms;
   }

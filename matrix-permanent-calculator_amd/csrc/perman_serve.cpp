@@ -168,8 +168,12 @@ void publish_all(perm_server* S, const std::vector<Request*>& rs) {
 }
 
 // FP64 VALU lane-instructions per subset of a double-double product pass
-// (TwoProd chains and TwoSum accumulation; scheduling estimate only).
-double dd_subset_cost(int n) { return 4.0 * n + 12.0; }
+// over walk shape s: the FP64 adds, three instructions per TwoProd multiply,
+// about 8 for the TwoSum accumulation (scheduling estimate only).
+double dd_subset_cost(int n, perm::NestShape s) {
+  const double half = (perm::nest_cost(n, s) / 8.0 - 2.0) / 2.0;  // adds = multiplies
+  return half + 3.0 * half + 8.0;
+}
 
 // All items of r are done: sum its partials in item order and state the
 // error.  An integer matrix whose FP64 error could reach 1/2 goes back to the
@@ -188,12 +192,11 @@ bool finalize(perm_server* S, Request* r) {
   perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
                          r->mode, tot, r->result);
   if (r->mode == 0 && perm::needs_dd_pass(r->result)) {
-    r->mode = 1;
-    r->shape = {r->n, r->n, r->n};
+    r->mode = 1;  // same walk shape, double-double products
     r->variant = perm::batch_variant(r->n, r->shape, 1);
     r->issued = r->done = 0;
     std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0});
-    r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n);
+    r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape);
     S->queued_cost += (double)r->items * r->item_cost;
     S->active.push_back(r);
     S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
@@ -488,7 +491,7 @@ int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
   }
   r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
                  (double)perm::nest_cost(n, r->shape) / 8.0;
-  if (perm::batch_variant(n, {n, n, n}, 1) < 0) {
+  if (perm::batch_variant(n, r->shape, 1) < 0) {
     delete r;
     return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
   }

@@ -55,10 +55,14 @@ hipError_t launch_batch_variant(int vid, const BatchMatrix* mats, const BatchIte
                                 uint32_t begin, uint32_t end, uint32_t* counter,
                                 Partial* partials, uint32_t grid, hipStream_t stream) {
   if (vid < 0 || vid >= kNumVariants || end <= begin) return hipErrorInvalidValue;
-  const void* fn = vid < kBatchRangeEnds[0]   ? batch_kernel_range_a(vid)
-                   : vid < kBatchRangeEnds[1] ? batch_kernel_range_b(vid)
-                   : vid < kBatchRangeEnds[2] ? batch_kernel_range_c(vid)
-                                              : batch_kernel_range_d(vid);
+  using RangeFn = const void* (*)(int);
+  static const RangeFn kRangeFns[kBatchRanges] = {
+      batch_kernel_range<0>, batch_kernel_range<1>, batch_kernel_range<2>,
+      batch_kernel_range<3>, batch_kernel_range<4>, batch_kernel_range<5>,
+      batch_kernel_range<6>, batch_kernel_range<7>};
+  int r = 0;
+  while (r + 1 < kBatchRanges && vid >= batch_range_lo(r + 1)) ++r;
+  const void* fn = kRangeFns[r](vid);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {(void*)&mats, (void*)&items, (void*)&begin, (void*)&end, (void*)&counter,
                   (void*)&partials};

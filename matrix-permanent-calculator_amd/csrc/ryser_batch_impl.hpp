@@ -15,15 +15,16 @@ struct Variant {
   int n, k0, k1, k2, mode;
 };
 
-// Variants: for every n <= kMaxBatchN the plain walk in both modes, then
-// the kMenuBatch shapes (every pair width; for n >= kMinNestN also the two
-// three-level shapes per width) in FP64.
+// Variants: for every n <= kMaxBatchN the plain walk, then the kMenuBatch
+// shapes (every pair width; for n >= kMinNestN also the two three-level
+// shapes per width), each in both modes.
 constexpr int count_variants() {
   int c = 0;
   for (int n = 1; n <= kMaxBatchN; ++n) {
     c += 2;
     for (int k0 = 4; k0 <= n - 4; k0 += 4)
-      for (int j = 0; j < kNestShapesPerK0; ++j) c += nest_menu_has(kMenuBatch, n, k0, j) ? 1 : 0;
+      for (int j = 0; j < kNestShapesPerK0; ++j)
+        c += nest_menu_has(kMenuBatch, n, k0, j) ? 2 : 0;
   }
   return c;
 }
@@ -40,6 +41,7 @@ constexpr std::array<Variant, kNumVariants> make_variants() {
         if (nest_menu_has(kMenuBatch, n, k0, j)) {
           const NestShape s = nest_menu_shape(n, k0, j);
           v[c++] = {n, s.k0, s.k1, s.k2, kFast};
+          v[c++] = {n, s.k0, s.k1, s.k2, kDDProd};
         }
   }
   return v;
@@ -88,22 +90,20 @@ constexpr std::array<const void*, sizeof...(I)> variant_kernels(std::integer_seq
                                                     kVariants[LO + I].k2>...}};
 }
 
-// Kernel entry of variant vid for vid in [lo, hi) (one translation unit per
-// range keeps the build parallel); null outside the range.
-const void* batch_kernel_range_a(int vid);
-const void* batch_kernel_range_b(int vid);
-const void* batch_kernel_range_c(int vid);
-const void* batch_kernel_range_d(int vid);
-constexpr int kBatchRangeEnds[4] = {kNumVariants / 4, kNumVariants / 2, 3 * kNumVariants / 4,
-                                    kNumVariants};
+// Kernel entry of variant vid for vid in range IDX of kBatchRanges (one
+// translation unit per range keeps the build parallel); null outside it.
+constexpr int kBatchRanges = 8;
+template <int IDX>
+const void* batch_kernel_range(int vid);
+constexpr int batch_range_lo(int idx) { return (int)((long)kNumVariants * idx / kBatchRanges); }
 
 }  // namespace perm
 
-#define PERM_BATCH_RANGE(TAG, IDX)                                                         \
+#define PERM_BATCH_RANGE(IDX)                                                              \
   namespace perm {                                                                         \
-  const void* batch_kernel_range_##TAG(int vid) {                                          \
-    constexpr int lo = IDX == 0 ? 0 : kBatchRangeEnds[IDX == 0 ? 0 : IDX - 1];             \
-    constexpr int hi = kBatchRangeEnds[IDX];                                               \
+  template <>                                                                              \
+  const void* batch_kernel_range<IDX>(int vid) {                                           \
+    constexpr int lo = batch_range_lo(IDX), hi = batch_range_lo(IDX + 1);                  \
     static const auto table = variant_kernels<lo>(std::make_integer_sequence<int, hi - lo>{}); \
     return vid >= lo && vid < hi ? table[vid - lo] : nullptr;                              \
   }                                                                                        \

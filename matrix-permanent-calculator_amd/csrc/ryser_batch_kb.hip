@@ -1,4 +1,4 @@
-// Batch kernel variants, range 1 of 4 (see ryser_batch_impl.hpp).
+// Batch kernel variants, range 1 of 8 (see ryser_batch_impl.hpp).
 #include "ryser_batch_impl.hpp"
 
-PERM_BATCH_RANGE(b, 1)
+PERM_BATCH_RANGE(1)

@@ -1,4 +1,4 @@
-// Batch kernel variants, range 3 of 4 (see ryser_batch_impl.hpp).
+// Batch kernel variants, range 3 of 8 (see ryser_batch_impl.hpp).
 #include "ryser_batch_impl.hpp"
 
-PERM_BATCH_RANGE(d, 3)
+PERM_BATCH_RANGE(3)
