@@ -1,7 +1,13 @@
-"""Shared test setup: import paths for the package and the oracle, and the
-`gpu` marker (tests that need an MI355X; everything else runs on the CPU)."""
+"""Shared test setup: import paths for the package and the oracle, the
+`gpu` marker (tests that need an MI355X; everything else runs on the CPU),
+and oracle values of the larger test matrices read from a golden fixture."""
+import hashlib
+import json
 import os
 import sys
+from fractions import Fraction
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "matrix-permanent-calculator_amd")
@@ -14,3 +20,33 @@ for p in (PKG, ORACLE):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a GPU (MI355X); run with -m gpu")
+
+
+_CACHE = None
+
+
+def matrix_key(mat) -> str:
+    """Key of a matrix in tests/golden/oracle_cache.json: dtype, shape, bytes."""
+    a = np.ascontiguousarray(mat)
+    h = hashlib.sha256()
+    h.update(str(a.dtype).encode())
+    h.update(str(a.shape).encode())
+    h.update(a.tobytes())
+    return h.hexdigest()[:32]
+
+
+def oracle_value(mat) -> float:
+    """The permanent of `mat` from the oracle: read from the golden cache
+    (written by tests/golden/make_golden.py --cache: exact for n <= 26 and
+    for integer matrices, double-double Ryser otherwise) when the matrix is
+    listed there, else computed with the exact oracle."""
+    global _CACHE
+    if _CACHE is None:
+        path = os.path.join(GOLDEN, "oracle_cache.json")
+        _CACHE = json.load(open(path)) if os.path.exists(path) else {}
+    rec = _CACHE.get(matrix_key(mat))
+    if rec is not None:
+        return float(Fraction(rec["exact"])) if "exact" in rec else rec["dd_hi"] + rec["dd_lo"]
+    import oracle as O
+
+    return float(O.exact_permanent(mat))

@@ -7,6 +7,10 @@ matrices and expected permanents.
   brute force (n <= 8) or a closed form where one exists.
 * c1_exact.json    -- the CPU configuration matrix (n=20, density 0.5,
   nonzeros U(0,1], seed 1) and its exact permanent.
+* oracle_cache.json -- oracle values of the larger matrices the GPU tests
+  use (keyed by dtype, shape and bytes; see tests/conftest.py): exact for
+  n <= 26 and for integer matrices, double-double Ryser otherwise.  Written
+  with --cache.
 * refdata.json     -- a few input files of the reference's own data set
   (int/, float/, double/, n = 30) copied verbatim as text, with their
   permanents from the double-double oracle (rounded to the integer for the
@@ -56,9 +60,59 @@ def small_cases():
     return cases
 
 
+def cache_cases():
+    """The matrices of the slowest oracle checks in the GPU tests, built as
+    the tests build them."""
+    cases = []
+    # test_gpu_batch.py::test_batch_nested_shapes_vs_exact_and_single
+    for n in (24, 26):
+        for d in (0.2, 0.35):
+            cases.append(synthetic_matrix(n, d, seed=500 + 7 * n + int(10 * d), full_support=True))
+    # test_gpu_nested.py::test_nested_vs_exact
+    for n, d, seed in ((24, 0.2, 1), (24, 0.35, 2), (25, 0.5, 3), (26, 0.3, 4), (25, 0.7, 5)):
+        cases.append(synthetic_matrix(n, d, seed, full_support=True))
+    # test_gpu_accuracy.py::test_error_estimate_covers_observed
+    for n, d in ((18, 0.5), (20, 0.9), (22, 0.3), (24, 0.6), (25, 0.5)):
+        cases.append(synthetic_matrix(n, d, seed=900 + n))
+    # test_gpu_serve.py::test_server_results_vs_oracle (integer and large requests)
+    for i, n in enumerate(list(range(1, 21)) + [22, 24, 26, 28, 30]):
+        binary = i % 3 == 0
+        m = synthetic_matrix(n, 0.3 + 0.5 * ((i * 7) % 10) / 10, seed=1000 + i, binary=binary)
+        m = m.astype(np.int32) if binary else m
+        if n >= 18 or binary:
+            cases.append(m)
+    return cases
+
+
+def _cache_entry(a):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import matrix_key  # noqa: E402
+
+    n = a.shape[0]
+    rec = {"n": n, "dtype": str(a.dtype)}
+    if n <= 26 or np.issubdtype(a.dtype, np.integer):
+        rec["exact"] = frac_str(O.exact_permanent(a))
+    else:
+        rec["dd_hi"], rec["dd_lo"] = O.ryser_dd(a.astype(np.float64), threads=2)
+    return matrix_key(a), rec
+
+
+def cache():
+    out = {}
+    with ProcessPoolExecutor(max_workers=8) as pool:
+        for key, rec in pool.map(_cache_entry, cache_cases()):
+            out[key] = rec
+            print(key, rec.get("exact", rec.get("dd_hi")), flush=True)
+    with open(os.path.join(OUT, "oracle_cache.json"), "w") as fh:
+        json.dump(out, fh, indent=0, sort_keys=True)
+
+
 def main():
     if "--refdata-only" in sys.argv:
         refdata()
+        return
+    if "--cache" in sys.argv:
+        cache()
         return
     out = []
     for name, a in small_cases():

@@ -12,6 +12,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
+from conftest import oracle_value
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +24,7 @@ def rel(a, b):
 @pytest.mark.parametrize("n,density", [(18, 0.5), (20, 0.9), (22, 0.3), (24, 0.6), (25, 0.5)])
 def test_error_estimate_covers_observed(n, density):
     a = P.synthetic_matrix(n, density, seed=900 + n)
-    ex = float(O.exact_permanent(a))
+    ex = oracle_value(a)
     r = P.dense(a, 4)
     assert abs(r.value - ex) <= r.err_est, (n, r.value, ex, r.err_est)
     assert r.err_est <= r.err_bound

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
+from conftest import oracle_value
 
 pytestmark = pytest.mark.gpu
 
@@ -127,7 +128,7 @@ def test_batch_nested_shapes_vs_exact_and_single():
         one = P.dense(m)
         assert abs(r.value - one.value) <= r.err_bound + one.err_bound, (n, r, one)
         if n <= 26:
-            ex = float(O.exact_permanent(m))
+            ex = oracle_value(m)
             assert abs(r.value - ex) <= r.err_bound
             assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_est)
     print("batch shapes:", sorted(shapes))

@@ -8,6 +8,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
+from conftest import oracle_value
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +24,7 @@ def run_with(monkeypatch, env, fn):
                                             (26, 0.3, 4), (25, 0.7, 5)])
 def test_nested_vs_exact(n, density, seed):
     a = P.synthetic_matrix(n, density, seed, full_support=True)
-    ex = float(O.exact_permanent(a))
+    ex = oracle_value(a)
     for sparse in (False, True):
         r = P.sparse(a) if sparse else P.dense(a)
         print(f"n={n} d={density} shape=({r.nu},{r.nu1},{r.nu2}) rel={abs(r.value - ex) / abs(ex):.2e}")

@@ -208,3 +208,23 @@ def test_reader_on_reference_data_copy(tmp_path):
         if rec["type"] == "int":
             # the double-double oracle of an integer matrix is an integer
             assert abs(rec["dd_hi"] + rec["dd_lo"] - float(int(rec["rounded"]))) < 1.0
+
+
+def test_oracle_cache_matches_its_matrices_and_recomputes():
+    """tests/golden/oracle_cache.json: one entry per matrix of
+    make_golden.cache_cases(), and the cheap entries recompute exactly."""
+    import importlib.util
+
+    from conftest import GOLDEN, matrix_key, oracle_value
+
+    spec = importlib.util.spec_from_file_location("make_golden",
+                                                  os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    cache = json.load(open(os.path.join(GOLDEN, "oracle_cache.json")))
+    mats = mg.cache_cases()
+    assert {matrix_key(m) for m in mats} == set(cache)
+    cheap = [m for m in mats if m.shape[0] <= 20]
+    assert cheap
+    for m in cheap[:3]:
+        assert oracle_value(m) == float(O.exact_permanent(m))
