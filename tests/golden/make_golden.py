@@ -74,6 +74,12 @@ def cache_cases():
     # test_gpu_accuracy.py::test_error_estimate_covers_observed
     for n, d in ((18, 0.5), (20, 0.9), (22, 0.3), (24, 0.6), (25, 0.5)):
         cases.append(synthetic_matrix(n, d, seed=900 + n))
+    # test_gpu_accuracy.py::test_every_split_within_stated_error
+    cases.append(synthetic_matrix(24, 0.5, seed=24))
+    # test_gpu_nested.py::test_integer_matrices_exact_under_every_shape
+    rng = np.random.default_rng(3)
+    for n in (24, 28):
+        cases.append(((rng.random((n, n)) < 0.3) * rng.integers(1, 3, (n, n))).astype(np.int32))
     # test_gpu_serve.py::test_server_results_vs_oracle (integer and large requests)
     for i, n in enumerate(list(range(1, 21)) + [22, 24, 26, 28, 30]):
         binary = i % 3 == 0
