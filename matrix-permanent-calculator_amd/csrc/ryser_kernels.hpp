@@ -45,6 +45,7 @@ struct SkipLaunch {
   Partial* result;            // device: 1 record
   uint64_t lanes, blocks;
   int n, walk_bits, high_bits, check_bits, mode;  // mode 0 FP64, 1 double-double products
+  int lane_reverse = 0;  // 1: lane l fixes the high columns by the bit-reversal of l
 };
 // Smallest n with a skipping kernel (below it the plain kernel takes microseconds).
 constexpr int kMinSkipN = 16;

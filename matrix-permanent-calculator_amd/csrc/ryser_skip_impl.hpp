@@ -148,14 +148,19 @@ __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
                        const int* __restrict__ row_level, Partial* __restrict__ partials,
                        uint32_t* __restrict__ walked, uint64_t lanes, int n_walk_bits,
-                       int n_high_bits, int check_bits) {
+                       int n_high_bits, int check_bits, int lane_reverse) {
   __shared__ Partial scratch[256 / kWave];
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = lane < lanes;
+  // lanes of one wave differ in the low bits of the lane id: reversed, they
+  // differ in the last high columns instead
+  uint64_t pattern = valid ? lane : 0ull;
+  if (lane_reverse && n_high_bits > 0)
+    pattern = __builtin_bitreverse64(pattern) >> (64 - n_high_bits);
   DD s;
   double a, q;
   uint32_t wg;
-  skip_walk<N, R, MODE>(colpm, x0, row_level, valid ? lane : 0ull, valid, n_walk_bits,
+  skip_walk<N, R, MODE>(colpm, x0, row_level, pattern, valid, n_walk_bits,
                         n_high_bits, check_bits, s, a, q, wg);
   if (!valid) {
     s = {0.0, 0.0};
@@ -175,11 +180,11 @@ hipError_t launch_skip_n(const SkipLaunch& L, hipStream_t stream) {
   if (L.mode == kFast)
     ryser_skip_kernel<N, R, kFast><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
         L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits);
+        L.check_bits, L.lane_reverse);
   else if (L.mode == kDDProd)
     ryser_skip_kernel<N, R, kDDProd><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
         L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits);
+        L.check_bits, L.lane_reverse);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
