@@ -20,6 +20,8 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
+# OpenMP threads of the CPU legs stay on their cores (read when libgomp loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 # HBM bytes per launch of the n=40 kernel on this workload: rocprofv3 FETCH_SIZE
 # + WRITE_SIZE, each scaled by its reading for a copy kernel of known size
@@ -29,6 +31,17 @@ FP64_PEAK_TFLOPS = 78.6
 N = 40
 DENSITY = 0.5
 SEED = 2
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def cpu_baseline(mat, log2_sample: int, threads: int):
@@ -45,8 +58,10 @@ def cpu_baseline(mat, log2_sample: int, threads: int):
         "unit": "subsets/s",
         "cores": threads,
         "kind": "port",
+        "cpu": cpu_model(),
         "sample": (f"parallel_perman64 float-X restatement (algo.h:662-746), OpenMP "
-                   f"{threads} threads, Gray slice of 2^{log2_sample} subsets of the same "
+                   f"{threads} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), "
+                   f"Gray slice of 2^{log2_sample} subsets of the same "
                    f"n=40 matrix ({dt:.2f} s); full n=40 run extrapolated "
                    f"x{2 ** (N - 1 - log2_sample)}: {dt * 2 ** (N - 1 - log2_sample):.0f} s"),
     }
