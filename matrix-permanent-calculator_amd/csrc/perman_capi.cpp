@@ -81,6 +81,8 @@ struct DeviceCache {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
   void* base = nullptr;
   size_t bytes = 0;
+  char* stage = nullptr;  // pinned upload staging: [column store | pad | x0]
+  size_t stage_bytes = 0;
 };
 
 DeviceCache& device_cache(int dev) {
@@ -291,7 +293,23 @@ int bind_oneshot(int n, const Prepared& prep, DeviceCache& C, perm_plan& P) {
     C.bytes = need;
   }
   bind_plan_memory(&P, C.base, cs.size());
-  return upload_plan(&P, cs, prep.x0);
+  // one asynchronous copy from pinned staging in the device layout; the
+  // stream orders the kernels after it, and the staging is reused only by
+  // the next call under C.mu, after this call's final synchronisation
+  const size_t col_bytes = (const char*)P.L.x0 - (const char*)P.L.colpm;
+  const size_t up = col_bytes + prep.x0.size() * sizeof(double);
+  if (up > C.stage_bytes) {
+    if (C.stage) HIP_TRY(hipHostFree(C.stage));
+    C.stage = nullptr;
+    C.stage_bytes = 0;
+    HIP_TRY(hipHostMalloc((void**)&C.stage, up, hipHostMallocDefault));
+    C.stage_bytes = up;
+  }
+  HIP_TRY(hipStreamSynchronize(P.stream));  // idle unless an earlier call failed midway
+  std::memcpy(C.stage, cs.data(), cs.size() * sizeof(double));
+  std::memcpy(C.stage + col_bytes, prep.x0.data(), prep.x0.size() * sizeof(double));
+  HIP_TRY(hipMemcpyAsync((void*)P.L.colpm, C.stage, up, hipMemcpyHostToDevice, P.stream));
+  return PERM_OK;
 }
 
 // One-shot run through the per-device cache (mode as in plan_run_impl).
