@@ -150,3 +150,32 @@ def test_reference_data_files_n30():
             want = float(int(rec["rounded"]))
         assert rel(r.value, want) < 1e-10
         assert abs(r.value - want) <= r.err_bound
+
+
+def test_cli_on_reference_data_files(tmp_path):
+    """`perman -f <reference data file>` (dense -p4 and sparse -s -p4): the
+    reference's own n = 30 inputs read by the CLI's reader, the Result line
+    against the double-double oracle."""
+    import re
+
+    from test_capi import run_cli
+
+    path = os.path.join(GOLDEN, "refdata.json")
+    if not os.path.exists(path):
+        pytest.skip("refdata fixture not generated")
+    for rec in json.load(open(path)):
+        f = tmp_path / rec["file"].replace("/", "_")
+        f.write_text(rec["text"])
+        assert all(float(line.split()[2]) > 0 for line in rec["text"].splitlines()[1:] if line)
+        want = rec["dd_hi"] + rec["dd_lo"]
+        if rec["type"] == "int":
+            want = float(int(rec["rounded"]))
+        for args in (("-p", "4"), ("-s", "-p", "4")):
+            r = run_cli("-f", str(f), "-g", *args)
+            assert r.returncode == 0, (rec["file"], args, r.stderr)
+            m = re.search(r"^Result: \S+ (\S+) in \S+$", r.stdout, re.M)
+            v = float(m.group(1))
+            print(f"{rec['file']} {' '.join(args)}: {v!r} (oracle {want!r})")
+            # (the sparse path keeps only entries > 0, as the reference's CCS;
+            # these files list positive entries only, so both paths agree)
+            assert rel(v, want) < 1e-10, (rec["file"], args, v, want)
