@@ -179,3 +179,18 @@ def test_cli_on_reference_data_files(tmp_path):
             # (the sparse path keeps only entries > 0, as the reference's CCS;
             # these files list positive entries only, so both paths agree)
             assert rel(v, want) < 1e-10, (rec["file"], args, v, want)
+
+
+def test_largest_sizes_exact_tilings():
+    """The largest orders: the dense path at n = 42 (grid 6x14, 2^41 subsets)
+    and the skipper at n = 48 = kMaxDenseN (grid 8x12), against the domino
+    tiling counts (Kasteleyn's product formula)."""
+    G42 = O.grid_graph_matrix(6, 14).astype(np.int32)
+    r = P.dense(G42, 4)
+    print(f"dense n=42: {r.value!r} mode={r.mode} kernel_ms={r.kernel_ms:.1f}")
+    assert r.value == float(O.domino_tilings(6, 14))
+    G48 = O.grid_graph_matrix(8, 12).astype(np.int32)
+    s = P.skip(G48)
+    print(f"skipper n=48: {s.value!r} walked {s.main_subsets / s.subsets:.2e} "
+          f"kernel_ms={s.kernel_ms:.1f}")
+    assert s.value == float(O.domino_tilings(8, 12))
