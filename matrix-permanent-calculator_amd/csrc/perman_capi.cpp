@@ -206,13 +206,17 @@ int make_plan(int n, const Prepared& prep, perm_plan** out) {
 // ms = device time of all its kernels, main_ms = of the main pass.
 int run_pass(perm_plan* P, int mode, perm::Partial& r, float& ms, float& main_ms) {
   configure_passes(P, mode);
+  // one workgroup (n <= 19): its record is the sum, the walk the only kernel
+  const bool single = P->L.total_blocks <= 1;
   HIP_TRY(hipEventRecord(P->ev0, P->stream));
-  HIP_TRY(perm::launch_dense(P->L, P->stream, P->evm));
+  HIP_TRY(perm::launch_dense(P->L, P->stream, single ? nullptr : P->evm));
   HIP_TRY(hipEventRecord(P->ev1, P->stream));
-  HIP_TRY(hipMemcpyAsync(&r, P->L.result, sizeof(r), hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipMemcpyAsync(&r, single ? P->L.partials : P->L.result, sizeof(r),
+                         hipMemcpyDeviceToHost, P->stream));
   HIP_TRY(hipStreamSynchronize(P->stream));
   HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1));
-  HIP_TRY(hipEventElapsedTime(&main_ms, P->ev0, P->evm));
+  if (single) main_ms = ms;
+  else HIP_TRY(hipEventElapsedTime(&main_ms, P->ev0, P->evm));
   return PERM_OK;
 }
 

@@ -74,7 +74,7 @@ hipError_t launch_reduce(const Partial* in, uint64_t count, Partial* out, hipStr
 
 hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   hipError_t e = launch_passes(L, stream, main_done);
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || L.total_blocks <= 1) return e;  // one record: it is the sum
   reduce_partials_kernel<<<1, 256, 0, stream>>>(L.partials, L.total_blocks, L.result);
   return hipGetLastError();
 }

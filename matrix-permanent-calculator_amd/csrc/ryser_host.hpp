@@ -62,7 +62,8 @@ inline void build_column_store(int n, ColFn col, std::vector<double>& out) {
 // lane sums -- so shorter walks give a more accurate sum.  Measured at
 // n = 40 (FP64 against the double-double mode): 2^18-step walks 2.3e-10,
 // 2^12-step walks 6e-12 relative error, in the same time.  Default:
-//   n <= 32: lanes = 2^min(21, n-11), at least 2^10 steps per lane;
+//   n <= 32: lanes = 2^min(n-1-R, max(n-11, 18)) (at most 2^21): walks of
+//            2^R steps up to n = 22, 2^10 steps from n = 29 on;
 //   n >= 33: 2^12 steps per lane, at most 2^28 lanes (2^20 workgroup
 //            partials), longer walks beyond n = 41.
 struct Split {
@@ -74,7 +75,10 @@ inline Split choose_split(int n, int unroll_bits, int walk_override) {
   const int B = n - 1;
   int h;
   if (B <= 31) {
-    h = B - 10;
+    // many short walks: up to 2^18 lanes (the chip holds ~2^18-2^19 in
+    // flight), so a small permanent is one short wave-parallel walk rather
+    // than a few long serial ones; 2^(B-10) lanes when that is more
+    h = std::min(B - unroll_bits, std::max(B - 10, 18));
     if (h > 21) h = 21;
     if (h < 0) h = 0;
   } else {

@@ -53,7 +53,8 @@ constexpr int kMinSkipN = 16;
 hipError_t launch_skip(const SkipLaunch& L, hipStream_t stream);
 
 // Enqueue the passes of the dense kernel and the grid reduction of all
-// partials on `stream`.
+// partials on `stream` (with a single workgroup record, no reduction: the
+// sum is partials[0]).
 // `main_done`, if not null, is recorded on `stream` after the main pass.
 hipError_t launch_dense(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done = nullptr);
 // Only the walk of L (its block range), no reduction (the caller sums the
