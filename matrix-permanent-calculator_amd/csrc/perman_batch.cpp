@@ -200,9 +200,12 @@ int build(perm_batch* B) {
   HIP_TRYB(hipMemcpyAsync(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
                           hipMemcpyHostToDevice, s0));
   HIP_TRYB(hipStreamSynchronize(s0));
-  // capture: reset -> fork -> variant kernels round-robin over streams -> join
-  // -> reduce -> result copy
+  // capture: reset -> fork -> variant kernels over streams -> join -> reduce ->
+  // result copy.  Groups come in decreasing work; each goes to the stream with
+  // the least work so far (longest-processing-time first), so the streams end
+  // close together instead of one long tail of round-robin leftovers.
   const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  std::vector<double> stream_work(ns, 0.0);
   hipGraph_t g;
   HIP_TRYB(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
   HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)B->groups.size(), s0));
@@ -212,9 +215,11 @@ int build(perm_batch* B) {
     const auto& G = B->groups[gi];
     const uint32_t waves = G.end - G.begin;
     const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, B->max_grid));
+    const int si = (int)(std::min_element(stream_work.begin(), stream_work.end()) -
+                         stream_work.begin());
+    stream_work[si] += G.work;
     HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
-                                        B->d_counters + gi, B->d_partials, grid,
-                                        B->s[gi % ns]));
+                                        B->d_counters + gi, B->d_partials, grid, B->s[si]));
   }
   for (int i = 1; i < ns; ++i) {
     HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
