@@ -13,6 +13,7 @@ on the host cores over a bounded Gray slice of the same matrix.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -31,6 +32,19 @@ FP64_PEAK_TFLOPS = 78.6
 N = 40
 DENSITY = 0.5
 SEED = 2
+
+
+@contextlib.contextmanager
+def affinity_kept():
+    """OpenMP under OMP_PROC_BIND pins the calling thread to one core at its
+    first parallel region, and threads started later (the server's scheduler
+    thread) inherit that mask; the CPU legs run inside this so the GPU legs
+    that follow them keep the process's full CPU set."""
+    mask = os.sched_getaffinity(0)
+    try:
+        yield
+    finally:
+        os.sched_setaffinity(0, mask)
 
 
 def cpu_model() -> str:
@@ -365,7 +379,8 @@ def main():
     if not args.no_mixed:
         out["binary_n40"] = binary_leg(P, 2)
     out["sparse"] = sparse_leg(P, 3)
-    out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
+    with affinity_kept():
+        out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
     if not args.no_mixed:
         out["batched"] = batched_leg(P, args.batch_count, 5)
         if args.serve_count > 0:
@@ -375,7 +390,8 @@ def main():
             out["served_small_n"] = served_leg(P, args.serve_count, args.serve_load, 7,
                                                decay=1.0)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mat, args.cpu_sample_log2, args.cpu_threads)
+        with affinity_kept():
+            out["cpu_baseline"] = cpu_baseline(mat, args.cpu_sample_log2, args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

@@ -123,7 +123,9 @@ void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector
   std::stable_sort(by_count.begin(), by_count.end(), [&](int a, int b) {
     return __builtin_popcountll(sup[a]) < __builtin_popcountll(sup[b]);
   });
-  const int n0 = std::min(B, 8), n12 = nested ? std::min(B, 12) : 0;
+  // pair shapes only: the cost grows with column 0's support, so the
+  // sparsest column (first in by_count) is the choice
+  const int n0 = std::min(B, nested ? 8 : 1), n12 = nested ? std::min(B, 12) : 0;
   int best_cost = perm::nest_cost(n, {n, n, n});
   perm::NestShape best{n, n, n};
   int bc[3] = {-1, -1, -1};
@@ -158,6 +160,18 @@ void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector
       const int b = by_count[i1];
       if (b == a) continue;
       const uint64_t u01 = sup[a] | sup[b];
+      // cover() can only grow with the third union's size, so no column 2
+      // beats best_cost unless the smallest union does: skip the scan then
+      // (the choice is the one the full scan makes, at a tenth of the cost)
+      int min012 = n + 1;
+      for (int i2 = 0; i2 < n12; ++i2) {
+        const int c = by_count[i2];
+        if (c != a && c != b) min012 = std::min(min012, __builtin_popcountll(u01 | sup[c]));
+      }
+      if (min012 > n) continue;
+      perm::NestShape lb_shape;
+      const int lb = cover(c0, __builtin_popcountll(u01), min012, lb_shape);
+      if (lb < 0 || lb >= best_cost) continue;
       for (int i2 = 0; i2 < n12; ++i2) {
         const int c = by_count[i2];
         if (c == a || c == b) continue;

@@ -211,8 +211,13 @@ bool finalize(perm_server* S, Request* r) {
 }
 
 // Copy newly admitted requests into their arena slots: each into the pinned
-// mirror, then one asynchronous copy per run of consecutive slots on s0,
-// ahead of any kernel that reads them.
+// mirror, then one asynchronous copy per run of nearby slots on s0, ahead of
+// any kernel that reads them.  A run may span up to kUploadGapSlots slots
+// that are not fresh: a resident slot's mirror holds the bytes its device
+// copy already has, a free slot's content is unused, so copying them over is
+// harmless and cheaper than another copy call (a slot is ~20 KB).
+constexpr int kUploadGapSlots = 4;
+
 int upload(perm_server* S, std::vector<Request*>& fresh) {
   std::sort(fresh.begin(), fresh.end(),
             [](const Request* a, const Request* b) { return a->slot < b->slot; });
@@ -223,7 +228,7 @@ int upload(perm_server* S, std::vector<Request*>& fresh) {
       double* h = S->h_arena + (size_t)r->slot * kSlotDoubles;
       std::memcpy(h, r->prep.cs.data(), r->prep.cs.size() * sizeof(double));
       std::memcpy(h + r->prep.cs.size(), r->prep.x0.data(), r->prep.x0.size() * sizeof(double));
-      if (j + 1 == fresh.size() || fresh[j + 1]->slot != r->slot + 1) break;
+      if (j + 1 == fresh.size() || fresh[j + 1]->slot > r->slot + 1 + kUploadGapSlots) break;
       ++j;
     }
     const Request* last = fresh[j];

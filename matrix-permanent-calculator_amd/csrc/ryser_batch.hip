@@ -20,11 +20,25 @@
 
 namespace perm {
 
+// First variant id of each order n (the table is sorted by n); entry
+// kMaxBatchN + 1 is kNumVariants.
+constexpr std::array<int, kMaxBatchN + 2> make_variant_start() {
+  std::array<int, kMaxBatchN + 2> st{};
+  int i = 0;
+  for (int n = 0; n <= kMaxBatchN + 1; ++n) {
+    while (i < kNumVariants && kVariants[i].n < n) ++i;
+    st[n] = i;
+  }
+  return st;
+}
+constexpr std::array<int, kMaxBatchN + 2> kVariantStart = make_variant_start();
+
 // Host lookup: variant id of walk shape s and kernel mode for order n, or -1.
 int batch_variant(int n, NestShape s, int mode) {
-  for (int i = 0; i < kNumVariants; ++i)
-    if (kVariants[i].n == n && kVariants[i].k0 == s.k0 && kVariants[i].k1 == s.k1 &&
-        kVariants[i].k2 == s.k2 && kVariants[i].mode == mode)
+  if (n < 1 || n > kMaxBatchN) return -1;
+  for (int i = kVariantStart[n]; i < kVariantStart[n + 1]; ++i)
+    if (kVariants[i].k0 == s.k0 && kVariants[i].k1 == s.k1 && kVariants[i].k2 == s.k2 &&
+        kVariants[i].mode == mode)
       return i;
   return -1;
 }
@@ -33,9 +47,10 @@ int batch_variant(int n, NestShape s, int mode) {
 // inside the variant's prefixes; extra rows add zeros); (n, n, n) if none.
 NestShape batch_shape(int n, NestShape want) {
   NestShape best{n, n, n};
-  for (int i = 0; i < kNumVariants; ++i) {
+  if (n < 1 || n > kMaxBatchN) return best;
+  for (int i = kVariantStart[n]; i < kVariantStart[n + 1]; ++i) {
     const Variant& v = kVariants[i];
-    if (v.n != n || v.mode != kFast || v.k0 < want.k0 || v.k1 < want.k1 || v.k2 < want.k2)
+    if (v.mode != kFast || v.k0 < want.k0 || v.k1 < want.k1 || v.k2 < want.k2)
       continue;
     if (nest_cost(n, {v.k0, v.k1, v.k2}) < nest_cost(n, best)) best = {v.k0, v.k1, v.k2};
   }

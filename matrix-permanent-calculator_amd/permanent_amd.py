@@ -448,25 +448,29 @@ def last_error() -> str:
 class Request:
     """A matrix ready to hand to the library: the contiguous array, its dtype
     code and, for the sparse path, its CCS (built once, outside any timing)."""
-    __slots__ = ("a", "dt", "sparse", "cptrs", "rows", "cvals")
+    __slots__ = ("a", "dt", "n", "sparse", "cptrs", "rows", "cvals", "_args")
 
     def __init__(self, mat, sparse_path: bool = False):
         self.a, self.dt = _mat(mat)
+        self.n = self.a.shape[0]
         self.sparse = bool(sparse_path)
         if self.sparse:
             self.cptrs, self.rows, self.cvals = compress(self.a)
+            # the C arguments, converted once (the arrays above keep them valid)
+            self._args = (self.dt, self.a.ctypes.data, _ip(self.cptrs), _ip(self.rows),
+                          self.cvals.ctypes.data, self.n)
         else:
             self.cptrs = self.rows = self.cvals = None
+            self._args = (self.dt, self.a.ctypes.data, self.n)
 
 
 def solve(req: Request) -> PermResult:
     """One plain call for a prepared request (perm_dense / perm_sparse, ID 4)."""
     r = PermResultC()
     if req.sparse:
-        st = lib().perm_sparse(req.dt, req.a.ctypes.data, _ip(req.cptrs), _ip(req.rows),
-                               req.cvals.ctypes.data, req.a.shape[0], 4, ctypes.byref(r))
+        st = lib().perm_sparse(*req._args, 4, ctypes.byref(r))
     else:
-        st = lib().perm_dense(req.dt, req.a.ctypes.data, req.a.shape[0], 4, ctypes.byref(r))
+        st = lib().perm_dense(*req._args, 4, ctypes.byref(r))
     _check(st)
     return PermResult.from_c(r)
 
@@ -502,13 +506,9 @@ class Server:
     def submit_request(self, req: Request) -> int:
         tid = ctypes.c_ulonglong()
         if req.sparse:
-            st = lib().perm_server_submit_sparse(self._h, req.dt, req.a.ctypes.data,
-                                                 _ip(req.cptrs), _ip(req.rows),
-                                                 req.cvals.ctypes.data, req.a.shape[0],
-                                                 ctypes.byref(tid))
+            st = lib().perm_server_submit_sparse(self._h, *req._args, ctypes.byref(tid))
         else:
-            st = lib().perm_server_submit_dense(self._h, req.dt, req.a.ctypes.data,
-                                                req.a.shape[0], ctypes.byref(tid))
+            st = lib().perm_server_submit_dense(self._h, *req._args, ctypes.byref(tid))
         _check(st)
         return tid.value
 
