@@ -127,6 +127,9 @@ struct perm_server {
   // neighbouring slots, uploaded by one copy)
   std::priority_queue<int, std::vector<int>, std::greater<int>> free_slots;
   double queued_cost = 0.0;
+  // work of the admitted items not yet issued (the sum of remaining_cost()
+  // over the active list), kept as items are admitted, requeued and issued
+  double unissued_cost = 0.0;
   // scheduler time per phase (ms) and upload copies: written by the
   // scheduler only, read by perm_server_stats_get
   std::atomic<double> t_admit{0.0}, t_issue{0.0}, t_retire{0.0}, t_wait{0.0}, t_gpu{0.0};
@@ -198,6 +201,7 @@ bool finalize(perm_server* S, Request* r) {
     std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0});
     r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape);
     S->queued_cost += (double)r->items * r->item_cost;
+    S->unissued_cost += (double)r->items * r->item_cost;
     S->active.push_back(r);
     S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
     return false;
@@ -243,28 +247,34 @@ int upload(perm_server* S, std::vector<Request*>& fresh) {
   return PERM_OK;
 }
 
+// Admit pending requests in arrival order while slots are free and queued
+// work stays under max_queued_work (the first one always, on an idle server);
+// the pending queue is taken under one lock.
 int admit(perm_server* S) {
   std::vector<Request*> fresh;
-  for (;;) {
-    Request* r = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      if (S->pending.empty() || S->free_slots.empty()) break;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    size_t slots = S->free_slots.size();
+    double queued = S->queued_cost;
+    while (!S->pending.empty() && slots > 0) {
       Request* c = S->pending.front();
       const double cost = (double)c->items * c->item_cost;
-      if (!(S->active.empty() && fresh.empty()) && S->queued_cost + cost > S->cfg.max_queued_work)
-        break;
+      if (!(S->active.empty() && fresh.empty()) && queued + cost > S->cfg.max_queued_work) break;
       S->pending.pop_front();
-      r = c;
-      S->stats.admitted += 1;
+      fresh.push_back(c);
+      queued += cost;
+      --slots;
     }
+    S->stats.admitted += (long long)fresh.size();
+  }
+  if (fresh.empty()) return PERM_OK;
+  for (Request* r : fresh) {
     r->slot = S->free_slots.top();
     S->free_slots.pop();
     r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0});
     S->queued_cost += (double)r->items * r->item_cost;
-    fresh.push_back(r);
+    S->unissued_cost += (double)r->items * r->item_cost;
   }
-  if (fresh.empty()) return PERM_OK;
   if (int st = upload(S, fresh)) return st;
   for (Request* r : fresh) S->active.push_back(r);
   return PERM_OK;
@@ -274,6 +284,10 @@ int admit(perm_server* S) {
 int issue(perm_server* S, bool& issued) {
   issued = false;
   if (S->n_busy >= S->cfg.inflight) return PERM_OK;
+  // continuous batching: while an iteration runs, let small work accumulate
+  // into the next one instead of paying an iteration's launch cost for it
+  // (checked on the running total, before any scan of the active list)
+  if (S->n_busy > 0 && S->unissued_cost < S->iter_cost / 16.0) return PERM_OK;
   std::vector<Request*> cand;
   double work = 0.0;
   for (Request* r : S->active)
@@ -281,10 +295,11 @@ int issue(perm_server* S, bool& issued) {
       cand.push_back(r);
       work += r->remaining_cost();
     }
-  if (cand.empty()) return PERM_OK;
-  // continuous batching: while an iteration runs, let small work accumulate
-  // into the next one instead of paying an iteration's launch cost for it
-  if (S->n_busy > 0 && work < S->iter_cost / 16.0) return PERM_OK;
+  if (cand.empty()) {
+    S->unissued_cost = 0.0;  // drop the rounding drift of the running total
+    return PERM_OK;
+  }
+  S->unissued_cost = work;
   std::sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
     const double ca = a->remaining_cost(), cb = b->remaining_cost();
     return ca != cb ? ca < cb : a->id < b->id;
@@ -302,6 +317,7 @@ int issue(perm_server* S, bool& issued) {
     I.segs.push_back({r, r->issued, (uint32_t)take, 0});
     r->issued += (uint32_t)take;
     budget -= (double)take * r->item_cost;
+    S->unissued_cost -= (double)take * r->item_cost;
     cap -= take;
   }
   // group by variant: segments of one variant are contiguous in the item list
