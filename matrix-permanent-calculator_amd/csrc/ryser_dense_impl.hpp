@@ -384,10 +384,16 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
     xl[j] = 0.0;
   }
 
-  // Closed-form start: add the high columns selected by the pattern
-  // (adding sel*c with sel in {0, 1} is exact selection, no branch).
+  // Closed-form start: add the high columns selected by the pattern.  Bits
+  // 0..5 differ between the lanes of a wave: adding sel*c with sel in {0, 1}
+  // is exact selection, no branch.  Bits 6 and up are the same in every lane
+  // of a wave (waves cover aligned runs of 64 lane ids, and pattern 0 past
+  // the last lane only occurs when that run is all 0 above bit 5), so those
+  // columns are added under a wave-uniform branch; skipping a column leaves
+  // x as adding its zero multiple does (x + 0 = x, up to the sign of a zero).
   int parity = 0;
-  for (int b = 0; b < n_high_bits; ++b) {
+  const int lane_bits = n_high_bits < 6 ? n_high_bits : 6;
+  for (int b = 0; b < lane_bits; ++b) {
     ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
     const int bit = (int)((pattern >> b) & 1);
     parity ^= bit;
@@ -395,6 +401,18 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
     double cv[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
+    add_column<N, MODE, N>(x, xl, cv);
+  }
+  const uint64_t wave_bits =
+      ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 38)) << 32) |
+      (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 6));
+  for (int b = lane_bits; b < n_high_bits; ++b) {
+    if (!((wave_bits >> (b - 6)) & 1)) continue;
+    parity ^= 1;
+    ColPtr c = cols + (size_t)(2 * (n_walk_bits + b)) * N;
+    double cv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) cv[j] = c[j];
     add_column<N, MODE, N>(x, xl, cv);
   }
 
