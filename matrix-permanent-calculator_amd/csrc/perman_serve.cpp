@@ -219,7 +219,7 @@ bool finalize(perm_server* S, Request* r) {
 // any kernel that reads them.  A run may span up to kUploadGapSlots slots
 // that are not fresh: a resident slot's mirror holds the bytes its device
 // copy already has, a free slot's content is unused, so copying them over is
-// harmless and cheaper than another copy call (a slot is ~20 KB).
+// harmless and cheaper than another copy call (a slot is ~26 KB).
 constexpr int kUploadGapSlots = 4;
 
 int upload(perm_server* S, std::vector<Request*>& fresh) {
