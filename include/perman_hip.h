@@ -39,14 +39,28 @@ extern "C" {
 
 /* Result of one permanent.  value = scale * (sum_hi + sum_lo) where
  * scale = 4(n&1)-2 and sum is the double-double Ryser sum; abs_sum is
- * |scale| * sum of |terms| (the condition of the alternating sum) and
- * err_bound the library's stated worst-case error for `value` (every rounding
- * aligned); err_est the expected error when roundings are independent,
- * (n + sqrt(n)) u sqrt(Q) with Q >= scale^2 * sum of term^2 measured by the
- * kernel.  mode: 0 = FP64 products (the fast kernel), 1 = double-double
- * products (used for integer-valued matrices whose FP64 result could not be
- * guaranteed to round to the exact integer), 2 = double-double X and
- * products. */
+ * |scale| * sum of |terms| (the condition of the alternating sum).
+ *
+ * err_bound: worst-case bound (every rounding aligned, first order in the
+ * unit roundoff u = 2^-53) of the product, accumulation and start-vector
+ * roundings.  For an integer matrix X is exact and this bounds the whole
+ * error.  For a non-integer matrix in mode 0 or 1 it does NOT include the
+ * drift of X along the Gray walk (each x_j accumulates the roundings of its
+ * column additions; how they move a term depends on how close x_j comes to
+ * zero, so no useful worst-case bound exists); err_est covers it.
+ * err_est: three standard deviations of the error when roundings behave as
+ * independent random errors, from moments the kernel measures (sums over
+ * runs of consecutive terms of the squared absolute and signed run sums);
+ * calibrated against the double-double and exact oracles.
+ * mode: 0 = FP64 products (the fast kernel), 1 = double-double products
+ * (integer-valued matrices: exact X, so the result rounds to the exact
+ * integer), 2 = double-double X and products (no X drift).  Results are held
+ * to 1e-10 relative: an FP64 result with err_est > 1e-10 |value| is computed
+ * again in mode 1 (integer matrices) or mode 2 (others), automatically, and
+ * `certified` says whether the final result meets the target (exact, or
+ * err_est <= 1e-10 |value|).  A matrix whose nonzero pattern has no perfect
+ * matching has permanent exactly 0: it is returned as 0 (zero_pattern = 1,
+ * errors 0) without running a walk. */
 typedef struct perm_result {
   double value;
   double sum_hi;
@@ -65,6 +79,8 @@ typedef struct perm_result {
   double main_subsets;  /* subsets that kernel evaluated */
   int nu1;              /* FP64 kernel: rows touched by Gray columns 0-1 (nested variant), or n */
   int nu2;              /* FP64 kernel: rows touched by Gray columns 0-2 (nested variant), or n */
+  int certified;        /* 1: exact, or err_est <= 1e-10 |value| */
+  int zero_pattern;     /* 1: no perfect matching in the nonzero pattern; value is exactly 0 */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */

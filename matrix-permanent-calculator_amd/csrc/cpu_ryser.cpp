@@ -256,7 +256,12 @@ int skip_cpu(int dtype_unused, const int* rptrs, const int* cols_idx, const T* r
   out->value = scale * (tot.hi + tot.lo);
   out->abs_sum = std::fabs(scale) * abs;
   out->err_bound = perm::error_statement(out->abs_sum, out->value, n, w, 3, integral, 0);
-  out->err_est = out->err_bound;
+  // no run moments on this path: the X drift of a non-integer matrix is
+  // estimated without cancellation, sqrt(n) sqrt(2^w + h) u A
+  out->err_est = out->err_bound +
+                 (integral ? 0.0
+                           : std::sqrt((double)n * (std::ldexp(1.0, w) + h)) *
+                                 std::ldexp(out->abs_sum, -53));
   out->subsets = std::ldexp(1.0, n - 1);
   out->main_subsets = evaluated;
   out->kernel_ms = (omp_get_wtime() - t0) * 1e3;
@@ -269,6 +274,7 @@ int skip_cpu(int dtype_unused, const int* rptrs, const int* cols_idx, const T* r
                               out->abs_sum, scale, n))
     out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(integral, out->value, out->err_bound, out->err_est);
+  out->certified = perm::is_certified(*out) ? 1 : 0;
   return perm::ok();
 }
 
@@ -290,13 +296,14 @@ namespace perm {
 // from the column store blocks 2(w+b) selected by the lane id, Gray walk over
 // the low w columns), FP64 X and products, lane sums folded into
 // double-double every 256 steps and combined in lane order.
-// out = {hi, lo, sum |term|, sum over runs of (run sum of |term|)^2}.
+// out = {hi, lo, sum |term|, sum over runs of (run sum of |term|)^2,
+// sum over runs of (signed run sum)^2}.
 void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int n,
                int walk_bits, int high_bits, uint64_t l0, uint64_t l1, int threads,
-               double out[4]) {
+               double out[5]) {
   const uint64_t count = l1 > l0 ? l1 - l0 : 0;
   std::vector<DDh> part(count);
-  std::vector<double> abs_part(count), sq_part(count);
+  std::vector<double> abs_part(count), sq_part(count), run_part(count);
   if (threads < 1) threads = 1;
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
   for (int64_t i = 0; i < (int64_t)count; ++i) {
@@ -311,7 +318,7 @@ void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int
         for (int j = 0; j < n; ++j) x[j] += c[j];
       }
     DDh acc;
-    double p = 0.0, a = 0.0, a_run = 0.0, q = 0.0;
+    double p = 0.0, a = 0.0, a_run = 0.0, q = 0.0, r = 0.0;
     const uint64_t steps = 1ull << walk_bits;
     for (uint64_t t = 0; t < steps; ++t) {
       if (t) {
@@ -325,12 +332,14 @@ void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int
       a += std::fabs(prod);
       if ((t & 255) == 255) {
         acc = dd_add_d(acc, p);
+        r += p * p;
         p = 0.0;
         q += (a - a_run) * (a - a_run);
         a_run = a;
       }
     }
     acc = dd_add_d(acc, p);
+    r += p * p;
     q += (a - a_run) * (a - a_run);
     if (parity) {
       acc.hi = -acc.hi;
@@ -339,18 +348,21 @@ void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int
     part[i] = acc;
     abs_part[i] = a;
     sq_part[i] = q;
+    run_part[i] = r;
   }
   DDh tot;
-  double a = 0.0, q = 0.0;
+  double a = 0.0, q = 0.0, r = 0.0;
   for (uint64_t i = 0; i < count; ++i) {
     tot = dd_add(tot, part[i]);
     a += abs_part[i];
     q += sq_part[i];
+    r += run_part[i];
   }
   out[0] = tot.hi;
   out[1] = tot.lo;
   out[2] = a;
   out[3] = q;
+  out[4] = r;
 }
 
 }  // namespace perm

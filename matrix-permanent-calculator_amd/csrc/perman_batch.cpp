@@ -38,6 +38,7 @@ struct Entry {
   int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
   perm::NestShape shape{0, 0, 0};
   bool integral = false;
+  bool zero = false;  // no perfect matching: exactly 0, no items
   double product_bits = 1e9;
   size_t col_off = 0, x0_off = 0;  // offsets (doubles) into the data buffer
   uint32_t first_item = 0, item_count = 0;
@@ -95,6 +96,7 @@ int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
   Entry e;
   e.n = n;
   e.integral = p.integral;
+  e.zero = p.zero;
   e.product_bits = p.product_bits;
   // the cheapest batch kernel shape covering the prepared one (extra rows
   // add zeros, exactly)
@@ -122,6 +124,59 @@ int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
+// Enqueue one run of the batch on its streams: reset -> fork -> variant
+// kernels over streams -> join -> reduce -> result copy.  Groups come in
+// decreasing work; each goes to the stream with the least work so far
+// (longest-processing-time first), so the streams end close together instead
+// of one long tail of round-robin leftovers.
+int enqueue_run(perm_batch* B) {
+  hipStream_t s0 = B->s[0];
+  const size_t nm = B->ents.size();
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  std::vector<double> stream_work(ns, 0.0);
+  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)B->groups.size(), s0));
+  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+  for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+    const auto& G = B->groups[gi];
+    const uint32_t waves = G.end - G.begin;
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, B->max_grid));
+    const int si = (int)(std::min_element(stream_work.begin(), stream_work.end()) -
+                         stream_work.begin());
+    stream_work[si] += G.work;
+    HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
+                                        B->d_counters + gi, B->d_partials, grid, B->s[si]));
+  }
+  for (int i = 1; i < ns; ++i) {
+    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+  }
+  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, (uint32_t)nm, s0));
+  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
+                          s0));
+  return PERM_OK;
+}
+
+// Capture one run into B->graph.  The capture is ended on every path, so a
+// failure while enqueueing leaves the stream usable for the next call.
+int capture(perm_batch* B) {
+  hipStream_t s0 = B->s[0];
+  hipGraph_t g = nullptr;
+  HIP_TRYB(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  const int st = enqueue_run(B);
+  const hipError_t ce = hipStreamEndCapture(s0, &g);
+  if (st != PERM_OK || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st != PERM_OK) return st;
+    return perm::fail(PERM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  }
+  const hipError_t ie = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess)
+    return perm::fail(PERM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  return PERM_OK;
+}
+
 // Plan items, upload, capture the launch graph.
 int build(perm_batch* B) {
   HIP_TRYB(hipSetDevice(B->device));
@@ -134,6 +189,9 @@ int build(perm_batch* B) {
   for (size_t m = 0; m < nm; ++m) {
     Entry& e = B->ents[m];
     perm::batch_split(e.n, B->item_log2, e.walk_bits, e.high_bits);
+    e.first_item = 0;
+    e.item_count = 0;
+    if (e.zero) continue;  // no walk: its result is exactly 0
     auto it = std::find(var_ids.begin(), var_ids.end(), e.variant);
     if (it == var_ids.end()) {
       var_ids.push_back(e.variant);
@@ -200,41 +258,54 @@ int build(perm_batch* B) {
   HIP_TRYB(hipMemcpyAsync(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
                           hipMemcpyHostToDevice, s0));
   HIP_TRYB(hipStreamSynchronize(s0));
-  // capture: reset -> fork -> variant kernels over streams -> join -> reduce ->
-  // result copy.  Groups come in decreasing work; each goes to the stream with
-  // the least work so far (longest-processing-time first), so the streams end
-  // close together instead of one long tail of round-robin leftovers.
+  const int st = capture(B);
+  if (st != PERM_OK) return st;
+  B->built = true;
+  return perm::ok();
+}
+
+// One escalation round: for each (matrix, mode) a persistent kernel of that
+// mode's variant over the matrix's items (double-double X runs the plain
+// walk), spread over the streams; then the per-matrix reduction and the
+// results of those matrices refilled in the new mode.  pass_ms: device time.
+int escalate(perm_batch* B, const std::vector<std::pair<size_t, int>>& esc,
+             perm_result* results, float& pass_ms) {
+  hipStream_t s0 = B->s[0];
+  uint32_t* ctr = B->d_counters + B->groups.size();
   const int ns = std::max(1, std::min(B->streams, kMaxStreams));
-  std::vector<double> stream_work(ns, 0.0);
-  hipGraph_t g;
-  HIP_TRYB(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)B->groups.size(), s0));
+  HIP_TRYB(hipEventRecord(B->ev0, s0));
+  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)esc.size(), s0));
   HIP_TRYB(hipEventRecord(B->ev_fork, s0));
   for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
-  for (size_t gi = 0; gi < B->groups.size(); ++gi) {
-    const auto& G = B->groups[gi];
-    const uint32_t waves = G.end - G.begin;
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, B->max_grid));
-    const int si = (int)(std::min_element(stream_work.begin(), stream_work.end()) -
-                         stream_work.begin());
-    stream_work[si] += G.work;
-    HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
-                                        B->d_counters + gi, B->d_partials, grid, B->s[si]));
+  for (size_t i = 0; i < esc.size(); ++i) {
+    const Entry& e = B->ents[esc[i].first];
+    const int md = esc[i].second;
+    const perm::NestShape sh = md == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+    const int vid = perm::batch_variant(e.n, sh, md);
+    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel of this mode for this n");
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
+    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
+                                        e.first_item + e.item_count, ctr + i, B->d_partials, grid,
+                                        B->s[i % ns]));
   }
   for (int i = 1; i < ns; ++i) {
     HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
     HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
   }
-  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, (uint32_t)nm, s0));
+  const uint32_t nm = (uint32_t)B->ents.size();
+  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
   HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
                           s0));
-  HIP_TRYB(hipStreamEndCapture(s0, &g));
-  hipError_t ie = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (ie != hipSuccess)
-    return perm::fail(PERM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-  B->built = true;
-  return perm::ok();
+  HIP_TRYB(hipEventRecord(B->ev1, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  HIP_TRYB(hipEventElapsedTime(&pass_ms, B->ev0, B->ev1));
+  for (const auto& em : esc) {
+    const Entry& e = B->ents[em.first];
+    const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
+                           B->h_results[em.first], results[em.first]);
+  }
+  return PERM_OK;
 }
 
 }  // namespace
@@ -253,9 +324,8 @@ void fill_item_result(int n, int walk_bits, bool integral, double product_bits, 
   o.abs_sum = std::fabs(scale) * r.abs_sum;
   o.err_bound =
       error_statement(o.abs_sum, o.value, n, walk_bits, dense_unroll_bits(n), integral, mode);
-  o.err_est =
-      error_estimate(scale * scale * r.sq_sum, o.value, n, walk_bits, high_bits, mode, integral);
-  o.err_est = std::min(o.err_est, o.err_bound);
+  o.err_est = error_estimate(scale * scale * r.sq_sum, scale * scale * r.sq_run, o.value, n,
+                             walk_bits, high_bits, mode, integral);
   o.subsets = std::ldexp(1.0, n - 1);
   o.integral = integral ? 1 : 0;
   o.nu = shape.k0;
@@ -266,6 +336,7 @@ void fill_item_result(int n, int walk_bits, bool integral, double product_bits, 
     o.err_bound = o.err_est = 0.0;  // exact arithmetic throughout
   round_integral(integral, o.value, o.err_bound, o.err_est);
   o.status = PERM_OK;
+  o.certified = is_certified(o) ? 1 : 0;
 }
 
 bool needs_dd_pass(const perm_result& o) {
@@ -346,55 +417,37 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   HIP_TRYB(hipStreamSynchronize(B->s[0]));
   float ms = 0.f;
   HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
-  if (device_ms) *device_ms = ms;
-  std::vector<size_t> dd;
   for (size_t m = 0; m < B->ents.size(); ++m) {
     const Entry& e = B->ents[m];
+    if (e.zero) {
+      perm::fill_zero_result(e.n, e.integral, results[m]);
+      continue;
+    }
     perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 0,
                            B->h_results[m], results[m]);
     results[m].kernel_ms = ms;
-    if (perm::needs_dd_pass(results[m])) dd.push_back(m);
   }
-  if (dd.empty()) return perm::ok();
-  // Integer matrices whose FP64 error could reach 1/2: a double-double
-  // product pass over the same items with the double-double variant (one
-  // persistent kernel per matrix, on the batch's streams), then the
-  // per-matrix reduction again.
-  hipStream_t s0 = B->s[0];
-  uint32_t* ctr = B->d_counters + B->groups.size();
-  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
-  HIP_TRYB(hipEventRecord(B->ev0, s0));
-  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)dd.size(), s0));
-  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
-  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
-  for (size_t i = 0; i < dd.size(); ++i) {
-    const Entry& e = B->ents[dd[i]];
-    const int vid = perm::batch_variant(e.n, e.shape, 1);
-    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
-    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
-                                        e.first_item + e.item_count, ctr + i, B->d_partials, grid,
-                                        B->s[i % ns]));
+  double total_ms = ms;
+  // Results that miss the accuracy target (perm::escalation_mode) get a pass
+  // in the more accurate mode over the same items: double-double products
+  // for integer matrices (same walk shape), double-double X for the others
+  // (plain walk); one persistent kernel per matrix on the batch's streams,
+  // then the per-matrix reduction again.  At most two rounds: the mode only
+  // increases.
+  for (int round = 0; round < 2; ++round) {
+    std::vector<std::pair<size_t, int>> esc;
+    for (size_t m = 0; m < B->ents.size(); ++m) {
+      if (B->ents[m].zero) continue;
+      const int md = perm::escalation_mode(results[m]);
+      if (md >= 0) esc.push_back({m, md});
+    }
+    if (esc.empty()) break;
+    float pass_ms = 0.f;
+    if (int s = escalate(B, esc, results, pass_ms)) return s;
+    total_ms += pass_ms;
   }
-  for (int i = 1; i < ns; ++i) {
-    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
-    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
-  }
-  const uint32_t nm = (uint32_t)B->ents.size();
-  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
-  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
-                          s0));
-  HIP_TRYB(hipEventRecord(B->ev1, s0));
-  HIP_TRYB(hipStreamSynchronize(s0));
-  float dd_ms = 0.f;
-  HIP_TRYB(hipEventElapsedTime(&dd_ms, B->ev0, B->ev1));
-  if (device_ms) *device_ms = ms + dd_ms;
-  for (size_t m : dd) {
-    const Entry& e = B->ents[m];
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 1,
-                           B->h_results[m], results[m]);
-    results[m].kernel_ms = ms + dd_ms;
-  }
+  if (device_ms) *device_ms = total_ms;
+  for (size_t m = 0; m < B->ents.size(); ++m) results[m].kernel_ms = total_ms;
   return perm::ok();
 }
 

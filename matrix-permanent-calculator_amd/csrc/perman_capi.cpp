@@ -67,6 +67,7 @@ struct perm_plan {
   int nu1 = 0, nu2 = 0;   // rows touched by Gray columns 0-1 and 0-2 (n: not nested)
   bool integral = false;  // every entry an integer of magnitude < 2^31
   double product_bits = 1e9;  // integer matrices: log2 bound of the walk's product numerators
+  bool zero = false;      // no perfect matching: the permanent is exactly 0
   bool borrowed = false;  // stream/events/memory belong to the device cache
 };
 
@@ -135,6 +136,7 @@ void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
 // The FP64 kernel's walk shape from the host preparation (0: not restricted).
 void set_shape(perm_plan* P, int n, const Prepared& prep) {
   P->product_bits = prep.product_bits;
+  P->zero = prep.zero;
   P->nu = prep.nu > 0 ? prep.nu : n;
   P->nu1 = prep.nu1 > 0 ? prep.nu1 : n;
   P->nu2 = prep.nu2 > 0 ? prep.nu2 : n;
@@ -231,9 +233,9 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->err_bound = perm::error_statement(out->abs_sum, out->value, n, P->walk_bits,
                                          P->unroll_bits, P->integral, mode);
   out->subsets = std::ldexp(1.0, n - 1);
-  out->err_est = perm::error_estimate(scale * scale * r.sq_sum, out->value, n, P->walk_bits,
-                                      P->high_bits, mode, P->integral);
-  out->err_est = std::min(out->err_est, out->err_bound);
+  out->err_est = perm::error_estimate(scale * scale * r.sq_sum, scale * scale * r.sq_run,
+                                      out->value, n, P->walk_bits, P->high_bits, mode,
+                                      P->integral);
   if (perm::integer_sum_exact(P->integral, P->product_bits, out->abs_sum, scale, n))
     out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(P->integral, out->value, out->err_bound, out->err_est);
@@ -243,30 +245,43 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->nu1 = mode <= 1 ? P->nu1 : P->n;
   out->nu2 = mode <= 1 ? P->nu2 : P->n;
   out->integral = P->integral ? 1 : 0;
+  out->zero_pattern = 0;
+  out->certified = perm::is_certified(*out) ? 1 : 0;
 }
 
-// mode < 0 (automatic): FP64 pass; for an integer matrix whose result is
-// below 2^53 but whose stated error could reach 1/2 (so that rounding could
-// miss the exact integer), a second pass with double-double products.
-// mode 0/1/2 runs exactly that kernel mode.
+// mode < 0 (automatic): a matrix without a perfect matching is exactly 0
+// (no pass); otherwise the FP64 pass, then, while the result misses the
+// accuracy target (perm::escalation_mode), a pass in the more accurate mode:
+// double-double products for an integer matrix (exact X; below 2^53 the
+// result then rounds to the exact integer), double-double X for any other
+// matrix.  mode 0/1/2 runs exactly that kernel mode.  kernel_ms sums the
+// passes; main_ms is the main walk of the first pass.
 int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
   if (!P || !out) return fail(PERM_ERR_ARG, "null pointer");
   if (mode > 2) return fail(PERM_ERR_ARG, "unknown mode");
+  if (mode < 0 && P->zero) {
+    perm::fill_zero_result(P->n, P->integral, *out);
+    return ok();
+  }
   HIP_TRY(hipSetDevice(P->device));
   perm::Partial r;
-  float ms = 0.f, main_ms = 0.f;
+  float ms = 0.f, main_ms = 0.f, first_main_ms = 0.f;
   const int first = mode < 0 ? 0 : mode;
-  if (int s = run_pass(P, first, r, ms, main_ms)) return s;
+  if (int s = run_pass(P, first, r, ms, first_main_ms)) return s;
   fill_result(P, r, first, out);
   double total_ms = ms;
-  if (mode < 0 && perm::needs_dd_pass(*out)) {
-    if (int s = run_pass(P, 1, r, ms, main_ms)) return s;
-    fill_result(P, r, 1, out);
-    total_ms += ms;
+  const double first_subsets = (double)P->L.main.lanes * std::ldexp(1.0, P->L.main.walk_bits);
+  if (mode < 0) {
+    // at most two more passes: the mode strictly increases
+    for (int next = perm::escalation_mode(*out); next >= 0; next = perm::escalation_mode(*out)) {
+      if (int s = run_pass(P, next, r, ms, main_ms)) return s;
+      fill_result(P, r, next, out);
+      total_ms += ms;
+    }
   }
   out->kernel_ms = total_ms;
-  out->main_ms = main_ms;
-  out->main_subsets = (double)P->L.main.lanes * std::ldexp(1.0, P->L.main.walk_bits);
+  out->main_ms = first_main_ms;
+  out->main_subsets = first_subsets;
   return ok();
 }
 
@@ -318,6 +333,10 @@ int bind_oneshot(int n, const Prepared& prep, DeviceCache& C, perm_plan& P) {
 
 // One-shot run through the per-device cache (mode as in plan_run_impl).
 int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
+  if (mode < 0 && prep.zero) {  // exactly 0: nothing to upload or run
+    perm::fill_zero_result(n, prep.integral, *out);
+    return ok();
+  }
   if (int s = select_device()) return s;
   DeviceCache& C = device_cache(perm::current_device());
   std::lock_guard<std::mutex> guard(C.mu);
@@ -340,6 +359,11 @@ int run_oneshot(int n, const Prepared& prep, int mode, perm_result* out) {
 // with it, chunks the CPU took differ in rounding only.
 int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_result* out,
                 int* cpu_chunks_out) {
+  if (cpu_chunks_out) *cpu_chunks_out = 0;
+  if (prep.zero) {
+    perm::fill_zero_result(n, prep.integral, *out);
+    return ok();
+  }
   if (int s = select_device()) return s;
   DeviceCache& C = device_cache(perm::current_device());
   std::lock_guard<std::mutex> guard(C.mu);
@@ -355,7 +379,7 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
   const uint64_t n_chunks = mb == 0 ? 0 : (mb + chunk_blocks - 1) / chunk_blocks;
   const uint64_t gpu_take = std::max<uint64_t>(1, 1024 / chunk_blocks);  // ~2^18 lanes a launch
   std::atomic<uint64_t> next{0};
-  std::vector<std::array<double, 4>> cpu_part;
+  std::vector<std::array<double, 5>> cpu_part;
   std::vector<uint64_t> cpu_ids;
   std::thread cpu;
   if (use_cpu && n_chunks > 0) {
@@ -365,7 +389,7 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
         if (c >= n_chunks) break;
         const uint64_t l0 = c * chunk_blocks * 256;
         const uint64_t l1 = std::min<uint64_t>((c + 1) * chunk_blocks * 256, L.main.lanes);
-        std::array<double, 4> r;
+        std::array<double, 5> r;
         perm::cpu_lanes(prep.cs, prep.x0, n, L.main.walk_bits, L.main.high_bits, l0, l1,
                         threads, r.data());
         cpu_part.push_back(r);
@@ -388,15 +412,16 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
   if (e == hipSuccess) e = hipEventRecord(P.ev1, P.stream);
   // the CPU's chunks: its sum in the chunk's first record, zeros in the rest,
   // then the device reduction over all records, in the one-launch order
-  std::vector<perm::Partial> staged(cpu_ids.size() * chunk_blocks, perm::Partial{0.0, 0.0, 0.0, 0.0});
+  std::vector<perm::Partial> staged(cpu_ids.size() * chunk_blocks,
+                                    perm::Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
   for (size_t i = 0; i < cpu_ids.size() && e == hipSuccess; ++i) {
     const uint64_t b0 = cpu_ids[i] * chunk_blocks, b1 = std::min(mb, b0 + chunk_blocks);
     perm::Partial* h = staged.data() + i * chunk_blocks;
-    h[0] = {cpu_part[i][0], cpu_part[i][1], cpu_part[i][2], cpu_part[i][3]};
+    h[0] = {cpu_part[i][0], cpu_part[i][1], cpu_part[i][2], cpu_part[i][3], cpu_part[i][4], 0.0};
     e = hipMemcpyAsync(L.partials + b0, h, (b1 - b0) * sizeof(perm::Partial),
                        hipMemcpyHostToDevice, P.stream);
   }
-  perm::Partial r{0.0, 0.0, 0.0, 0.0};
+  perm::Partial r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (e == hipSuccess) e = perm::launch_reduce(L.partials, L.total_blocks, L.result, P.stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, P.stream);
@@ -408,7 +433,19 @@ int run_chunked(int n, const Prepared& prep, bool use_cpu, int threads, perm_res
   (void)hipEventElapsedTime(&ms, P.ev0, P.ev1);
   out->kernel_ms = ms;
   if (cpu_chunks_out) *cpu_chunks_out = (int)cpu_ids.size();
-  if (perm::needs_dd_pass(*out)) return plan_run_impl(&P, 1, out);
+  // a result that misses the accuracy target is computed again in the more
+  // accurate mode (one launch), as plan_run_impl does
+  const int esc = perm::escalation_mode(*out);
+  if (esc >= 0) {
+    const double chunk_ms = out->kernel_ms;
+    if (int s = plan_run_impl(&P, esc, out)) return s;
+    for (int m = perm::escalation_mode(*out); m >= 0; m = perm::escalation_mode(*out)) {
+      const double prev = out->kernel_ms;
+      if (int s = plan_run_impl(&P, m, out)) return s;
+      out->kernel_ms += prev;
+    }
+    out->kernel_ms += chunk_ms;
+  }
   return ok();
 }
 

@@ -216,6 +216,70 @@ void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector
   nu2 = best.k2;
 }
 
+// Kuhn's augmenting path from row `row` over columns not yet visited.
+static bool augment(int row, const uint64_t* rowsup, uint64_t& visited, int* col_row) {
+  uint64_t cand = rowsup[row] & ~visited;
+  while (cand) {
+    const int c = __builtin_ctzll(cand);
+    cand &= cand - 1;
+    visited |= 1ull << c;
+    if (col_row[c] < 0 || augment(col_row[c], rowsup, visited, col_row)) {
+      col_row[c] = row;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool has_perfect_matching(int n, const uint64_t* rowsup) {
+  int col_row[64];
+  for (int c = 0; c < n; ++c) col_row[c] = -1;
+  for (int r = 0; r < n; ++r) {
+    uint64_t visited = 0;
+    if (!augment(r, rowsup, visited, col_row)) return false;
+  }
+  return true;
+}
+
+void fill_zero_result(int n, bool integral, perm_result& o) {
+  std::memset(&o, 0, sizeof(o));
+  o.n = n;
+  o.value = 0.0;
+  o.subsets = std::ldexp(1.0, n - 1);
+  o.status = PERM_OK;
+  o.mode = 0;
+  o.integral = integral ? 1 : 0;
+  o.nu = o.nu1 = o.nu2 = n;
+  o.certified = 1;
+  o.zero_pattern = 1;
+}
+
+// Nonzero pattern of a dense row-major matrix (rowsup[j] bit k: a[j][k] != 0).
+static bool pattern_has_matching(int n, const std::vector<double>& a) {
+  uint64_t rowsup[64];
+  for (int j = 0; j < n; ++j) {
+    rowsup[j] = 0;
+    for (int k = 0; k < n; ++k)
+      if (a[(size_t)j * n + k] != 0.0) rowsup[j] |= 1ull << k;
+  }
+  return has_perfect_matching(n, rowsup);
+}
+
+bool is_certified(const perm_result& o) {
+  if (o.zero_pattern) return true;
+  if (o.err_bound == 0.0 && o.err_est == 0.0) return true;  // exact arithmetic
+  return o.value != 0.0 && o.err_est <= kTargetRel * std::fabs(o.value);
+}
+
+int escalation_mode(const perm_result& o) {
+  if (o.zero_pattern) return -1;
+  if (o.mode == 0 && o.integral && needs_dd_pass(o)) return 1;
+  if (is_certified(o)) return -1;
+  if (o.mode == 0) return o.integral ? 1 : 2;
+  if (o.mode == 1 && !o.integral) return 2;
+  return -1;
+}
+
 bool integral_matrix(const std::vector<double>& a) {
   for (double v : a)
     if (!(std::fabs(v) < 2147483648.0) || v != std::nearbyint(v)) return false;
@@ -234,6 +298,7 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p, const NestMenu
   std::vector<double> a;
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
   p.integral = integral_matrix(a);
+  p.zero = !pattern_has_matching(n, a);
   p.x0.assign(n, 0.0);
   perm::nw_start_vector(a.data(), n, p.x0.data());
   std::vector<double> cols((size_t)(n - 1) * n);
@@ -275,8 +340,23 @@ int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows
                    const void* cvals, int n, Prepared& p, const NestMenu& menu) {
   std::vector<double> cols;
   if (int s = gather_sparse(dtype, mat, cptrs, rows, cvals, n, p.x0, cols, p.integral)) return s;
+  p.zero = sparse_pattern_zero(dtype, mat, cols, n);
   finish_prepare(n, menu, cols, p);
   return PERM_OK;
+}
+
+// The sparse sum is the permanent of the dense matrix when the CCS lists
+// exactly its nonzero entries of the walked columns (every nonzero entry
+// positive, as the reference's CCS keeps entries > 0 only); then the
+// matching check of the dense pattern applies.  Otherwise (entries <= 0
+// dropped by the CCS) no shortcut is taken.
+bool sparse_pattern_zero(int dtype, const void* mat, const std::vector<double>& cols, int n) {
+  std::vector<double> a;
+  if (!to_double(dtype, mat, (size_t)n * n, a)) return false;
+  for (int k = 0; k < n - 1; ++k)
+    for (int j = 0; j < n; ++j)
+      if (cols[(size_t)k * n + j] != a[(size_t)j * n + k]) return false;
+  return !pattern_has_matching(n, a);
 }
 
 }  // namespace perm

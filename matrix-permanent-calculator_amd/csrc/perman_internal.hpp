@@ -8,6 +8,8 @@
 
 #include "nest_shape.hpp"
 
+struct perm_result;
+
 namespace perm {
 
 // Per-thread status (perm_last_status / perm_last_error) and device.
@@ -31,7 +33,21 @@ struct Prepared {
   int nu = 0, nu1 = 0, nu2 = 0;
   bool integral = false;
   double product_bits = 1e9;  // integer_product_bits (integer matrices)
+  // no perfect matching in the matrix's nonzero pattern: every term of the
+  // permanent vanishes, the result is exactly 0 and no walk is run
+  bool zero = false;
 };
+
+// Perfect matching in the bipartite graph of a 0/1 pattern: rowsup[j] has
+// bit k set when entry (j, k) is nonzero (n <= 64).  Kuhn's augmenting
+// paths, O(n^3) at most.
+bool has_perfect_matching(int n, const uint64_t* rowsup);
+// Sparse inputs (cols as gather_sparse returns them): true when the CCS
+// lists exactly the dense matrix's entries and its pattern has no perfect
+// matching, so the sparse sum is exactly 0.
+bool sparse_pattern_zero(int dtype, const void* mat, const std::vector<double>& cols, int n);
+// The result of a permanent whose pattern has no perfect matching: exactly 0.
+void fill_zero_result(int n, bool integral, perm_result& out);
 // menu: the walk shapes the kernels that will run the matrix are compiled
 // for (kMenuSingle: single-matrix paths; kMenuBatch: batch and server).
 int prepare_dense(int dtype, const void* mat, int n, Prepared& p,
@@ -65,12 +81,20 @@ void fill_item_result(int n, int walk_bits, bool integral, double product_bits, 
 // An integer matrix whose FP64 result could be off by 1/2 or more while
 // below 2^53: run again with double-double products (mode 1).
 bool needs_dd_pass(const perm_result& o);
+// A result meets the accuracy target: exact (stated error 0), or its
+// estimated error is at most kTargetRel |value|.
+bool is_certified(const perm_result& o);
+// The mode to run next for a result computed in o.mode, or -1 if none:
+// integer matrices go to double-double products (mode 1: exact X, exact
+// integer rounding below 2^53), other matrices whose estimated error
+// exceeds kTargetRel |value| to double-double X (mode 2: no X drift).
+int escalation_mode(const perm_result& o);
 
 // Lanes [l0, l1) of the GPU walk computed on the host (the hybrid chunk
-// scheduler's CPU worker); out = {hi, lo, sum |term|, sq_sum}.
+// scheduler's CPU worker); out = {hi, lo, sum |term|, sq_sum, sq_run}.
 void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int n,
                int walk_bits, int high_bits, uint64_t l0, uint64_t l1, int threads,
-               double out[4]);
+               double out[5]);
 
 // Split of an n-matrix into 64-lane batch items of about 2^item_log2
 // subsets (walk_bits + high_bits = n - 1); depends on n and item_log2 only.

@@ -19,11 +19,12 @@
 //    all done is summed on the host in item order (double-double) -- so its
 //    result depends only on the matrix, bitwise the same alone or in any mix
 //    and whatever iterations its items ran in;
-//  * integer matrices whose FP64 error bound reaches 1/2 get a
-//    double-double product pass: the request goes back to the active list
-//    with the batch kernels' double-double variant, its matrix still in its
-//    arena slot
-//    thread, so the scheduler never blocks on it.
+//  * a result that misses the accuracy target goes back to the active list
+//    for a pass in a double-double mode (integer matrices: double-double
+//    products, exact below 2^53; others: double-double X), its matrix still
+//    in its arena slot;
+//  * a matrix whose nonzero pattern has no perfect matching is exactly 0 and
+//    finishes at submission.
 //
 // Iterations change every time, so they are launched directly (no graph).
 #include <hip/hip_runtime.h>
@@ -70,7 +71,7 @@ constexpr size_t kSlotDoubles =
 struct Request {
   uint64_t id = 0;
   int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
-  int mode = 0;  // 0 FP64 items; 1 the double-double pass of an integer matrix
+  int mode = 0;  // 0 FP64 items; 1, 2 a pass in a double-double mode (escalation)
   perm::NestShape shape{0, 0, 0};
   bool integral = false;
   perm::Prepared prep;
@@ -170,36 +171,45 @@ void publish_all(perm_server* S, const std::vector<Request*>& rs) {
   S->cv_done.notify_all();
 }
 
-// FP64 VALU lane-instructions per subset of a double-double product pass
-// over walk shape s: the FP64 adds, three instructions per TwoProd multiply,
-// about 8 for the TwoSum accumulation (scheduling estimate only).
-double dd_subset_cost(int n, perm::NestShape s) {
+// FP64 VALU lane-instructions per subset of a pass in a double-double mode
+// (scheduling estimate only).  Mode 1 over walk shape s: the FP64 adds,
+// three instructions per TwoProd multiply, about 8 for the TwoSum
+// accumulation.  Mode 2 (plain walk): a TwoSum per column addition (7), a
+// double-double multiply step per factor (4), the accumulation (8).
+double dd_subset_cost(int n, perm::NestShape s, int mode) {
+  if (mode == 2) return 11.0 * n + 8.0;
   const double half = (perm::nest_cost(n, s) / 8.0 - 2.0) / 2.0;  // adds = multiplies
   return half + 3.0 * half + 8.0;
 }
 
 // All items of r are done: sum its partials in item order and state the
-// error.  An integer matrix whose FP64 error could reach 1/2 goes back to the
-// active list for a double-double product pass over the same slot (returns
-// false); otherwise the slot is freed and r is ready to publish (true).
+// error.  A result that misses the accuracy target (perm::escalation_mode)
+// goes back to the active list for a pass in the more accurate mode over the
+// same slot -- double-double products (same walk shape) for an integer
+// matrix, double-double X (plain walk) for any other -- and this returns
+// false; otherwise the slot is freed and r is ready to publish (true).
 bool finalize(perm_server* S, Request* r) {
   perm::HostDD s = {0.0, 0.0};
-  Partial tot{0.0, 0.0, 0.0, 0.0};
+  Partial tot{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (const Partial& p : r->parts) {
     s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
     tot.abs_sum += p.abs_sum;
     tot.sq_sum += p.sq_sum;
+    tot.sq_run += p.sq_run;
   }
   tot.hi = s.hi;
   tot.lo = s.lo;
   perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
                          r->mode, tot, r->result);
-  if (r->mode == 0 && perm::needs_dd_pass(r->result)) {
-    r->mode = 1;  // same walk shape, double-double products
-    r->variant = perm::batch_variant(r->n, r->shape, 1);
+  const int next = perm::escalation_mode(r->result);
+  if (next >= 0) {
+    r->mode = next;
+    if (next == 2) r->shape = perm::NestShape{r->n, r->n, r->n};
+    r->variant = perm::batch_variant(r->n, r->shape, next);
     r->issued = r->done = 0;
-    std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0});
-    r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape);
+    std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+    r->item_cost =
+        std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape, next);
     S->queued_cost += (double)r->items * r->item_cost;
     S->unissued_cost += (double)r->items * r->item_cost;
     S->active.push_back(r);
@@ -271,7 +281,7 @@ int admit(perm_server* S) {
   for (Request* r : fresh) {
     r->slot = S->free_slots.top();
     S->free_slots.pop();
-    r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0});
+    r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
     S->queued_cost += (double)r->items * r->item_cost;
     S->unissued_cost += (double)r->items * r->item_cost;
   }
@@ -512,12 +522,30 @@ int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
   }
   r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
                  (double)perm::nest_cost(n, r->shape) / 8.0;
-  if (perm::batch_variant(n, r->shape, 1) < 0) {
+  if (perm::batch_variant(n, r->shape, 1) < 0 || perm::batch_variant(n, {n, n, n}, 2) < 0) {
     delete r;
     return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
   }
+  const bool zero = p.zero;
   r->prep = std::move(p);
   r->t_submit = Clock::now();
+  if (zero) {  // no perfect matching: exactly 0, finished at once
+    perm::fill_zero_result(n, r->integral, r->result);
+    r->t_done = r->t_submit;
+    r->finished = true;
+    r->prep = perm::Prepared();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->fatal != PERM_OK) {
+      delete r;
+      return perm::fail(S->fatal, "server failed: " + S->fatal_msg);
+    }
+    r->id = S->next_id++;
+    S->table[r->id] = r;
+    S->stats.submitted += 1;
+    S->stats.completed += 1;
+    *id = r->id;
+    return perm::ok();
+  }
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->fatal != PERM_OK) {
@@ -550,8 +578,31 @@ int collect(perm_server* S, std::unordered_map<uint64_t, Request*>::iterator it,
   const std::string err = r->error;
   S->table.erase(it);
   delete r;
+  S->cv_done.notify_all();  // other waiters on this ticket see it gone
   if (st != PERM_OK) return perm::fail(st, err);
   return perm::ok();
+}
+
+// Wait (mu held through lk) until request `id` is finished and collect it.
+// The ticket is looked up again after every wake-up: another thread may
+// have collected it (wait, poll or wait_many on the same id), and then this
+// call returns PERM_ERR_ARG with out->status = PERM_ERR_ARG instead of
+// touching the freed request.
+int wait_collect(perm_server* S, std::unique_lock<std::mutex>& lk, uint64_t id, perm_result* out,
+                 double* latency_ms) {
+  S->cv_done.wait(lk, [&] {
+    const auto it = S->table.find(id);
+    return it == S->table.end() || it->second->finished;
+  });
+  const auto it = S->table.find(id);
+  if (it == S->table.end()) {
+    if (out) {
+      std::memset(out, 0, sizeof(perm_result));
+      out->status = PERM_ERR_ARG;
+    }
+    return perm::fail(PERM_ERR_ARG, "request collected by another caller");
+  }
+  return collect(S, it, out, latency_ms);
 }
 
 void release(perm_server* S) {
@@ -682,11 +733,9 @@ int perm_server_wait(perm_server* S, unsigned long long id, perm_result* out,
                      double* latency_ms) {
   if (!S) return perm::fail(PERM_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(S->mu);
-  auto it = S->table.find(id);
-  if (it == S->table.end()) return perm::fail(PERM_ERR_ARG, "unknown request id");
-  Request* r = it->second;
-  S->cv_done.wait(lk, [&] { return r->finished; });
-  return collect(S, S->table.find(id), out, latency_ms);
+  if (S->table.find(id) == S->table.end())
+    return perm::fail(PERM_ERR_ARG, "unknown request id");
+  return wait_collect(S, lk, id, out, latency_ms);
 }
 
 int perm_server_wait_many(perm_server* S, const unsigned long long* ids, int count,
@@ -705,9 +754,8 @@ int perm_server_wait_many(perm_server* S, const unsigned long long* ids, int cou
       if (first_error == PERM_OK) first_error = perm::fail(PERM_ERR_ARG, "request id listed twice");
       continue;
     }
-    Request* r = it->second;
-    S->cv_done.wait(lk, [&] { return r->finished; });
-    const int st = collect(S, S->table.find(ids[i]), out + i, latency_ms ? latency_ms + i : nullptr);
+    // (a ticket collected meanwhile by another caller comes back as PERM_ERR_ARG)
+    const int st = wait_collect(S, lk, ids[i], out + i, latency_ms ? latency_ms + i : nullptr);
     if (st != PERM_OK && first_error == PERM_OK) first_error = st;
   }
   return first_error == PERM_OK ? perm::ok() : first_error;

@@ -175,9 +175,8 @@ void fill(int n, const SkipPrepared& sp, const perm::Partial& r, int mode, perm_
   out->abs_sum = std::fabs(scale) * r.abs_sum;
   out->err_bound = perm::error_statement(out->abs_sum, out->value, n, sp.walk_bits,
                                          perm::dense_unroll_bits(n), true, mode);
-  out->err_est = std::min(out->err_bound,
-                          perm::error_estimate(scale * scale * r.sq_sum, out->value, n,
-                                               sp.walk_bits, sp.high_bits, mode, true));
+  out->err_est = perm::error_estimate(scale * scale * r.sq_sum, 0.0, out->value, n, sp.walk_bits,
+                                      sp.high_bits, mode, true);
   out->subsets = std::ldexp(1.0, n - 1);
   out->mode = mode;
   out->integral = 1;
@@ -186,6 +185,7 @@ void fill(int n, const SkipPrepared& sp, const perm::Partial& r, int mode, perm_
   if (perm::integer_sum_exact(true, sp.product_bits, out->abs_sum, scale, n))
     out->err_bound = out->err_est = 0.0;  // exact arithmetic throughout
   perm::round_integral(true, out->value, out->err_bound, out->err_est);
+  out->certified = perm::is_certified(*out) ? 1 : 0;
 }
 
 }  // namespace
@@ -202,6 +202,10 @@ int perm_skip(int dtype, const void* mat, const int* rptrs, const int* cols, con
     return s;
   if (!integral || nov < perm::kMinSkipN)
     return perm_sparse(dtype, mat, cptrs, rows, cvals, nov, 4, out);
+  if (perm::sparse_pattern_zero(dtype, mat, cc, nov)) {  // exactly 0: nothing to walk
+    perm::fill_zero_result(nov, true, *out);
+    return perm::ok();
+  }
   if (int s = perm::select_device()) return s;
   SkipPrepared sp;
   const int R = perm::dense_unroll_bits(nov);
@@ -220,7 +224,9 @@ int perm_skip(int dtype, const void* mat, const int* rptrs, const int* cols, con
   float ms = 0.f, ms2 = 0.f;
   if (int s = run_skip(nov, sp, 0, check_bits, r, walked, ms)) return s;
   fill(nov, sp, r, 0, out);
-  if (perm::needs_dd_pass(*out)) {
+  // integer matrix: X is exact, so the only escalation is double-double
+  // products (exact rounding below 2^53, error ~2^-104 A above)
+  if (perm::escalation_mode(*out) == 1) {
     double walked2 = 0.0;
     if (int s = run_skip(nov, sp, 1, check_bits, r, walked2, ms2)) return s;
     fill(nov, sp, r, 1, out);

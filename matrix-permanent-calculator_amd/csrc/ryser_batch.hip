@@ -91,15 +91,16 @@ __global__ __launch_bounds__(64) void batch_reduce_kernel(const BatchMatrix* __r
                                                           Partial* __restrict__ results) {
   const BatchMatrix m = mats[blockIdx.x];
   DD s = {0.0, 0.0};
-  double a = 0.0, q = 0.0;
+  Moments mo = {0.0, 0.0, 0.0};
   for (uint32_t i = threadIdx.x; i < m.item_count; i += kWave) {
     const Partial p = partials[m.first_item + i];
     s = dd_add(s, DD{p.hi, p.lo});
-    a += p.abs_sum;
-    q += p.sq_sum;
+    mo.a += p.abs_sum;
+    mo.q += p.sq_sum;
+    mo.r += p.sq_run;
   }
-  wave_reduce(s, a, q);
-  if (threadIdx.x == 0) results[blockIdx.x] = {s.hi, s.lo, a, q};
+  wave_reduce(s, mo);
+  if (threadIdx.x == 0) results[blockIdx.x] = make_partial(s, mo);
 }
 
 hipError_t launch_batch_reduce(const BatchMatrix* mats, const Partial* partials, Partial* results,

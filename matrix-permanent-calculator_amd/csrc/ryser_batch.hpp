@@ -35,8 +35,8 @@ struct BatchItem {
 };
 
 // Variant id of walk shape s and kernel mode (0 FP64, 1 double-double
-// products; mode 1 only for the plain shape) for order n (-1 if none), and
-// the cheapest FP64 variant shape covering `want` ((n, n, n) if none).
+// products, 2 double-double X: plain shape only) for order n (-1 if none),
+// and the cheapest FP64 variant shape covering `want` ((n, n, n) if none).
 int batch_variant(int n, NestShape s, int mode = 0);
 NestShape batch_shape(int n, NestShape want);
 

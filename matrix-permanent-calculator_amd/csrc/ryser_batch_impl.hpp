@@ -10,18 +10,21 @@
 namespace perm {
 
 // A batch kernel variant: order n, walk shape (k0, k1, k2) and kernel mode
-// (kFast, or kDDProd for the exact pass of integer matrices).
+// (kFast; kDDProd for the exact pass of integer matrices; kDDAll for the
+// drift-free pass of other matrices whose FP64 result misses the accuracy
+// target).
 struct Variant {
   int n, k0, k1, k2, mode;
 };
 
-// Variants: for every n <= kMaxBatchN the plain walk, then the kMenuBatch
-// shapes (every pair width; for n >= kMinNestN also the two three-level
-// shapes per width), each in both modes.
+// Variants: for every n <= kMaxBatchN the plain walk in all three modes,
+// then the kMenuBatch shapes (every pair width; for n >= kMinNestN also the
+// two three-level shapes per width) in the FP64 and double-double product
+// modes (double-double X runs the plain walk only).
 constexpr int count_variants() {
   int c = 0;
   for (int n = 1; n <= kMaxBatchN; ++n) {
-    c += 2;
+    c += 3;
     for (int k0 = 4; k0 <= n - 4; k0 += 4)
       for (int j = 0; j < kNestShapesPerK0; ++j)
         c += nest_menu_has(kMenuBatch, n, k0, j) ? 2 : 0;
@@ -36,6 +39,7 @@ constexpr std::array<Variant, kNumVariants> make_variants() {
   for (int n = 1; n <= kMaxBatchN; ++n) {
     v[c++] = {n, n, n, n, kFast};
     v[c++] = {n, n, n, n, kDDProd};
+    v[c++] = {n, n, n, n, kDDAll};
     for (int k0 = 4; k0 <= n - 4; k0 += 4)
       for (int j = 0; j < kNestShapesPerK0; ++j)
         if (nest_menu_has(kMenuBatch, n, k0, j)) {
@@ -70,16 +74,14 @@ void ryser_batch_variant_kernel(
     const BatchMatrix m = mats[d.matrix];
     const uint64_t lane = d.lane_base + (threadIdx.x & (kWave - 1));
     DD s;
-    double a, q;
-    lane_walk<N, R, MODE, Nest<K0, K1, K2>>(m.colpm, m.x0, lane, m.walk_bits, m.high_bits, s, a,
-                                            q);
+    Moments mo;
+    lane_walk<N, R, MODE, Nest<K0, K1, K2>>(m.colpm, m.x0, lane, m.walk_bits, m.high_bits, s, mo);
     if (lane >= m.lanes) {
       s = {0.0, 0.0};
-      a = 0.0;
-      q = 0.0;
+      mo = {0.0, 0.0, 0.0};
     }
-    wave_reduce(s, a, q);
-    if ((threadIdx.x & (kWave - 1)) == 0) partials[it] = {s.hi, s.lo, a, q};
+    wave_reduce(s, mo);
+    if ((threadIdx.x & (kWave - 1)) == 0) partials[it] = make_partial(s, mo);
   }
 }
 

@@ -24,13 +24,14 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const Partial* __r
                                                               Partial* __restrict__ out) {
   __shared__ Partial scratch[256 / kWave];
   DD s = {0.0, 0.0};
-  double a = 0.0, q = 0.0;
+  Moments m = {0.0, 0.0, 0.0};
   for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
     s = dd_add(s, DD{in[i].hi, in[i].lo});
-    a += in[i].abs_sum;
-    q += in[i].sq_sum;
+    m.a += in[i].abs_sum;
+    m.q += in[i].sq_sum;
+    m.r += in[i].sq_run;
   }
-  block_reduce_store(s, a, q, scratch, out);
+  block_reduce_store(s, m, scratch, out);
 }
 
 int dense_unroll_bits(int n) { return n - 1 < 3 ? n - 1 : 3; }

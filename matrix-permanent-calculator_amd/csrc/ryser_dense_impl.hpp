@@ -312,12 +312,13 @@ __device__ __forceinline__ void group_steps(double (&x)[N], double (&xl)[N], dou
 }
 
 // The Gray walk of one lane over its groups.  Every 32 groups (a run of
-// 2^(R+5) terms) the FP64 lane sum is folded into the double-double s and
-// the square of the run's sum of |terms| is added to q.
+// 2^(R+5) terms) the FP64 lane sum is folded into the double-double s, the
+// square of the run's sum of |terms| is added to q and (FP64 mode) the
+// square of its signed sum to r.
 template <int N, int R, int MODE, class S>
 __device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], ColPtr cols,
                                             uint32_t groups, int zero_blk, DD& s, LaneAcc& acc,
-                                            double& q) {
+                                            double& q, double& r) {
   Tails tl;
   double a_run = 0.0;  // acc.a at the start of the current run
   double cv[N];
@@ -327,6 +328,7 @@ __device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], Col
     if ((g & 31u) == 31u) {
       if constexpr (MODE == kFast) {  // fold into double-double
         s = dd_add_d(s, acc.p);
+        r = __builtin_fma(acc.p, acc.p, r);
         acc.p = 0.0;
       }
       const double d = acc.a - a_run;
@@ -359,16 +361,16 @@ __device__ __forceinline__ void wave_min_max(int v, int& mn, int& mx) {
 
 // The walk of one lane: closed-form start from its high columns, then 2^w
 // subsets of its low columns.  Returns the lane's signed double-double sum
-// (sign of the high pattern applied), its sum of |terms| and its sum of
-// squared run sums (see walk_groups).
+// (sign of the high pattern applied) and its moments: sum of |terms|, sums
+// over runs of the squared absolute and signed run sums (see walk_groups).
 // colpm: 2(n-1)+1 blocks of N doubles: block 2k = +a_k, 2k+1 = -a_k for the
 // Gray columns k < n-1, block 2(n-1) = zeros.  Shape S: column l < 3 is zero
 // in rows [K_l, N).
 template <int N, int R, int MODE, class S>
 __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0, uint64_t pattern,
-                                          int n_walk_bits, int n_high_bits, DD& s, double& a,
-                                          double& q) {
+                                          int n_walk_bits, int n_high_bits, DD& s,
+                                          Moments& m) {
   static_assert(S::K0 <= S::K1 && S::K1 <= S::K2 && S::K2 <= N, "nested shape");
   static_assert(S::K0 == N || (MODE != kDDAll && R >= 1 && S::K0 >= 1),
                 "nested: FP64 or double-double products");
@@ -417,13 +419,17 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
   }
 
   s = {0.0, 0.0};
-  q = 0.0;
+  m = {0.0, 0.0, 0.0};
   LaneAcc acc;
   const uint32_t groups = 1u << (n_walk_bits - R);
-  walk_groups<N, R, MODE, S>(x, xl, cols, groups, zero_blk, s, acc, q);
-  if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
-  else s = two_sum(acc.p, acc.plo);
-  a = acc.a;
+  walk_groups<N, R, MODE, S>(x, xl, cols, groups, zero_blk, s, acc, m.q, m.r);
+  if constexpr (MODE == kFast) {  // the last (partial) run
+    s = dd_add_d(s, acc.p);
+    m.r = __builtin_fma(acc.p, acc.p, m.r);
+  } else {
+    s = two_sum(acc.p, acc.plo);
+  }
+  m.a = acc.a;
   if (parity) {
     s.hi = -s.hi;
     s.lo = -s.lo;
@@ -442,14 +448,13 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
   const uint64_t lane = lane_base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t pattern = lane < lanes ? lane : 0ull;  // high columns fixed by the lane id
   DD s;
-  double a, q;
-  lane_walk<N, R, MODE, S>(colpm, x0, pattern, n_walk_bits, n_high_bits, s, a, q);
+  Moments m;
+  lane_walk<N, R, MODE, S>(colpm, x0, pattern, n_walk_bits, n_high_bits, s, m);
   if (lane >= lanes) {
     s = {0.0, 0.0};
-    a = 0.0;
-    q = 0.0;
+    m = {0.0, 0.0, 0.0};
   }
-  block_reduce_store(s, a, q, scratch, partials + blockIdx.x);
+  block_reduce_store(s, m, scratch, partials + blockIdx.x);
 }
 
 template <int N>

@@ -15,11 +15,19 @@ namespace perm {
 
 constexpr int kWave = 64;
 
-// One workgroup's contribution: signed sum as hi+lo, sum of |terms|, and
-// sum over runs of consecutive terms of (sum of |terms| in the run)^2 -- an
-// upper bound on sum term^2 that sizes the statistical error estimate.
+// One workgroup's contribution: signed sum as hi+lo, sum of |terms|, and two
+// sums over runs of 2^(R+5) consecutive terms of one lane that size the
+// statistical error estimate: sq_sum = sum of (sum of |terms| in the run)^2
+// (an upper bound on sum term^2: the product roundings) and sq_run = sum of
+// (signed sum of the run)^2 (the X drift: it moves a run's terms together,
+// so its effect follows the run's cancelled sum).  48 bytes.
 struct Partial {
-  double hi, lo, abs_sum, sq_sum;
+  double hi, lo, abs_sum, sq_sum, sq_run, pad;
+};
+
+// The plain-summed statistics carried through every reduction.
+struct Moments {
+  double a, q, r;  // abs_sum, sq_sum, sq_run
 };
 
 struct DD {
@@ -71,15 +79,16 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// Wave-wide dd sum (and plain sums of `a` and `q`); the full result is
+// Wave-wide dd sum (and plain sums of the moments); the full result is
 // valid in lane 0.
-__device__ __forceinline__ void wave_reduce(DD& s, double& a, double& q) {
+__device__ __forceinline__ void wave_reduce(DD& s, Moments& m) {
 #define PERM_XOR_STEP(M)                                 \
   {                                                      \
     DD o = {swizzle_xor<M>(s.hi), swizzle_xor<M>(s.lo)}; \
     s = dd_add(s, o);                                    \
-    a += swizzle_xor<M>(a);                              \
-    q += swizzle_xor<M>(q);                              \
+    m.a += swizzle_xor<M>(m.a);                          \
+    m.q += swizzle_xor<M>(m.q);                          \
+    m.r += swizzle_xor<M>(m.r);                          \
   }
   PERM_XOR_STEP(1)
   PERM_XOR_STEP(2)
@@ -88,31 +97,36 @@ __device__ __forceinline__ void wave_reduce(DD& s, double& a, double& q) {
   PERM_XOR_STEP(16)
 #undef PERM_XOR_STEP
   DD up = {readlane_d(s.hi, 32), readlane_d(s.lo, 32)};
-  double aup = readlane_d(a, 32);
-  double qup = readlane_d(q, 32);
+  const Moments mu = {readlane_d(m.a, 32), readlane_d(m.q, 32), readlane_d(m.r, 32)};
   s = dd_add(s, up);
-  a += aup;
-  q += qup;
+  m.a += mu.a;
+  m.q += mu.q;
+  m.r += mu.r;
 }
 
-// Workgroup reduction of (s, a, q) into one Partial written by thread 0.
+__device__ __forceinline__ Partial make_partial(DD s, const Moments& m) {
+  return {s.hi, s.lo, m.a, m.q, m.r, 0.0};
+}
+
+// Workgroup reduction of (s, m) into one Partial written by thread 0.
 // `scratch` must hold blockDim.x/64 entries.
-__device__ __forceinline__ void block_reduce_store(DD s, double a, double q, Partial* scratch,
+__device__ __forceinline__ void block_reduce_store(DD s, Moments m, Partial* scratch,
                                                    Partial* out) {
-  wave_reduce(s, a, q);
+  wave_reduce(s, m);
   const int wid = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) scratch[wid] = {s.hi, s.lo, a, q};
+  if ((threadIdx.x & (kWave - 1)) == 0) scratch[wid] = make_partial(s, m);
   __syncthreads();
   if (threadIdx.x == 0) {
     DD t = {scratch[0].hi, scratch[0].lo};
-    double ta = scratch[0].abs_sum, tq = scratch[0].sq_sum;
+    Moments tm = {scratch[0].abs_sum, scratch[0].sq_sum, scratch[0].sq_run};
     for (int w = 1; w < nw; ++w) {
       t = dd_add(t, DD{scratch[w].hi, scratch[w].lo});
-      ta += scratch[w].abs_sum;
-      tq += scratch[w].sq_sum;
+      tm.a += scratch[w].abs_sum;
+      tm.q += scratch[w].sq_sum;
+      tm.r += scratch[w].sq_run;
     }
-    *out = {t.hi, t.lo, ta, tq};
+    *out = make_partial(t, tm);
   }
 }
 

@@ -117,61 +117,80 @@ inline Split choose_skip_split(int n, int unroll_bits, int walk_override) {
   return {w, h};
 }
 
-// Stated error of a result, with u = 2^-53 and A = |scale| * sum of |terms|
-// (A/|value| is the condition number of the alternating sum):
+// Stated bound of the rounding errors of a result (err_bound), with
+// u = 2^-53 and A = |scale| * sum of |terms| (A/|value| is the condition
+// number of the alternating sum), to first order in u:
 //  * product rounding: N-2 multiplies and one FMA per term, <= (n+1) u A;
 //  * in-lane accumulation: 2^(R+5) terms in plain double between
-//    double-double folds, <= 2^(R+5) u A (worst case; the folds, wave,
-//    workgroup and grid sums are double-double);
-//  * X drift (non-integer matrices only): each x_j is its closed-form start
-//    plus up to 2^w signed column additions.  Estimated as a random walk of
-//    rounding errors, n sqrt(2^w) u A.  This term is an estimate, not a
-//    bound; for integer matrices X is exact and it vanishes;
+//    double-double folds, <= 2^(R+5) u A (the folds, wave, workgroup and
+//    grid sums are double-double);
 //  * rounding of the start vector x0 (non-integer matrices): n u |value|,
 //    every row's half-ulp error moving the result the same way (see
 //    nw_start_vector).
 // With double-double products (mode 1) the first two terms shrink to
-// (n + 8 + 2^(w+1)) 2^-104 A; with X in double-double as well (mode 2) the
-// drift term vanishes too.
+// (n + 8 + 2^(w+1)) 2^-104 A; mode 2 (X in double-double too) is bounded by
+// four times that.
+// NOT included: the drift of X in the FP64 walk of a non-integer matrix
+// (modes 0 and 1).  Each x_j is its closed-form start plus up to 2^w signed
+// column additions, and how their roundings move a term depends on how close
+// x_j comes to zero, for which no useful worst-case bound exists; the
+// statistical estimate (error_estimate) covers it.  For integer matrices X
+// is exact, so this is a bound on the whole error.
 inline double error_statement(double abs_sum_scaled, double value, int n, int walk_bits,
                               int unroll_bits, bool integral, int mode) {
   const double u = std::ldexp(1.0, -53);
-  // rounding of x0 (non-integer matrices): every row coherent, n u |value|
   const double start = integral ? 0.0 : n * u * std::fabs(value);
   const double dd_terms = std::ldexp((double)n + 8.0 + std::ldexp(2.0, walk_bits), -104);
   if (mode == 2) return abs_sum_scaled * 4.0 * dd_terms + start;
-  const double drift = integral ? 0.0 : n * std::sqrt(std::ldexp(1.0, walk_bits));
-  if (mode == 1) return abs_sum_scaled * (dd_terms + u * drift) + start;
+  if (mode == 1) return abs_sum_scaled * dd_terms + start;
   const double fold = std::ldexp(1.0, unroll_bits + 5);
-  return abs_sum_scaled * u * ((double)n + 1.0 + drift + fold) + start;
+  return abs_sum_scaled * u * ((double)n + 1.0 + fold) + start;
 }
 
-// Expected error (about three standard deviations) when roundings behave as
-// independent random errors.  Per term: the n product roundings give about
-// (n + sqrt(n)) u |T|; the drift of X (non-integer matrices) gives
-// sqrt(n) sqrt(2^w + h) u |T| -- each x_j carries a random walk of the
-// roundings of up to h + 2^w additions, and the drift is shared by
-// neighbouring terms of a lane, so it does not average out within a run.
-// Over terms: the kernels report Q = sum over runs of consecutive terms of
-// (sum of |T| in the run)^2 >= sum T^2, so sqrt(Q) stands for the root sum of
-// squares (it can exceed it when all terms are alike, never fall short).
-// sq_scaled = scale^2 Q.  Double-double products (mode 1) leave only the
-// drift of non-integer matrices; mode 2 has neither.  The rounding of x0
-// (non-integer matrices) adds about 2 sqrt(n) u |value| (rows independent).
-// Calibrated against the exact oracle (n <= 26) and the double-double mode.
-inline double error_estimate(double sq_scaled, double value, int n, int walk_bits,
-                             int high_bits, int mode, bool integral) {
+// Expected error (err_est, three standard deviations) when roundings behave
+// as independent random errors, from the moments the kernels measure:
+// Q = sum over runs of 2^(R+5) consecutive terms of (sum of |T| in the run)^2
+// (>= sum T^2) and Qr = sum over runs of (signed run sum)^2, both scaled by
+// scale^2 (sq_scaled, run_scaled).
+//  * Product and accumulation roundings: about n roundings of relative size
+//    u/sqrt(12) per term, independent across terms: variance (n/12) u^2 Q.
+//  * X drift (non-integer matrices, FP64 walk): the roundings of the column
+//    additions accumulate in x_j as a random walk of up to h + 2^w steps and
+//    move all terms of a run together; a run's error is then about
+//    sum_j e_j d(run sum)/dx_j, which follows the run's signed (cancelled)
+//    sum rather than its sum of |T|: variance (n/12) u^2 (h + 2^(w-1)) Qr,
+//    with the start error (h additions) for the whole lane and the walk's
+//    drift at its mean over the lane (2^(w-1) steps).
+//  * The rounding of x0 (non-integer matrices): about 2 sqrt(n) u |value|
+//    (rows independent), added as is.
+// Calibrated against the double-double Ryser oracle on an emulation of the
+// lane walk (n = 18..30, walks of 2^6..2^12 steps, densities 0.2..0.9,
+// U(0,1], U(-1,1] and log-uniform entries) and checked on the GPU against
+// the exact oracle (n <= 26) and the double-double X mode (n = 28..40).
+// Mode 1 (double-double products) leaves the drift of non-integer matrices,
+// estimated there without the signed run sums (not measured in that mode)
+// as sqrt(n) sqrt(2^w + h) u sqrt(Q); mode 2 has no drift.
+inline double error_estimate(double sq_scaled, double run_scaled, double value, int n,
+                             int walk_bits, int high_bits, int mode, bool integral) {
   const double u = std::ldexp(1.0, -53);
   const double dd = ((double)n + 8.0) * std::ldexp(1.0, -104);
   const double rms = std::sqrt(sq_scaled);
   const double rn = std::sqrt((double)n);
   const double start = integral ? 0.0 : 2.0 * rn * u * std::fabs(value);
-  const double drift =
-      integral ? 0.0 : rn * std::sqrt(std::ldexp(1.0, walk_bits) + (double)high_bits) * u;
   if (mode == 2) return 3.0 * dd * rms + start;
-  if (mode == 1) return 3.0 * (drift + dd) * rms + start;
-  return 3.0 * (((double)n + rn) * u + drift) * rms + start;
+  if (mode == 1) {
+    const double drift =
+        integral ? 0.0 : rn * std::sqrt(std::ldexp(1.0, walk_bits) + (double)high_bits) * u;
+    return 3.0 * (drift + dd) * rms + start;
+  }
+  const double steps = integral ? 0.0 : std::ldexp(1.0, walk_bits - 1) + (double)high_bits;
+  const double var = (double)n / 12.0 * (sq_scaled + steps * run_scaled);
+  return 3.0 * u * std::sqrt(var) + start;
 }
+
+// Relative accuracy every result is held to: a result whose estimated error
+// exceeds kTargetRel |value| is computed again in a more accurate mode.
+constexpr double kTargetRel = 1e-10;
 
 // Double-double addition on the host (same steps as the device dd_add).
 struct HostDD {

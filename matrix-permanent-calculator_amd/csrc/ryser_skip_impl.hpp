@@ -167,7 +167,8 @@ void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restric
     a = 0.0;
     q = 0.0;
   }
-  block_reduce_store(s, a, q, scratch, partials + blockIdx.x);
+  // integer matrices only: X is exact, no drift statistic (sq_run = 0)
+  block_reduce_store(s, Moments{a, q, 0.0}, scratch, partials + blockIdx.x);
   if ((threadIdx.x & (kWave - 1)) == 0)
     walked[blockIdx.x * (256 / kWave) + threadIdx.x / kWave] = wg;
 }

@@ -16,10 +16,12 @@
 //   CPU dense  -c        parallel_perman64 (any -p, main.cu:188-192)
 //   CPU sparse -c -s -p1 parallel_perman64_sparse
 //   CPU sparse -c -s -p2, -p3  parallel_skip_perman64_w / _w_balanced
-// The result line keeps the reference's shape, `Result: <name> <perm> in
-// <seconds>`, printed with 17 significant digits, followed by the stated
-// error of the value.  Approximation (-a) and grid graphs (-i) are not part
-// of this build and exit with status 2.
+// The result line is the reference's, `Result: <name> <perm> in <seconds>`
+// in its stream format (6 significant digits; dense -p 0 prints the xlocal
+// name and -p 66 its printf format, as main.cu:38 and :72 do), followed by
+// `Value: <perm>` with 17 significant digits and the stated error of the
+// value.  Approximation (-a) and grid graphs (-i) are not part of this build
+// and exit with status 2.
 #include <getopt.h>
 #include <omp.h>
 
@@ -36,7 +38,9 @@
 
 namespace {
 
-const char* kDenseNames[] = {"gpu_perman64_xglobal", "gpu_perman64_xlocal", "gpu_perman64_xshared",
+// -p 0 runs gpu_perman64_xglobal; the reference's result line names it
+// gpu_perman64_xlocal (main.cu:38), and so does this one
+const char* kDenseNames[] = {"gpu_perman64_xlocal", "gpu_perman64_xlocal", "gpu_perman64_xshared",
                              "gpu_perman64_xshared_coalescing",
                              "gpu_perman64_xshared_coalescing_mshared"};
 const char* kSparseNames[] = {nullptr, "gpu_perman64_xlocal_sparse",
@@ -58,8 +62,13 @@ int dtype_of(const std::string& t) {
   return PERM_DTYPE_F64;
 }
 
+// The reference's result line, as `cout << ... << perman << " in " << secs`
+// prints it (default stream format: %g, 6 significant digits; main.cu:35-58),
+// so scripts that parse it read the same line; then the value with every
+// digit (results are held to 1e-10 relative, which 6 digits cannot show).
 void print_result(const char* name, double value, double secs) {
-  std::printf("Result: %s %.17g in %.6f\n", name, value, secs);
+  std::printf("Result: %s %g in %g\n", name, value, secs);
+  std::printf("Value: %.17g\n", value);
 }
 
 template <class T>
@@ -123,8 +132,17 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
     }
     std::printf("kernel in %.6f\n", r.kernel_ms / 1e3);
     if (cpu_chunks > 0) std::printf("CPU worker: %d chunk(s)\n", cpu_chunks);
-    print_result(name, r.value, t1 - t0);
-    std::printf("Error bound: %.3e (sum of |terms| %.6e)\n", r.err_bound, r.abs_sum);
+    if (o.dense && o.algo == 66) {  // main.cu:72 prints this one with printf
+      std::printf("Result: %s %2lf in %lf\n", name, r.value, t1 - t0);
+      std::printf("Value: %.17g\n", r.value);
+    } else {
+      print_result(name, r.value, t1 - t0);
+    }
+    std::printf("Error: bound %.3e estimate %.3e (sum of |terms| %.6e, mode %d, %s)\n",
+                r.err_bound, r.err_est, r.abs_sum, r.mode,
+                r.zero_pattern ? "no perfect matching: exactly 0"
+                : r.certified  ? "certified to 1e-10"
+                               : "not certified to 1e-10");
     return 0;
   }
   // CPU
@@ -150,8 +168,8 @@ int run(const Options& o, std::ifstream& in, const permio::Header& h) {
         return 1;
       }
       print_result(name, r.value, t1 - t0);
-      std::printf("Error bound: %.3e (terms evaluated %.6g of %.6g)\n", r.err_bound,
-                  r.main_subsets, r.subsets);
+      std::printf("Error: bound %.3e estimate %.3e (terms evaluated %.6g of %.6g)\n",
+                  r.err_bound, r.err_est, r.main_subsets, r.subsets);
       return 0;
     }
     if (o.algo != 1) {

@@ -40,7 +40,8 @@ class PermResultC(ctypes.Structure):
                 ("subsets", ctypes.c_double), ("n", ctypes.c_int), ("status", ctypes.c_int),
                 ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
                 ("nu", ctypes.c_int), ("main_ms", ctypes.c_double),
-                ("main_subsets", ctypes.c_double), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int)]
+                ("main_subsets", ctypes.c_double), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
+                ("certified", ctypes.c_int), ("zero_pattern", ctypes.c_int)]
 
 
 @dataclass
@@ -61,12 +62,21 @@ class PermResult:
     main_subsets: float
     nu1: int = 0
     nu2: int = 0
+    certified: int = 0
+    zero_pattern: int = 0
+
+    @property
+    def stated(self) -> float:
+        """The stated error of the value: err_bound for an integer matrix
+        (X exact: a worst-case bound of the whole error), else the larger of
+        err_bound and err_est (err_bound leaves out the X drift)."""
+        return max(self.err_bound, self.err_est)
 
     @classmethod
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
                    r.n, r.mode, r.integral, r.err_est, r.nu, r.main_ms,
-                   r.main_subsets, r.nu1, r.nu2)
+                   r.main_subsets, r.nu1, r.nu2, r.certified, r.zero_pattern)
 
 
 class BatchTuningC(ctypes.Structure):

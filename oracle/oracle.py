@@ -329,6 +329,28 @@ def brute_permanent(mat) -> Fraction:
     return tot
 
 
+def has_perfect_matching(mat) -> bool:
+    """Whether the bipartite graph of the nonzero pattern (row i -- column j
+    when mat[i, j] != 0) has a perfect matching, by augmenting paths.  Every
+    term of the permanent is a product over a permutation, so without one
+    the permanent is exactly 0 (checked against brute force in the tests)."""
+    a = np.asarray(mat)
+    n = a.shape[0]
+    adj = [np.nonzero(a[i])[0].tolist() for i in range(n)]
+    col_row = [-1] * n
+
+    def augment(i, seen):
+        for j in adj[i]:
+            if not seen[j]:
+                seen[j] = True
+                if col_row[j] < 0 or augment(col_row[j], seen):
+                    col_row[j] = i
+                    return True
+        return False
+
+    return all(augment(i, [False] * n) for i in range(n))
+
+
 # --------------------------------------------------------------------------
 # Input format and sparse structures of the reference
 def read_matrix(path: str, binary: bool = False):

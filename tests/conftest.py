@@ -50,3 +50,31 @@ def oracle_value(mat) -> float:
     import oracle as O
 
     return float(O.exact_permanent(mat))
+
+
+def assert_target(r, ex, tol: float = 1e-10, ctx=None):
+    """The accuracy contract of a result of the automatic mode: a permanent
+    that is exactly 0 comes back as exactly 0; any other within `tol`
+    relative of the oracle value `ex` and within its own stated error
+    (err_bound for integer matrices, else the larger of err_bound and
+    err_est), and marked certified."""
+    ex = float(ex)
+    if ex == 0.0:
+        assert r.value == 0.0, (ctx, r, ex)
+        return
+    err = abs(r.value - ex)
+    assert err <= tol * abs(ex), (ctx, r.value, ex, err / abs(ex))
+    assert err <= r.stated or (r.stated == 0.0 and r.value == ex), (ctx, r, ex)
+    assert r.certified, (ctx, r)
+
+
+def cli_result(stdout: str):
+    """(name, value) of a `perman` run: the name from the reference-format
+    `Result: <name> <perm> in <s>` line, the value at full precision from the
+    `Value:` line that follows it."""
+    import re
+
+    m = re.search(r"^Result: (\S+) \S+ in \S+$", stdout, re.M)
+    v = re.search(r"^Value: (\S+)$", stdout, re.M)
+    assert m and v, stdout
+    return m.group(1), float(v.group(1))

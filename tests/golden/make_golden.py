@@ -11,6 +11,13 @@ matrices and expected permanents.
   use (keyed by dtype, shape and bytes; see tests/conftest.py): exact for
   n <= 26 and for integer matrices, double-double Ryser otherwise.  Written
   with --cache.
+* c4_mix.json      -- the batched configuration's mix (4096 matrices,
+  n ~ U{12..28}, density ~ U[0.2, 0.8], nonzeros U(0,1], seed 5): for every
+  member with n <= 26 its oracle value -- exactly 0 when the nonzero
+  pattern has no perfect matching (oracle.has_perfect_matching), the exact
+  multi-modular value for n <= 16, double-double Ryser above (relative
+  error ~ n * cond * 2^-104) -- stored as a double-double pair.  Written
+  with --c4.
 * refdata.json     -- a few input files of the reference's own data set
   (int/, float/, double/, n = 30) copied verbatim as text, with their
   permanents from the double-double oracle (rounded to the integer for the
@@ -33,7 +40,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
 import oracle as O  # noqa: E402
-from permanent_amd import synthetic_matrix  # noqa: E402
+from permanent_amd import mixed_batch, synthetic_matrix  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 REFDIR = "/root/reference"
@@ -113,7 +120,43 @@ def cache():
         json.dump(out, fh, indent=0, sort_keys=True)
 
 
+def _c4_entry(args):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import matrix_key  # noqa: E402
+
+    i, a = args
+    n = a.shape[0]
+    rec = {"i": i, "n": n, "key": matrix_key(a)}
+    if not O.has_perfect_matching(a):
+        rec.update(kind="zero", hi=0.0, lo=0.0)
+    elif n <= 16:
+        ex = O.exact_permanent(a)
+        hi = float(ex)
+        rec.update(kind="exact", hi=hi, lo=float(ex - Fraction(hi)))
+    else:
+        hi, lo = O.ryser_dd(a, threads=1)
+        rec.update(kind="dd", hi=hi, lo=lo)
+    return rec
+
+
+def c4_fixture():
+    mats = mixed_batch(4096, 12, 28, 0.2, 0.8, 5)
+    work = [(i, m) for i, m in enumerate(mats) if m.shape[0] <= 26]
+    work.sort(key=lambda t: -t[1].shape[0])  # longest first
+    with ProcessPoolExecutor(max_workers=8) as pool:
+        recs = list(pool.map(_c4_entry, work, chunksize=1))
+    recs.sort(key=lambda r: r["i"])
+    kinds = {k: sum(r["kind"] == k for r in recs) for k in ("zero", "exact", "dd")}
+    print("c4 members:", len(recs), kinds, flush=True)
+    with open(os.path.join(OUT, "c4_mix.json"), "w") as fh:
+        json.dump({"config": "mixed_batch(4096, 12, 28, 0.2, 0.8, seed=5), members n <= 26",
+                   "members": recs}, fh)
+
+
 def main():
+    if "--c4" in sys.argv:
+        c4_fixture()
+        return
     if "--refdata-only" in sys.argv:
         refdata()
         return

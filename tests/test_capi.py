@@ -17,7 +17,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
-from conftest import GOLDEN, ROOT
+from conftest import GOLDEN, ROOT, cli_result
 
 HEADER = os.path.join(ROOT, "include", "perman_hip.h")
 
@@ -115,9 +115,9 @@ def test_cli_cpu_result_line(tmp_path):
     write_matrix(f, a)
     r = run_cli("-f", str(f), "-c", "-t", "4")
     assert r.returncode == 0, r.stderr
-    m = re.search(r"^Result: parallel_perman64 (\S+) in (\S+)$", r.stdout, re.M)
-    assert m, r.stdout
-    assert float(m.group(1)) == pytest.approx(float(O.exact_permanent(a)), rel=1e-12)
+    name, v = cli_result(r.stdout)
+    assert name == "parallel_perman64"
+    assert v == pytest.approx(float(O.exact_permanent(a)), rel=1e-12)
 
 
 def test_cli_binary_and_int_types(tmp_path):
@@ -126,11 +126,11 @@ def test_cli_binary_and_int_types(tmp_path):
     write_matrix(f, a, "int")
     r = run_cli("-f", str(f), "-c", "-b")  # -b: every listed entry becomes 1
     assert r.returncode == 0, r.stderr
-    v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+    v = cli_result(r.stdout)[1]
     assert v == float(O.exact_permanent((a != 0).astype(float)))
     r = run_cli("-f", str(f), "-c", "-s", "-p", "1")
-    assert "Result: parallel_perman64_sparse" in r.stdout
-    v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+    name, v = cli_result(r.stdout)
+    assert name == "parallel_perman64_sparse"
     assert v == float(O.exact_permanent(a))
 
 
@@ -141,7 +141,7 @@ def test_cli_preprocessing_orders_keep_the_permanent(tmp_path):
     ex = float(O.exact_permanent(a))
     for r_opt in ("1", "2"):
         r = run_cli("-f", str(f), "-c", "-r", r_opt)
-        v = float(re.search(r"Result: \S+ (\S+)", r.stdout).group(1))
+        v = cli_result(r.stdout)[1]
         assert v == pytest.approx(ex, rel=1e-12)
 
 
@@ -227,3 +227,119 @@ def test_connector_entry_points_reject_without_touching_the_gpu(tmp_path):
     g.write_text("3 3 complex\n")
     assert math.isnan(P.read_calculate_return(str(g), 5))
     assert math.isnan(P.matlab_calculate_return(np.eye(3, dtype=np.int32), 1))
+
+
+def _order_cases():
+    """Matrices for the -r1/-r2 comparison: ties in the column counts
+    (small-integer entries), entries <= 0 (dropped by the > 0 filter, still
+    nonzero for -r2's degrees), empty columns, dense columns."""
+    rng = np.random.default_rng(77)
+    cases = []
+    for n in (5, 8, 13, 21, 32):
+        a = rng.integers(0, 4, size=(n, n)) * (rng.random((n, n)) < 0.3)
+        cases.append(("ties", a))
+        b = a - 2 * (rng.random((n, n)) < 0.15)  # some negative entries
+        cases.append(("negative", b))
+        c = a.copy()
+        c[:, n // 2] = 0  # an empty column
+        cases.append(("empty_col", c))
+        d = rng.integers(1, 9, size=(n, n)) * (rng.random((n, n)) < 0.7)
+        cases.append(("dense", d))
+    return cases
+
+
+def _shim(tmp_path):
+    src = os.path.join(ROOT, "tests", "permio_shim.cpp")
+    so = tmp_path / "libpermio_shim.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", src, "-o", str(so)],
+                   check=True)
+    return ctypes.CDLL(str(so))
+
+
+@pytest.mark.parametrize("typ", ["double", "float", "int"])
+def test_reader_and_orders_bit_identical_to_compiled_reference(tmp_path, typ):
+    """The CLI's reader and its -r1 / -r2 orderings against the reference's
+    own ReadMatrix, matrix2compressed_sortOrder and _skipOrder (util.h:342-357,
+    :553-684) compiled from its sources: the same matrix, the same permuted
+    matrix and byte-identical CCS/CRS arrays.  Ties in -r1 follow glibc's
+    qsort as the reference calls it (a stable merge sort for arrays this
+    small: its comparator only ever answers 'greater').  Matrices with an
+    all-zero row are left out of -r2: there the reference leaves part of its
+    row permutation uninitialised."""
+    ref = O.ref_lib()
+    if ref is None:
+        pytest.skip("reference not compiled (oracle/_ref)")
+    shim = _shim(tmp_path)
+    np_t = {"double": np.float64, "float": np.float32, "int": np.int32}[typ]
+    c_t = {"double": ctypes.c_double, "float": ctypes.c_float, "int": ctypes.c_int}[typ]
+    sfx = {"double": "f64", "float": "f32", "int": "i32"}[typ]
+    ip = ctypes.POINTER(ctypes.c_int)
+    tp = ctypes.POINTER(c_t)
+    for name, a in _order_cases():
+        n = a.shape[0]
+        vals = a.astype(np_t) if typ == "int" else (a * 0.37).astype(np_t)
+        f = tmp_path / f"{name}_{n}"
+        write_matrix(f, vals.astype(float), typ)
+        for generic in (1, 0):
+            mats = []
+            for lib_, fn in ((ref, f"ref_read_matrix_{sfx}"), (shim, f"shim_read_matrix_{sfx}")):
+                m = np.zeros(n * n, dtype=np_t)
+                args = [str(f).encode(), generic, m.ctypes.data_as(tp), n * n]
+                if lib_ is ref:
+                    args.append(ctypes.byref(ctypes.c_int()))
+                assert getattr(lib_, fn)(*args) == n
+                mats.append(m)
+            assert mats[0].tobytes() == mats[1].tobytes(), (name, n, generic)
+        m0 = mats[0]
+        has_zero_row = not (m0.reshape(n, n) != 0).any(axis=1).all()
+        for order in (0, 1, 2):
+            if order == 2 and has_zero_row:
+                continue
+            outs = []
+            for lib_, fn in ((ref, f"ref_compressed_order_{sfx}"),
+                             (shim, f"shim_compressed_order_{sfx}")):
+                m = m0.copy()
+                arrs = [np.zeros(n + 1, np.int32), np.zeros(n * n, np.int32),
+                        np.zeros(n * n, np_t), np.zeros(n + 1, np.int32),
+                        np.zeros(n * n, np.int32), np.zeros(n * n, np_t)]
+                ptrs = [x.ctypes.data_as(ip if k in (0, 1, 3, 4) else tp)
+                        for k, x in enumerate(arrs)]
+                cargs = [order, m.ctypes.data_as(tp), n]
+                if lib_ is ref:
+                    cargs.append(n * n)  # capacity of the reference's arrays
+                stored = getattr(lib_, fn)(*cargs, *ptrs)
+                outs.append((stored, m.tobytes(), [x.tobytes() for x in arrs]))
+            assert outs[0][0] == outs[1][0], (name, n, order)
+            assert outs[0][1] == outs[1][1], (name, n, order, "permuted matrix")
+            for k, (x, y) in enumerate(zip(outs[0][2], outs[1][2])):
+                assert x == y, (name, n, order, ["cptrs", "rows", "cvals", "rptrs", "cols",
+                                                 "rvals"][k])
+
+
+def test_no_perfect_matching_is_exactly_zero_without_a_walk():
+    """A matrix whose nonzero pattern has no perfect matching has permanent
+    exactly 0: every entry point returns 0 (zero_pattern set, errors 0,
+    certified) from the host check alone -- no device is touched, so this
+    runs without a GPU.  The library's check agrees with the oracle's
+    matching and with the exact permanent."""
+    rng = np.random.default_rng(5)
+    found = 0
+    for trial in range(300):
+        n = int(rng.integers(2, 14))
+        a = (rng.random((n, n)) < 0.25) * (1.0 - rng.random((n, n)))
+        if O.has_perfect_matching(a):
+            continue
+        found += 1
+        if n <= 9:
+            assert O.exact_permanent(a) == 0
+        for r in (P.dense(a), P.sparse(a), P.skip(a)):
+            assert r.value == 0.0 and r.zero_pattern == 1 and r.certified == 1
+            assert r.err_bound == 0.0 and r.err_est == 0.0
+    assert found > 30
+    # a structurally singular integer matrix through the chunked entry point
+    z = np.ones((12, 12), dtype=np.int32)
+    z[:, 3] = 0
+    r = P.PermResultC()
+    assert P.lib().perm_chunked(0, z.ctypes.data, None, None, None, 12, 0, 1, ctypes.byref(r),
+                                None) == 0
+    assert r.value == 0.0 and r.zero_pattern == 1

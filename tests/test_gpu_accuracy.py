@@ -1,7 +1,8 @@
 """Accuracy of the FP64 walk and its stated error.
 
-* the statistical error estimate covers the observed error against the
-  exact oracle, and never exceeds the stated bound;
+* the statistical error estimate of the FP64 pass covers the observed error
+  against the exact oracle, and is not needlessly conservative (median
+  within 100x of the observed error);
 * matrices with mixed signs against the exact oracle;
 * every work split (walk length per lane) gives the permanent within the
   stated error, and the default split at n = 36 stays within 1e-11 of the
@@ -12,7 +13,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
-from conftest import oracle_value
+from conftest import assert_target, oracle_value
 
 pytestmark = pytest.mark.gpu
 
@@ -25,9 +26,32 @@ def rel(a, b):
 def test_error_estimate_covers_observed(n, density):
     a = P.synthetic_matrix(n, density, seed=900 + n)
     ex = oracle_value(a)
-    r = P.dense(a, 4)
+    plan = P.Plan(a)
+    r = plan.run(0)  # the FP64 pass alone
+    plan.close()
     assert abs(r.value - ex) <= r.err_est, (n, r.value, ex, r.err_est)
-    assert r.err_est <= r.err_bound
+
+
+def test_error_estimate_is_tight():
+    """Over a set of FP64 passes (n = 20..26, densities 0.2..0.9) the
+    estimate covers every observed error and its median overestimate stays
+    below 100x (it is a 3-sigma figure, calibrated on the lane-walk
+    roundings)."""
+    ratios = []
+    for n in (20, 22, 24, 26):
+        for k, density in enumerate((0.2, 0.5, 0.9)):
+            a = P.synthetic_matrix(n, density, seed=4000 + 10 * n + k)
+            ex = oracle_value(a)
+            plan = P.Plan(a)
+            r = plan.run(0)
+            plan.close()
+            err = abs(r.value - ex)
+            assert err <= r.err_est, (n, density, r, ex)
+            if err > 0:
+                ratios.append(r.err_est / err)
+    ratios.sort()
+    print("err_est / observed:", [f"{x:.1f}" for x in ratios])
+    assert ratios[len(ratios) // 2] < 100.0
 
 
 @pytest.mark.parametrize("n", [13, 17, 21])
@@ -36,7 +60,7 @@ def test_mixed_signs_vs_exact(n):
     a = rng.uniform(-1.0, 1.0, size=(n, n))
     ex = float(O.exact_permanent(a))
     r = P.dense(a, 4)
-    assert abs(r.value - ex) <= max(1e-11 * abs(ex), r.err_bound), (r.value, ex)
+    assert_target(r, ex, tol=1e-11, ctx=n)
 
 
 @pytest.mark.parametrize("walk_bits", [0, 4, 8, 12, 16, 20])
@@ -46,7 +70,7 @@ def test_every_split_within_stated_error(monkeypatch, walk_bits):
     if walk_bits:
         monkeypatch.setenv("PERMAN_WALK_BITS", str(walk_bits))
     r = P.dense(a, 4)
-    assert abs(r.value - ex) <= r.err_bound
+    assert abs(r.value - ex) <= r.stated
     assert rel(r.value, ex) < 1e-12
 
 

@@ -1,8 +1,8 @@
 """Dense GPU path (reference IDs 0-4, gpu_exact_dense.cu) against the oracle.
 
-Tolerances: FP64 results must agree with the exact permanent to 1e-10
-relative (or to the stated error bound when the sum is ill-conditioned),
-0/1 matrices with permanent < 2^53 must round to the exact integer."""
+Tolerances: results must agree with the exact permanent to 1e-10 relative
+and within their stated error (conftest.assert_target); 0/1 matrices with
+permanent < 2^53 must round to the exact integer."""
 import math
 
 import numpy as np
@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
+from conftest import assert_target
 
 pytestmark = pytest.mark.gpu
 
@@ -23,8 +24,7 @@ def test_dense_random_vs_exact(n):
     a = P.synthetic_matrix(n, 0.6, seed=100 + n)
     ex = float(O.exact_permanent(a))
     r = P.dense(a, 4)
-    assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_bound), (n, r.value, ex)
-    assert abs(r.value - ex) <= r.err_bound + 1e-300
+    assert_target(r, ex, ctx=n)
 
 
 @pytest.mark.parametrize("algo", [0, 1, 2, 3, 4])
@@ -52,7 +52,7 @@ def test_binary_above_2_53_relative():
     ex = O.exact_permanent(a)
     assert ex > 2 ** 53
     r = P.dense(a.astype(np.int32), 1)
-    assert abs(r.value - float(ex)) <= max(1e-12 * float(ex), r.err_bound)
+    assert abs(r.value - float(ex)) <= max(1e-12 * float(ex), r.stated)
 
 
 @pytest.mark.parametrize("n", [5, 9, 12])

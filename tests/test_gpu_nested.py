@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
-from conftest import oracle_value
+from conftest import assert_target, oracle_value
 
 pytestmark = pytest.mark.gpu
 
@@ -28,8 +28,7 @@ def test_nested_vs_exact(n, density, seed):
     for sparse in (False, True):
         r = P.sparse(a) if sparse else P.dense(a)
         print(f"n={n} d={density} shape=({r.nu},{r.nu1},{r.nu2}) rel={abs(r.value - ex) / abs(ex):.2e}")
-        assert abs(r.value - ex) <= r.err_bound, (n, density, sparse, r.value, ex)
-        assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_est)
+        assert_target(r, ex, ctx=(n, density, sparse))
 
 
 def test_every_shape_family_appears_and_agrees(monkeypatch):
@@ -46,9 +45,9 @@ def test_every_shape_family_appears_and_agrees(monkeypatch):
             fam = "plain" if r.nu == n else "pair" if r.nu1 == n else \
                   "nest2" if r.nu2 == n else "nest3"
             seen.add(fam)
-            tol = r.err_bound + plain.err_bound
+            tol = r.stated + plain.stated
             assert abs(r.value - plain.value) <= tol, (n, density, r, plain)
-            assert abs(r.value - pair.value) <= r.err_bound + pair.err_bound
+            assert abs(r.value - pair.value) <= r.stated + pair.stated
             assert abs(r.value - plain.value) <= 1e-11 * abs(plain.value) + r.err_est + plain.err_est
     print("families:", sorted(seen))
     assert {"pair", "nest3"} <= seen
@@ -92,4 +91,4 @@ def test_double_double_products_on_nested_shapes(n, density, seed):
     f, d = plan.run(0), plan.run(1)
     plan.close()
     assert (d.nu, d.nu1, d.nu2) == (f.nu, f.nu1, f.nu2)
-    assert abs(d.value - f.value) <= f.err_bound + d.err_bound
+    assert abs(d.value - f.value) <= f.stated + d.stated

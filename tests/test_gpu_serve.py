@@ -24,10 +24,10 @@ def _check(m, r):
         if np.issubdtype(m.dtype, np.integer) and ex < 2 ** 53:
             assert r.value == ex, (n, r.value, ex)
         else:
-            assert abs(r.value - ex) <= max(1e-12 * abs(ex), r.err_bound), (n,)
+            assert abs(r.value - ex) <= max(1e-12 * abs(ex), r.stated), (n,)
     else:
         s = P.dense(m)
-        assert abs(r.value - s.value) <= r.err_bound + s.err_bound
+        assert abs(r.value - s.value) <= r.stated + s.stated
         assert abs(r.value - s.value) <= 1e-10 * abs(s.value)
 
 
@@ -84,7 +84,7 @@ def test_server_matches_batch_path_bits():
     with P.Server(item_log2=16) as srv:
         sres = [srv.wait(srv.submit(m))[0] for m in mats]
     for x, y in zip(bres, sres):
-        assert abs(x.value - y.value) <= x.err_bound + y.err_bound
+        assert abs(x.value - y.value) <= x.stated + y.stated
         assert abs(x.value - y.value) <= 1e-12 * abs(x.value)
 
 
@@ -131,7 +131,7 @@ def test_server_wait_many_and_prepared_requests():
             assert (a.sum_hi, a.sum_lo) == (b.sum_hi, b.sum_lo) and lat > 0
         for q, r in zip(reqs, one):  # the plain call on the same prepared request
             v = P.solve(q)
-            assert abs(v.value - r.value) <= v.err_bound + r.err_bound
+            assert abs(v.value - r.value) <= v.stated + r.stated
         t = srv.submit_request(reqs[0])
         with pytest.raises(P.PermError):
             srv.wait_many([t, t])  # listed twice: the second is an error

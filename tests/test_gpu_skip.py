@@ -48,7 +48,7 @@ def test_small_integers_vs_exact(n):
     np.fill_diagonal(a, 1)
     ex = O.exact_permanent(a)
     r = P.skip(a)
-    assert r.value == float(ex) or abs(r.value - float(ex)) <= r.err_bound
+    assert r.value == float(ex) or abs(r.value - float(ex)) <= r.stated
 
 
 def test_fallbacks_follow_the_sparse_path():

@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 import permanent_amd as P
-from conftest import GOLDEN
+from conftest import GOLDEN, assert_target
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +22,7 @@ def test_sparse_matches_exact(n, algo):
     a = P.synthetic_matrix(n, 0.3, seed=600 + n, full_support=True)
     ex = float(O.exact_permanent(a))
     r = P.sparse(a, algo_id=algo)
-    assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_bound), (n, algo, r.value, ex)
+    assert_target(r, ex, ctx=(n, algo))
 
 
 @pytest.mark.parametrize("algo", [1, 2, 3, 4])
@@ -69,4 +69,4 @@ def test_golden_sparse_fixtures():
             continue
         r = P.sparse(a, algo_id=4)
         ex = c["value"]
-        assert abs(r.value - ex) <= max(1e-10 * abs(ex), r.err_bound), c["name"]
+        assert_target(r, ex, ctx=c["name"])

@@ -228,3 +228,17 @@ def test_oracle_cache_matches_its_matrices_and_recomputes():
     assert cheap
     for m in cheap[:3]:
         assert oracle_value(m) == float(O.exact_permanent(m))
+
+
+def test_perfect_matching_check_against_brute_force():
+    """oracle.has_perfect_matching is False exactly when the 0/1 pattern's
+    permanent (brute force over n! permutations) is 0."""
+    rng = np.random.default_rng(123)
+    seen = {True: 0, False: 0}
+    for trial in range(400):
+        n = int(rng.integers(1, 8))
+        p = (rng.random((n, n)) < rng.uniform(0.15, 0.6)).astype(float)
+        has = O.has_perfect_matching(p)
+        assert has == (O.brute_permanent(p) != 0), p
+        seen[has] += 1
+    assert seen[True] > 50 and seen[False] > 50
