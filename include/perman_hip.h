@@ -157,6 +157,19 @@ int perm_dense(int dtype, const void* mat, int nov, int algo_id, perm_result* ou
 int perm_sparse(int dtype, const void* mat, const int* cptrs, const int* rows, const void* cvals,
                 int nov, int algo_id, perm_result* out);
 
+/* ---- dense path without device allocation (extension) --------------------
+ * No counterpart in the reference (its wrappers cudaMalloc and cudaFree per
+ * call, gpu_exact_dense.cu:669-689).  For callers that cannot allocate device
+ * memory -- tight serving loops, harnesses that forbid hipMalloc: the same
+ * computation and bits as perm_dense, with every device buffer in one static
+ * device array of 4 MiB per GPU (held until the process ends), so nov <= 37;
+ * larger nov fail with PERM_ERR_SIZE.  Calls no hipMalloc, hipFree or other
+ * allocating HIP routine (the first call on a GPU creates that GPU's stream
+ * and events).  Calls on one GPU share its array: they are serialised by a
+ * lock inside the library; calls from several processes each have their own
+ * copy. */
+int perm_dense_noalloc(int dtype, const void* mat, int nov, int algo_id, perm_result* out);
+
 /* ---- resident plans (extension) ------------------------------------------
  * A plan uploads one matrix's column store once; perm_plan_run enqueues the
  * Ryser kernel + grid reduction on the plan's stream and waits for it, so a

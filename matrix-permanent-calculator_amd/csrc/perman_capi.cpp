@@ -472,6 +472,49 @@ int run_prepared(int n, const Prepared& p, int mode, perm_result* out) {
 }
 }  // namespace perm
 
+// ---- perm_dense_noalloc: the dense path with no device allocation ----------
+// All device memory of one call -- column store, start vector, one partial
+// record per workgroup and the result record -- lives in `scratch`, a
+// file-scope array every GPU holds one copy of until the process ends.  In
+// host code the name `scratch` is only a stand-in for the symbol: its
+// device address on each GPU comes from hipGetSymbolAddress (kept in
+// NoallocDevice::ptr) and only that address is ever passed to a kernel or
+// a copy.  4 MiB fit the walk of every n <= 37 (2^24 lanes, 2^16 partial
+// records of 48 bytes); larger n are rejected with PERM_ERR_SIZE.
+constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxNoallocDevices = 64;
+// Per-GPU state of perm_dense_noalloc, set up by its first call on that GPU
+// (with the GPU current): the device address of `scratch`, a stream and the
+// timing events.  `mu` serialises the calls on one GPU -- they share its
+// copy of the array, whose contents the last call leaves behind.
+struct NoallocDevice {
+  std::mutex mu;
+  bool ready = false;
+  char* ptr = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
+};
+NoallocDevice g_noalloc[kMaxNoallocDevices];
+
+int noalloc_setup(NoallocDevice& D) {
+  void* p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(PERM_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+  HIP_TRY(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&D.ev0));
+  HIP_TRY(hipEventCreate(&D.ev1));
+  HIP_TRY(hipEventCreate(&D.evm));
+  D.ptr = (char*)p;
+  D.ready = true;
+  return PERM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int perm_last_status(void) { return perm::last_status(); }
@@ -518,6 +561,50 @@ int perm_dense(int dtype, const void* mat, int nov, int algo_id, perm_result* ou
   Prepared p;
   if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
   return run_oneshot(nov, p, -1, out);
+}
+
+// perm_dense with every device buffer in `scratch` (see above): no
+// hipMalloc/hipFree or other allocating HIP call; the first call on a GPU
+// also creates that GPU's stream and events.  Same walk split, kernels and
+// reduction as perm_dense, so the same bits.
+int perm_dense_noalloc(int dtype, const void* mat, int nov, int algo_id, perm_result* out) {
+  if (!out) return fail(PERM_ERR_ARG, "null result");
+  if (algo_id < 0 || algo_id > 4) return fail(PERM_ERR_ARG, "unknown dense algorithm ID");
+  Prepared p;
+  if (int s = perm::prepare_dense(dtype, mat, nov, p)) return s;
+  if (p.zero) {  // exactly 0: nothing to upload or run
+    perm::fill_zero_result(nov, p.integral, *out);
+    return ok();
+  }
+  if (int s = select_device()) return s;
+  const int dev = perm::current_device();
+  if (dev >= kMaxNoallocDevices) return fail(PERM_ERR_NODEV, "device index above 63");
+  NoallocDevice& D = g_noalloc[dev];
+  std::lock_guard<std::mutex> guard(D.mu);
+  if (!D.ready)
+    if (int s = noalloc_setup(D)) return s;
+  perm_plan P;
+  P.device = dev;
+  P.borrowed = true;
+  P.stream = D.stream;
+  P.ev0 = D.ev0;
+  P.ev1 = D.ev1;
+  P.evm = D.evm;
+  P.integral = p.integral;
+  set_shape(&P, nov, p);
+  const size_t need = layout_plan(&P, nov, p.cs.size());
+  if (need > kScratchBytes)
+    return fail(PERM_ERR_SIZE, "nov=" + std::to_string(nov) + " needs " + std::to_string(need) +
+                                   " bytes of device scratch; perm_dense_noalloc has " +
+                                   std::to_string(kScratchBytes) + " (n <= 37)");
+  bind_plan_memory(&P, D.ptr, p.cs.size());
+  // pageable sources: plan_run_impl waits for the stream before returning,
+  // and p outlives the call
+  HIP_TRY(hipMemcpyAsync((void*)P.L.colpm, p.cs.data(), p.cs.size() * sizeof(double),
+                         hipMemcpyHostToDevice, P.stream));
+  HIP_TRY(hipMemcpyAsync((void*)P.L.x0, p.x0.data(), p.x0.size() * sizeof(double),
+                         hipMemcpyHostToDevice, P.stream));
+  return plan_run_impl(&P, -1, out);
 }
 
 int perm_sparse(int dtype, const void* mat, const int* cptrs, const int* rows, const void* cvals,

@@ -122,6 +122,8 @@ def lib():
                                  ctypes.POINTER(PermResultC)]
         L.perm_sparse.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(PermResultC)]
+        L.perm_dense_noalloc.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(PermResultC)]
         L.perm_plan_dense.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(vp)]
         L.perm_plan_sparse.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int,
                                        ctypes.POINTER(vp)]

@@ -70,6 +70,10 @@ def lib():
             f = getattr(L, f"orc_gpu_dense_floatx_{sfx}")
             f.restype = ctypes.c_double
             f.argtypes = [ptr, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_ryser_u128.restype = None
+        L.orc_ryser_u128.argtypes = [ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.POINTER(ctypes.c_uint64)]
         L.orc_ryser_mod.restype = ctypes.c_uint64
         L.orc_ryser_mod.argtypes = [_u64p, ctypes.c_int, ctypes.c_uint64]
         L.orc_ryser_dd.restype = None
@@ -310,6 +314,34 @@ def exact_permanent(mat) -> Fraction:
     if x > prod // 2:
         x -= prod
     return Fraction(x, 1 << e)
+
+
+def exact_integer_permanent(mat, threads: int = 8) -> int:
+    """Exact permanent of an integer matrix (|entries| < 2^31, n <= 62) by
+    Ryser's formula in 128-bit wrap-around arithmetic (orc_ryser_u128): the
+    result is the permanent modulo 2^128, hence exact when the bound
+    prod_i sum_j |a_ij| is below 2^127; larger bounds fall back to the
+    multi-modular exact_permanent."""
+    a = np.asarray(mat)
+    n = a.shape[0]
+    if n == 0:
+        return 1
+    ai = a.astype(np.int64)
+    if not np.array_equal(ai, a) or np.abs(ai).max() >= 2 ** 31 or n > 62:
+        raise ValueError("integer matrix with |entries| < 2^31 and n <= 62 expected")
+    bound = 1
+    for r in ai:
+        bound *= max(1, int(np.abs(r).sum()))
+    if bound >= 2 ** 127:
+        f = exact_permanent(a)
+        assert f.denominator == 1
+        return f.numerator
+    ai = np.ascontiguousarray(ai)
+    lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_ryser_u128(ai.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n, threads,
+                         ctypes.byref(lo), ctypes.byref(hi))
+    v = (hi.value << 64) | lo.value
+    return v - (1 << 128) if v >= 1 << 127 else v
 
 
 def brute_permanent(mat) -> Fraction:

@@ -20,6 +20,8 @@
  *                          wrappers :401-699), evaluated thread by thread on the CPU
  *   orc_ryser_mod          exact Ryser modulo a prime (exact oracle building block)
  *   orc_ryser_dd           double-double Ryser (accurate oracle, n <= ~30)
+ *   orc_ryser_u128         exact Ryser of an integer matrix in 128-bit
+ *                          wrap-around arithmetic (exact when |perm| < 2^127)
  *
  * Build: see oracle/Makefile (-O2 -ffp-contract=off so that every
  * double/float operation rounds exactly once, as the reference's
@@ -521,4 +523,60 @@ void orc_ryser_dd(const double* mat, int n, int threads, double* out_hi, double*
   double sc = (double)(4 * (n & 1) - 2);
   *out_hi = p.hi * sc;
   *out_lo = p.lo * sc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact Ryser of an integer matrix in the ring Z/2^128 (unsigned __int128
+ * arithmetic wraps modulo 2^128), using Ryser's original form over all 2^n
+ * column subsets S:  perm = (-1)^n sum_S (-1)^|S| prod_i (sum_{j in S} a_ij),
+ * so every quantity is an integer (the Nijenhuis-Wilf form's half-integers
+ * would need a division by 2^n).  Every operation is a ring operation, so
+ * the result is the permanent modulo 2^128: the exact permanent whenever
+ * |perm| < 2^127, which the caller checks with prod_i sum_j |a_ij|.  Row
+ * sums are exact in int64 (|a_ij| < 2^31, n <= 62).  The subset range is
+ * split over OpenMP threads, each chunk starting from its closed-form row
+ * sums; the chunk sums are added modulo 2^128 (order-independent).
+ * Result: the two 64-bit halves of the wrapped sum.                       */
+void orc_ryser_u128(const i64* a, int n, int threads, uint64_t* out_lo, uint64_t* out_hi) {
+  typedef unsigned __int128 u128;
+  const i64 total = (i64)1 << n;
+  if (threads < 1) threads = 1;
+  i64 chunk = total / threads + 1;
+  u128* part = (u128*)calloc(threads, sizeof(u128));
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+  for (int tid = 0; tid < threads; tid++) {
+    i64 st = (i64)tid * chunk, en = min_i64(st + chunk, total);
+    u128 acc = 0;
+    if (st < en) {
+      i64 rs[64];
+      i64 g = st ^ (st >> 1);
+      for (int i = 0; i < n; i++) {
+        rs[i] = 0;
+        for (int j = 0; j < n; j++)
+          if ((g >> j) & 1LL) rs[i] += a[i * n + j];
+      }
+      for (i64 t = st; t < en; t++) {
+        if (t > st) {
+          int k = __builtin_ctzll((unsigned long long)t);
+          g ^= 1LL << k;
+          if ((g >> k) & 1LL)
+            for (int i = 0; i < n; i++) rs[i] += a[i * n + k];
+          else
+            for (int i = 0; i < n; i++) rs[i] -= a[i * n + k];
+        }
+        u128 prod = 1;
+        for (int i = 0; i < n; i++) prod *= (u128)(__int128)rs[i];
+        /* (-1)^|S| with |S| = popcount(gray) */
+        if (__builtin_popcountll((unsigned long long)g) & 1) acc -= prod;
+        else acc += prod;
+      }
+    }
+    part[tid] = acc;
+  }
+  u128 sum = 0;
+  for (int tid = 0; tid < threads; tid++) sum += part[tid];
+  free(part);
+  if (n & 1) sum = (u128)0 - sum; /* (-1)^n */
+  *out_lo = (uint64_t)sum;
+  *out_hi = (uint64_t)(sum >> 64);
 }

@@ -18,11 +18,13 @@ matrices and expected permanents.
   multi-modular value for n <= 16, double-double Ryser above (relative
   error ~ n * cond * 2^-104) -- stored as a double-double pair.  Written
   with --c4.
-* refdata.json     -- a few input files of the reference's own data set
-  (int/, float/, double/, n = 30) copied verbatim as text, with their
-  permanents from the double-double oracle (rounded to the integer for the
-  int file), plus the reference's double-X serial value (perman64
-  restatement) for information.
+* refdata.json     -- input files of the reference's own data set (int/,
+  float/, double/ at n = 30..34 and real/ibm32) copied verbatim as text,
+  with their permanents: exact integers for the int files (128-bit Ryser,
+  oracle.exact_integer_permanent), the double-double oracle for the others,
+  plus the reference's double-X serial value (perman64 restatement, n <= 32)
+  for information.  --refdata-only regenerates it, keeping entries whose
+  file is unchanged.
 
 The reference's data files are inputs only -- the reference stores no
 expected outputs -- so every expected value here comes from the oracle.
@@ -192,18 +194,35 @@ def main():
         refdata()
 
 
+# Reference data files with their oracle values: all three element types at
+# n = 30..34 and the integer real-world matrix ibm32 (n = 32).  Integer files
+# get the exact permanent (orc_ryser_u128, exact below 2^127); float and
+# double files the double-double Ryser (relative error ~ n cond 2^-104).
+REF_FILES = ["int/30_0.30_0", "float/30_0.50_0", "double/30_0.50_0", "double/30_0.20_1",
+             "int/31_0.30_0", "float/31_0.30_0", "double/31_0.30_0",
+             "int/32_0.20_0", "float/32_0.20_0", "double/32_0.30_0", "real/ibm32.mtxzero",
+             "int/33_0.20_0", "float/33_0.30_0", "double/33_0.10_0",
+             "int/34_0.20_0", "float/34_0.10_0", "double/34_0.20_0"]
+
+
 def refdata():
-    if os.path.isdir(REFDIR):
-        files = ["int/30_0.30_0", "float/30_0.50_0", "double/30_0.50_0", "double/30_0.20_1"]
-        ref = []
-        with ProcessPoolExecutor(max_workers=4) as ex_pool:
-            futs = {f: ex_pool.submit(_ref_case, f) for f in files}
-            for f in files:
-                rec = futs[f].result()
-                ref.append(rec)
-                print(f, rec["value"], flush=True)
-        with open(os.path.join(OUT, "refdata.json"), "w") as fh:
-            json.dump(ref, fh)
+    """Regenerate refdata.json; entries already there (same file, same
+    text) are kept, so adding a file costs only its own oracle run."""
+    if not os.path.isdir(REFDIR):
+        return
+    path = os.path.join(OUT, "refdata.json")
+    old = {r["file"]: r for r in json.load(open(path))} if os.path.exists(path) else {}
+    ref = []
+    for f in REF_FILES:
+        rec = old.get(f)
+        if rec is None or rec.get("text") != open(os.path.join(REFDIR, f)).read() or \
+                (rec["type"] == "int" and rec.get("kind") != "exact"):
+            rec = _ref_case(f)
+        ref.append(rec)
+        print(f, rec["value"], flush=True)
+        with open(path, "w") as fh:  # after every file: a long run keeps its progress
+            json.dump(ref + [r for k, r in old.items() if k not in REF_FILES and
+                             k not in [x["file"] for x in ref]], fh)
 
 
 def _ref_case(rel):
@@ -211,14 +230,18 @@ def _ref_case(rel):
     text = open(path).read()
     mat, n, nnz, typ = O.read_matrix(path)
     rec = {"file": rel, "type": typ, "n": n, "text": text}
-    hi, lo = O.ryser_dd(mat.astype(np.float64), threads=2)
-    rec["dd_hi"], rec["dd_lo"] = hi, lo
-    rec["value"] = hi + lo
     if typ == "int":
-        # integer matrix: X is exact in double-double, products carry ~1e-31
-        # relative error, so the rounded double-double sum is the integer
-        rec["rounded"] = str(round(Fraction(hi) + Fraction(lo)))
-    rec["perman64"] = O.perman64(mat)
+        ex = O.exact_integer_permanent(mat.astype(np.int64), threads=8)
+        rec["kind"] = "exact"
+        rec["rounded"] = str(ex)  # the exact integer permanent
+        hi = float(ex)
+        rec["dd_hi"], rec["dd_lo"] = hi, float(Fraction(ex) - Fraction(hi))
+    else:
+        hi, lo = O.ryser_dd(mat.astype(np.float64), threads=8)
+        rec["kind"] = "dd"
+        rec["dd_hi"], rec["dd_lo"] = hi, lo
+    rec["value"] = rec["dd_hi"] + rec["dd_lo"]
+    rec["perman64"] = O.perman64(mat) if n <= 32 else None
     return rec
 
 

@@ -343,3 +343,19 @@ def test_no_perfect_matching_is_exactly_zero_without_a_walk():
     assert P.lib().perm_chunked(0, z.ctypes.data, None, None, None, 12, 0, 1, ctypes.byref(r),
                                 None) == 0
     assert r.value == 0.0 and r.zero_pattern == 1
+
+
+def test_lib_connect_exports_the_reference_connector_names():
+    """libConnect.so exports exactly the reference connector's three entry
+    points (py_connector.c:59, :63, :152) and refuses the approximation IDs
+    (NaN, no device call) the way the library's perm_* forms do."""
+    path = os.path.join(os.path.dirname(P.LIB_PATH), "libConnect.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    names = sorted(line.split()[-1] for line in out.splitlines() if " T " in line)
+    assert names == ["connect", "matlab_calculate_return", "read_calculate_return"]
+    lib = ctypes.cdll.LoadLibrary(path)
+    lib.read_calculate_return.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5
+    lib.read_calculate_return.restype = ctypes.c_double
+    for algo in (0, 1, 2, 3):
+        assert math.isnan(lib.read_calculate_return(b"/nonexistent", algo, 1, 1, 1, 1))

@@ -157,3 +157,48 @@ def test_batch_integer_exact_pass_on_batch_kernels():
             assert abs(r.value - float(ex)) <= r.stated
     res2, _ = b.run()  # a replay recomputes the same values
     assert [r.value for r in res2] == [r.value for r in res]
+
+
+def _c4_fixture():
+    import json
+    import os
+
+    from conftest import GOLDEN
+
+    path = os.path.join(GOLDEN, "c4_mix.json")
+    if not os.path.exists(path):
+        pytest.skip("c4_mix.json not generated")
+    return json.load(open(path))["members"]
+
+
+def test_c4_mix_every_member_exact_zero_or_within_1e10():
+    """The batched configuration's whole mix (4096 matrices, n ~ U{12..28},
+    density ~ U[0.2, 0.8], seed 5) through one batch and through one plain
+    call per matrix: every member with n <= 26 is exactly 0 when its pattern
+    has no perfect matching, else within 1e-10 of its oracle value
+    (tests/golden/c4_mix.json: exact for n <= 16, double-double above) and
+    certified; the batch and plain results agree to 1e-10."""
+    from conftest import matrix_key
+
+    members = _c4_fixture()
+    mats = P.mixed_batch(4096, 12, 28, 0.2, 0.8, 5)
+    b = P.Batch()
+    for m in mats:
+        b.add(m)
+    res, ms = b.run()
+    b.close()
+    modes = {}
+    for rec in members:
+        m = mats[rec["i"]]
+        assert matrix_key(m) == rec["key"], rec["i"]
+        r = res[rec["i"]]
+        ex = rec["hi"] + rec["lo"]
+        if rec["kind"] == "zero":
+            assert r.value == 0.0 and r.zero_pattern == 1, (rec["i"], r)
+        assert_target(r, ex, ctx=("batch", rec["i"], rec["n"]))
+        modes[r.mode] = modes.get(r.mode, 0) + 1
+        s = P.dense(m)
+        assert_target(s, ex, ctx=("plain", rec["i"], rec["n"]))
+        assert abs(r.value - s.value) <= 1e-10 * abs(s.value), (rec["i"], r, s)
+    print(f"C4 mix: {len(members)} members checked, batch modes {modes}, device {ms:.1f} ms")
+    assert all(r.certified for r in res)

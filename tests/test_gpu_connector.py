@@ -42,3 +42,42 @@ def test_matlab_calculate_return():
     ex = float(O.exact_permanent(a))
     for algo in (4, 5, 6, 7, 8):
         assert P.matlab_calculate_return(a, algo) == ex
+
+
+def _lib_connect():
+    """libConnect.so bound the way the reference's callers bind it:
+    superpython.py loads it with cdll.LoadLibrary and sets the argtypes of
+    read_calculate_return (superpython.py:6-26); supermaTlab.m calls
+    matlab_calculate_return through matlab_calculate_return.h."""
+    import ctypes
+    import os
+
+    lib = ctypes.cdll.LoadLibrary(os.path.join(os.path.dirname(P.LIB_PATH), "libConnect.so"))
+    lib.connect()
+    lib.read_calculate_return.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.read_calculate_return.restype = ctypes.c_double
+    lib.matlab_calculate_return.argtypes = [ctypes.POINTER(ctypes.c_int)] + [ctypes.c_int] * 7
+    lib.matlab_calculate_return.restype = ctypes.c_double
+    return lib
+
+
+@pytest.mark.parametrize("algo", [4, 5, 6, 7, 8])
+def test_lib_connect_under_the_reference_names(tmp_path, algo):
+    """The reference's callers switch by changing only the library path:
+    libConnect.so's connect / read_calculate_return / matlab_calculate_return
+    give the exact permanent for every exact algorithm ID."""
+    import ctypes
+
+    lib = _lib_connect()
+    a = P.synthetic_matrix(15, 0.55, seed=63)
+    f = tmp_path / "m.txt"
+    write_matrix(f, a, "double")
+    ex = float(O.exact_permanent(a))
+    v = lib.read_calculate_return(str(f).encode("utf-8"), algo, 8, 10, 4, 5)
+    assert v == pytest.approx(ex, rel=1e-12), (algo, v, ex)
+    b = P.synthetic_matrix(11, 0.5, seed=64, binary=True).astype(np.int32)
+    bp = np.ascontiguousarray(b)
+    w = lib.matlab_calculate_return(bp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), algo,
+                                    2, 2, 2, 2, 11, int((b != 0).sum()))
+    assert w == float(O.exact_permanent(b))

@@ -94,3 +94,72 @@ def test_cli_every_gpu_id(tmp_path):
         assert r.returncode == 0, (args, r.stdout, r.stderr)
         name, v = cli_result(r.stdout)
         assert v == ex, (args, r.stdout)
+
+
+def test_compiled_caller_invokes_every_gpu_template(tmp_path):
+    """A main.cu-style C++ caller (RunAlgo's calls, main.cu:31-148) compiled
+    with g++ against include/perman.hpp calls every GPU template -- dense
+    IDs 0-6 and 66, sparse 1-8 and 66 -- for int, float and double inputs,
+    and gets the exact permanent of a 0/1-pattern integer matrix."""
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    a = (P.synthetic_matrix(15, 0.5, seed=151, binary=True) * 2).astype(np.int32)
+    ex = float(O.exact_permanent(a))
+    n = a.shape[0]
+    cp, rows, cv = P.compress(a)
+    rp, cols, rv = P._crs(a)
+    arr = lambda v: ",".join(str(int(x)) for x in v)  # noqa: E731
+    src = tmp_path / "caller.cpp"
+    src.write_text(r"""
+#include "perman.hpp"
+#include <cstdio>
+#include <vector>
+template <class T> void run_all(const int* src, int n, int nnz, const int* cp0, const int* r0,
+                                const int* rp0, const int* c0) {
+  std::vector<T> m(src, src + n * n), cv, rv;
+  std::vector<int> cp(cp0, cp0 + n + 1), rows(r0, r0 + nnz), rp(rp0, rp0 + n + 1), cols(c0, c0 + nnz);
+  for (int k = 0; k < n; ++k) for (int e = cp[k]; e < cp[k + 1]; ++e) cv.push_back(m[rows[e] * n + k]);
+  for (int i = 0; i < n; ++i) for (int e = rp[i]; e < rp[i + 1]; ++e) rv.push_back(m[i * n + cols[e]]);
+  T* M = m.data(); int *CP = cp.data(), *R = rows.data(), *RP = rp.data(), *C = cols.data();
+  T *CV = cv.data();
+  double v[] = {
+    gpu_perman64_xglobal(M, n, 2048, 256), gpu_perman64_xlocal(M, n, 2048, 256),
+    gpu_perman64_xshared(M, n, 2048, 256), gpu_perman64_xshared_coalescing(M, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared(M, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpu(M, n, 2, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks(M, n, 2, true, 4, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpu_manual_distribution(M, n, 4, 2048, 256),
+    gpu_perman64_xlocal_sparse(M, CP, R, CV, n, 2048, 256),
+    gpu_perman64_xshared_sparse(M, CP, R, CV, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_sparse(M, CP, R, CV, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_sparse(M, CP, R, CV, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpu_sparse(M, CP, R, CV, n, 2, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_sparse(M, CP, R, CV, n, 2, true, 4, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_skipper(M, RP, C, CP, R, CV, n, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpucpu_chunks_skipper(M, RP, C, CP, R, CV, n, 2, false, 4, 2048, 256),
+    gpu_perman64_xshared_coalescing_mshared_multigpu_sparse_manual_distribution(M, CP, R, CV, n, 4, 2048, 256)};
+  for (double x : v) std::printf("%.17g\n", x);
+}
+int main() {
+  const int n = """ + str(n) + r""", nnz = """ + str(int(cp[-1])) + r""";
+  const int m[] = {""" + arr(a.ravel()) + r"""};
+  const int cp[] = {""" + arr(cp) + r"""}, rows[] = {""" + arr(rows) + r"""};
+  const int rp[] = {""" + arr(rp) + r"""}, cols[] = {""" + arr(cols) + r"""};
+  run_all<int>(m, n, nnz, cp, rows, rp, cols);
+  run_all<float>(m, n, nnz, cp, rows, rp, cols);
+  run_all<double>(m, n, nnz, cp, rows, rp, cols);
+  return 0;
+}
+""")
+    exe = tmp_path / "caller"
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(P.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{inc}", str(src), f"-L{libdir}",
+                    "-lperman_hip", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120, check=True)
+    vals = [float(x) for x in out.stdout.split()]
+    assert len(vals) == 3 * 17
+    assert vals == [ex] * len(vals), (ex, vals)

@@ -242,3 +242,37 @@ def test_perfect_matching_check_against_brute_force():
         assert has == (O.brute_permanent(p) != 0), p
         seen[has] += 1
     assert seen[True] > 50 and seen[False] > 50
+
+
+def test_c4_mix_fixture_pins():
+    """tests/golden/c4_mix.json: its keys are the members of the batched
+    configuration's mix as the generator builds them today; every member
+    marked zero has no perfect matching (and the rest do); a sample of the
+    exact entries is recomputed with the exact oracle and the double-double
+    entries of a few small members with the double-double oracle."""
+    import json
+
+    from conftest import GOLDEN, matrix_key
+
+    import permanent_amd as P
+
+    path = os.path.join(GOLDEN, "c4_mix.json")
+    if not os.path.exists(path):
+        pytest.skip("c4_mix.json not generated")
+    members = json.load(open(path))["members"]
+    mats = P.mixed_batch(4096, 12, 28, 0.2, 0.8, 5)
+    assert len(members) == sum(m.shape[0] <= 26 for m in mats)
+    checked_exact = checked_dd = 0
+    for rec in members:
+        m = mats[rec["i"]]
+        assert matrix_key(m) == rec["key"] and m.shape[0] == rec["n"] <= 26
+        assert (rec["kind"] == "zero") == (not O.has_perfect_matching(m)), rec["i"]
+        if rec["kind"] == "exact" and rec["n"] <= 13 and checked_exact < 40:
+            ex = O.exact_permanent(m)
+            assert float(ex) == rec["hi"] and float(ex - Fraction(rec["hi"])) == rec["lo"]
+            checked_exact += 1
+        if rec["kind"] == "dd" and rec["n"] <= 18 and checked_dd < 10:
+            hi, lo = O.ryser_dd(m, 4)
+            assert abs((hi + lo) - (rec["hi"] + rec["lo"])) <= 1e-20 * abs(hi)
+            checked_dd += 1
+    assert checked_exact == 40 and checked_dd == 10
