@@ -47,6 +47,16 @@ def affinity_kept():
         os.sched_setaffinity(0, mask)
 
 
+def cpu_share() -> int:
+    """Host threads the CPU legs use: OMP_NUM_THREADS when the environment
+    sets it (the CPU share given to one GPU on a shared host), else every
+    core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as fh:
@@ -73,6 +83,9 @@ def cpu_baseline(mat, log2_sample: int, threads: int):
         "cores": threads,
         "kind": "port",
         "cpu": cpu_model(),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "sample": (f"parallel_perman64 float-X restatement (algo.h:662-746), OpenMP "
                    f"{threads} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), "
                    f"Gray slice of 2^{log2_sample} subsets of the same "
@@ -95,9 +108,30 @@ def sparse_leg(P, reps: int):
     wall = (time.perf_counter() - t0) / reps
     plan.close()
     subsets = 2.0 ** 35
+    # the same pattern, all-ones (-b): an integer matrix, exact integer result
+    mb = (m > 0).astype("int32")
+    planb = P.Plan(mb, sparse_path=True)
+    planb.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rb = planb.run()
+    wall_b = (time.perf_counter() - t0) / reps
+    planb.close()
+    binary = {
+        "workload": "sparse path -b: the same n=36 pattern, every entry 1, 2^35 subsets",
+        "subsets_per_s": subsets / wall_b,
+        "ms_per_run": wall_b * 1e3,
+        "kernel_ms": rb.kernel_ms,
+        "modes_run": "FP64 pass then double-double products" if rb.mode == 1 else "FP64 pass",
+        "value": rb.value,
+        "exact_integer": rb.err_bound == 0.0 and rb.value == round(rb.value),
+    }
     return {
         "workload": "sparse path, n=36, density 0.2 (every row/column nonzero), U(0,1], "
                     "seed 4, 2^35 subsets, matrix resident",
+        "binary": binary,
+        "certified": r.certified,
+        "rel_err_est": r.err_est / abs(r.value),
         "subsets_per_s": subsets / wall,
         "ms_per_run": wall * 1e3,
         "kernel_ms": r.kernel_ms,
@@ -157,9 +191,8 @@ def batched_leg(P, count: int, seed: int):
     subsets = sum(2.0 ** (m.shape[0] - 1) for m in mats)
     t0 = time.perf_counter()
     b = P.Batch()
-    for m in mats:
-        b.add(m)
-    res, _ = b.run()  # builds: plan items, upload, capture the graph
+    b.add_many(mats, threads=cpu_share())  # host preparation on the CPU share's threads
+    res, _ = b.run()  # builds (plan items, upload) and launches; no graph yet
     t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     tuned = b.autotune(reps=2)  # picks the work split on this batch
@@ -176,14 +209,8 @@ def batched_leg(P, count: int, seed: int):
     t0 = time.perf_counter()
     plain = [P.dense(m) for m in mats]
     t_plain = time.perf_counter() - t0
-    # difference relative to the permanent, or to the stated error when the
-    # permanent is (structurally) zero
-    worst = max(abs(r.value - v.value) / max(abs(v.value), r.err_bound + v.err_bound, 1e-300)
-                for r, v in zip(res, plain))
-    # the same difference in units of the two results' stated error bounds
-    # (ill-conditioned sparse matrices have relative bounds far above 1e-10)
-    worst_err = max(abs(r.value - v.value) / max(r.err_bound + v.err_bound, 1e-300)
-                    for r, v in zip(res, plain))
+    worst, worst_err = _diff_vs_plain(res, plain)
+    oracle_err = _vs_c4_fixture(mats, res)
     return {
         "workload": f"{count} matrices, n~U{{12..28}}, density~U[0.2,0.8], U(0,1], seed {seed}",
         "subsets": subsets,
@@ -198,8 +225,46 @@ def batched_leg(P, count: int, seed: int):
         "speedup_vs_plain": t_plain / t_batch,
         "speedup_end_to_end_vs_plain": t_plain / t_first,
         "max_rel_diff_vs_plain": worst,
-        "max_diff_over_err_bound": worst_err,
+        "max_diff_over_stated_error": worst_err,
+        "certified_fraction": sum(r.certified for r in res) / len(res),
+        "zero_pattern": sum(r.zero_pattern for r in res),
+        "passes": {f"mode{m}": sum(r.mode == m for r in res) for m in (0, 1, 2)},
+        "vs_oracle_fixture": oracle_err,
     }
+
+
+def _diff_vs_plain(res, plain):
+    """Largest difference between two result lists relative to the
+    permanent (0 when both are exactly 0), and in units of the sum of their
+    stated errors."""
+    rel = err = 0.0
+    for r, v in zip(res, plain):
+        d = abs(r.value - v.value)
+        if d == 0.0:
+            continue
+        rel = max(rel, d / abs(v.value) if v.value != 0.0 else float("inf"))
+        err = max(err, d / max(r.stated + v.stated, 1e-300))
+    return rel, err
+
+
+def _vs_c4_fixture(mats, res):
+    """Largest relative error of the batch results against the oracle values
+    of tests/golden/c4_mix.json (the members with n <= 26; data only)."""
+    path = os.path.join(ROOT, "tests", "golden", "c4_mix.json")
+    if not os.path.exists(path) or len(mats) != 4096:
+        return None
+    members = json.load(open(path))["members"]
+    worst, zeros_exact = 0.0, 0
+    for rec in members:
+        ex = rec["hi"] + rec["lo"]
+        v = res[rec["i"]].value
+        if ex == 0.0:
+            zeros_exact += v == 0.0
+        else:
+            worst = max(worst, abs(v - ex) / abs(ex))
+    return {"members": len(members), "max_rel_err": worst,
+            "exact_zeros": zeros_exact,
+            "zeros": sum(rec["hi"] + rec["lo"] == 0.0 for rec in members)}
 
 
 def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
@@ -220,9 +285,10 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
         for q in reqs[:32]:  # warm the kernels of the common sizes
             srv.wait(srv.submit_request(q))
         t0 = time.perf_counter()
-        tickets = [srv.submit_request(q) for q in reqs]
+        tickets = srv.submit_many(reqs)  # one call; preparation runs in the server's pool
         closed = srv.wait_many(tickets)
         t_closed = time.perf_counter() - t0
+        stats = srv.stats()
     capacity = count / t_closed
     rate = load * capacity
     gaps = -np.log(1.0 - P.uniforms((seed << 20) + (1 << 18), count)) / rate
@@ -262,10 +328,7 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
     t0 = time.perf_counter()
     plain = [P.solve(q) for q in reqs]
     t_plain = time.perf_counter() - t0
-    worst = max(abs(r.value - v.value) / max(abs(v.value), r.err_bound, 1e-300)
-                for (r, _), v in zip(closed, plain))
-    worst_err = max(abs(r.value - v.value) / max(r.err_bound + v.err_bound, 1e-300)
-                    for (r, _), v in zip(closed, plain))
+    worst, worst_err = _diff_vs_plain([r for r, _ in closed], plain)
     return {
         "workload": f"{count} requests, n in [16,36] with P(n)~2^(-(n-16)/{decay:g}), density "
                     f"U[0.1,0.9], half 0/1 half FP64, 1/3 sparse path, seed {seed}",
@@ -281,7 +344,10 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
         "plain_req_per_s": count / t_plain,
         "capacity_vs_plain": t_plain / t_closed,
         "max_rel_diff_vs_plain": worst,
-        "max_diff_over_err_bound": worst_err,
+        "max_diff_over_stated_error": worst_err,
+        "certified_fraction": sum(r.certified for r, _ in closed) / len(closed),
+        "passes": {f"mode{m}": sum(r.mode == m for r, _ in closed) for m in (0, 1, 2)},
+        "server_stats_closed_loop": stats,
     }
 
 
@@ -292,7 +358,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=32)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=cpu_share())
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 0/1, batched and served legs")
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -352,12 +418,19 @@ def main():
         "pct_fp64_peak_whole_step": 100.0 * 2.0 * N * subsets / (ms_per_step / 1e3) / 1e12
                                     / FP64_PEAK_TFLOPS,
         "result": {"value": r.value, "err_est": r.err_est, "err_bound": r.err_bound,
-                   "abs_sum": r.abs_sum, "shape": [r.nu, r.nu1, r.nu2]},
+                   "rel_err_est": r.err_est / abs(r.value), "certified": r.certified,
+                   "mode": r.mode, "abs_sum": r.abs_sum, "shape": [r.nu, r.nu1, r.nu2]},
+        "shader_clock_ghz": r.clock_ghz,
         "roofline": {
-            "bound": "mfma",
-            "bound_detail": "FP64 compute: the kernel issues FP64 VALU instructions, whose MI355X "
-                            "peak (78.6 TFLOP/s) equals the FP64 matrix-core peak; HBM does not "
-                            "bound it (traffic below, ~26 MB/s)",
+            "bound": "fp64_valu",
+            "bound_detail": "FP64 vector (VALU) compute: the kernel issues FP64 VALU instructions "
+                            "and no MFMA; the FP64 vector peak is 78.6 TFLOP/s (256 CUs x 128 "
+                            "flop/clk x 2.4 GHz, FMA = 2); HBM does not bound it (traffic below, "
+                            "~26 MB/s)",
+            "clock_ghz": r.clock_ghz,
+            "peak_at_measured_clock": FP64_PEAK_TFLOPS * r.clock_ghz / 2.4 if r.clock_ghz else None,
+            "frac_at_measured_clock": achieved / (FP64_PEAK_TFLOPS * r.clock_ghz / 2.4)
+                                      if r.clock_ghz else None,
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

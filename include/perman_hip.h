@@ -58,7 +58,12 @@ extern "C" {
  * to 1e-10 relative: an FP64 result with err_est > 1e-10 |value| is computed
  * again in mode 1 (integer matrices) or mode 2 (others), automatically, and
  * `certified` says whether the final result meets the target (exact, or
- * err_est <= 1e-10 |value|).  A matrix whose nonzero pattern has no perfect
+ * err_est <= 1e-10 |value|).  On the single-matrix path a non-integer FP64
+ * result whose modelled err_est misses the target first has its error
+ * measured: 256 of its workgroups, sampled in proportion to their modelled
+ * variance, are walked again in mode 2, and err_est becomes three standard
+ * deviations of the error estimated from them (verified_units); the full
+ * mode 2 pass runs only if that does not certify it.  A matrix whose nonzero pattern has no perfect
  * matching has permanent exactly 0: it is returned as 0 (zero_pattern = 1,
  * errors 0) without running a walk. */
 typedef struct perm_result {
@@ -81,6 +86,10 @@ typedef struct perm_result {
   int nu2;              /* FP64 kernel: rows touched by Gray columns 0-2 (nested variant), or n */
   int certified;        /* 1: exact, or err_est <= 1e-10 |value| */
   int zero_pattern;     /* 1: no perfect matching in the nonzero pattern; value is exactly 0 */
+  double clock_ghz;     /* single-matrix path: mean shader clock of the first pass's walk,
+                           from its workgroups' cycle and 100 MHz real-time counters (0: none) */
+  int verified_units;   /* single-matrix path: workgroups of the FP64 walk recomputed in mode 2
+                           to measure its error (err_est then comes from them); 0: none */
 } perm_result;
 
 /* ---- library state ------------------------------------------------------ */
@@ -162,7 +171,7 @@ int perm_sparse(int dtype, const void* mat, const int* cptrs, const int* rows, c
  * call, gpu_exact_dense.cu:669-689).  For callers that cannot allocate device
  * memory -- tight serving loops, harnesses that forbid hipMalloc: the same
  * computation and bits as perm_dense, with every device buffer in one static
- * device array of 4 MiB per GPU (held until the process ends), so nov <= 37;
+ * device array of 4 MiB per GPU (held until the process ends), so nov <= 35;
  * larger nov fail with PERM_ERR_SIZE.  Calls no hipMalloc, hipFree or other
  * allocating HIP routine (the first call on a GPU creates that GPU's stream
  * and events).  Calls on one GPU share its array: they are serialised by a
@@ -184,12 +193,27 @@ int perm_plan_run(perm_plan* plan, perm_result* out);
  * mode, several times slower).  perm_plan_run picks 0, then 1 when an
  * integer matrix needs it to round to the exact integer. */
 int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out);
+/* The plan's walk split: 2^high_bits lanes of 2^walk_bits subsets each, in
+ * `blocks` workgroups of 256 lanes; lane l fixes the Gray columns
+ * [walk_bits, n-1) (in the plan's walk order) by the bits of l. */
+int perm_plan_geometry(const perm_plan* plan, int* walk_bits, int* high_bits,
+                       unsigned long long* blocks);
+/* The sum over the workgroups [block_begin, block_end) only (lanes
+ * [256 block_begin, 256 block_end)) in kernel mode `mode`: the unit one
+ * device of a multi-GPU run computes (the ranges of all devices add up to
+ * the permanent), and a piece of the walk small enough to check against a
+ * CPU evaluation of the same subsets.  out->sum_hi/sum_lo: the range's
+ * signed sum; value = (4(n&1)-2) * sum; certified is 0 (the target applies
+ * to whole results). */
+int perm_plan_run_blocks(perm_plan* plan, int mode, unsigned long long block_begin,
+                         unsigned long long block_end, perm_result* out);
 void perm_plan_destroy(perm_plan* plan);
 
 /* ---- batches of mixed n (extension) --------------------------------------
  * Many matrices (n <= 40) resident on the device, computed by persistent
- * kernels (one per variant: n and walk shape, concurrent on several streams)
- * captured in one hipGraph; perm_batch_run replays it.  Integer matrices
+ * kernels (one per variant: n and walk shape, concurrent on several streams);
+ * the first perm_batch_run after a change launches them directly, the
+ * second captures them in one hipGraph, and later runs replay it.  Integer matrices
  * whose FP64 error could reach 1/2 then get a double-double pass over the
  * same items (the batch kernels' double-double variant).  A matrix is cut into 64-lane
  * items of about 2^item_log2 subsets that depend only on its own n, so its
@@ -202,6 +226,12 @@ int perm_batch_set(perm_batch* batch, int item_log2, int streams, int max_grid);
 int perm_batch_add_dense(perm_batch* batch, int dtype, const void* mat, int nov, int* index);
 int perm_batch_add_sparse(perm_batch* batch, int dtype, const void* mat, const int* cptrs,
                           const int* rows, const void* cvals, int nov, int* index);
+/* Adds `count` dense matrices at once (mats[i]: dtypes[i], order novs[i]),
+ * preparing them on `threads` host threads; on failure nothing is added and
+ * the first failing matrix is reported.  *first_index: index of the first. */
+int perm_batch_add_dense_many(perm_batch* batch, int count, const int* dtypes,
+                              const void* const* mats, const int* novs, int threads,
+                              int* first_index);
 int perm_batch_size(const perm_batch* batch);
 /* results: one per matrix, in insertion order; device_ms: graph time plus
  * the double-double passes, if any. */
@@ -338,10 +368,14 @@ double perm_matlab_calculate_return(const int* mat, int algorithm, int nt, int x
 /* ---- server: continuous batching of requests (extension) ---------------
  * No counterpart in the reference (its entry points compute one permanent
  * per call).  Requests (n <= 40, dense or sparse) are submitted from any
- * thread and return a ticket at once; a scheduler thread packs the 64-lane
- * items of all admitted requests into iterations of a few milliseconds of
- * GPU work, shortest remaining work first, and sums each request's items in
- * a fixed order: a result is bitwise the same alone or in any mix.
+ * thread and return a ticket at once: the submission only copies the matrix;
+ * a pool of host threads prepares it (start vector, column store, walk
+ * shape) and a scheduler thread packs the 64-lane items of all admitted
+ * requests into iterations of a few milliseconds of GPU work, shortest
+ * remaining work first (a request older than max_wait_ms goes ahead), and
+ * sums each request's items in a fixed order: a result is bitwise the same
+ * alone or in any mix.  Argument errors found while preparing (a malformed
+ * CCS) are reported by the wait/poll of that request.
  * Admission is bounded by a device arena of resident matrices and by the
  * queued work.  perm_server_wait blocks until the request is done and
  * frees its ticket; perm_server_poll returns PERM_PENDING until then.  In a
@@ -358,6 +392,9 @@ typedef struct perm_server_config {
   double iteration_us;    /* work per iteration at full FP64 issue rate, microseconds (8000) */
   int max_resident;       /* matrices resident on the device (default 2048) */
   double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */
+  int prep_threads;       /* host threads preparing submitted matrices (default 4) */
+  double max_wait_ms;     /* age after which a request is scheduled ahead of the
+                             shortest-remaining-work order (default 50) */
 } perm_server_config;
 typedef struct perm_server_stats {
   long long submitted, admitted, completed, iterations, items, pending;
@@ -367,7 +404,7 @@ typedef struct perm_server_stats {
   double retire_ms;       /* scheduler time summing and publishing results */
   double gpu_wait_ms;     /* scheduler time blocked on the GPU */
   double gpu_ms;          /* summed GPU span of the iterations (upload to result copy) */
-  long long dd_passes;    /* integer requests recomputed with double-double products */
+  long long dd_passes;    /* passes in a double-double mode: escalations and sampled error measurements */
 } perm_server_stats;
 void perm_server_default_config(perm_server_config* cfg);
 int perm_server_create(const perm_server_config* cfg, perm_server** server);
@@ -376,6 +413,15 @@ int perm_server_submit_dense(perm_server* server, int dtype, const void* mat, in
 int perm_server_submit_sparse(perm_server* server, int dtype, const void* mat, const int* cptrs,
                               const int* rows, const void* cvals, int nov,
                               unsigned long long* id);
+/* Submits `count` matrices at once (one lock, one wake-up): ids[i] receives
+ * the ticket of mats[i] (dtypes[i], order novs[i]); a request takes the
+ * sparse path when cptrs is not NULL and cptrs[i] is not NULL (with rows[i],
+ * cvals[i]).  Returns the first argument error; requests before it are
+ * submitted. */
+int perm_server_submit_many(perm_server* server, int count, const int* dtypes,
+                            const void* const* mats, const int* const* cptrs,
+                            const int* const* rows, const void* const* cvals, const int* novs,
+                            unsigned long long* ids);
 int perm_server_wait(perm_server* server, unsigned long long id, perm_result* out,
                      double* latency_ms);
 /* Waits for `count` requests in order; out[i] / latency_ms[i] (latency_ms

@@ -10,7 +10,8 @@
 // one persistent kernel per variant drains its group, the groups running
 // concurrently on several streams.  The whole launch sequence (counter reset,
 // variant kernels, per-matrix reduction, result copy) is captured once in a
-// hipGraph and replayed by perm_batch_run.
+// hipGraph at the second run and replayed from then on (the first run after a
+// build launches directly: a batch computed once pays no capture).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/perman_hip.h"
@@ -74,6 +76,10 @@ struct perm_batch {
   hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams] = {}, ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
   bool built = false;
+  bool ran = false;  // one direct run since the build: the next run captures the graph
+  // sampled verification: copies of the chosen items and their records
+  void* vmem = nullptr;
+  size_t vbytes = 0;
 };
 
 namespace {
@@ -81,10 +87,14 @@ namespace {
 void release_device(perm_batch* B) {
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
   B->graph = nullptr;
+  B->ran = false;
   if (B->dmem) (void)hipFree(B->dmem);
   B->dmem = nullptr;
   if (B->h_results) (void)hipHostFree(B->h_results);
   B->h_results = nullptr;
+  if (B->vmem) (void)hipFree(B->vmem);
+  B->vmem = nullptr;
+  B->vbytes = 0;
   B->built = false;
 }
 
@@ -177,7 +187,7 @@ int capture(perm_batch* B) {
   return PERM_OK;
 }
 
-// Plan items, upload, capture the launch graph.
+// Plan items and upload.
 int build(perm_batch* B) {
   HIP_TRYB(hipSetDevice(B->device));
   release_device(B);
@@ -258,9 +268,7 @@ int build(perm_batch* B) {
   HIP_TRYB(hipMemcpyAsync(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
                           hipMemcpyHostToDevice, s0));
   HIP_TRYB(hipStreamSynchronize(s0));
-  const int st = capture(B);
-  if (st != PERM_OK) return st;
-  B->built = true;
+  B->built = true;  // the graph is captured at the second run (a batch run once needs none)
   return perm::ok();
 }
 
@@ -304,6 +312,96 @@ int escalate(perm_batch* B, const std::vector<std::pair<size_t, int>>& esc,
     const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
     perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
                            B->h_results[em.first], results[em.first]);
+  }
+  return PERM_OK;
+}
+
+// Sampled error measurement (perm::pps_sample) of the FP64 results of the
+// matrices `ms` (non-integer, at least kVerifyMinUnits items): for each, up
+// to kVerifyUnits of its items, drawn in proportion to their modelled
+// variance, are walked again in mode 2 (copies of the items in a separate
+// list, records in a separate array), and err_est becomes the estimate from
+// their measured errors (perm::sampled_error).  Values are unchanged.
+int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* results,
+                 float& pass_ms) {
+  pass_ms = 0.f;
+  hipStream_t s0 = B->s[0];
+  const size_t n_items = B->items.size();
+  std::vector<Partial> fp(n_items);
+  HIP_TRYB(hipMemcpyAsync(fp.data(), B->d_partials, n_items * sizeof(Partial),
+                          hipMemcpyDeviceToHost, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  struct Pick {
+    std::vector<uint32_t> ids;
+    std::vector<double> weight;
+    uint32_t certain = 0, begin = 0;
+  };
+  std::vector<Pick> picks(ms.size());
+  std::vector<BatchItem> vitems;
+  for (size_t k = 0; k < ms.size(); ++k) {
+    const Entry& e = B->ents[ms[k]];
+    std::vector<double> z(e.item_count);
+    for (uint32_t i = 0; i < e.item_count; ++i) {
+      const Partial& p = fp[e.first_item + i];
+      z[i] = perm::unit_weight(p.sq_sum, p.sq_run, e.walk_bits, e.high_bits);
+    }
+    perm::pps_sample(z, perm::kVerifyUnits, picks[k].ids, picks[k].weight, picks[k].certain);
+    picks[k].begin = (uint32_t)vitems.size();
+    for (uint32_t i : picks[k].ids) vitems.push_back(B->items[e.first_item + i]);
+  }
+  if (vitems.empty()) return PERM_OK;
+  const size_t it_b = align256(vitems.size() * sizeof(BatchItem));
+  const size_t need = it_b + align256(vitems.size() * sizeof(Partial));
+  if (need > B->vbytes) {
+    if (B->vmem) HIP_TRYB(hipFree(B->vmem));
+    B->vmem = nullptr;
+    B->vbytes = 0;
+    HIP_TRYB(hipMalloc(&B->vmem, need));
+    B->vbytes = need;
+  }
+  BatchItem* d_vitems = (BatchItem*)B->vmem;
+  Partial* d_vparts = (Partial*)((char*)B->vmem + it_b);
+  uint32_t* ctr = B->d_counters + B->groups.size();
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  HIP_TRYB(hipEventRecord(B->ev0, s0));
+  HIP_TRYB(hipMemcpyAsync(d_vitems, vitems.data(), vitems.size() * sizeof(BatchItem),
+                          hipMemcpyHostToDevice, s0));
+  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)ms.size(), s0));
+  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+  for (size_t k = 0; k < ms.size(); ++k) {
+    const Entry& e = B->ents[ms[k]];
+    const uint32_t cnt = (uint32_t)picks[k].ids.size();
+    if (cnt == 0) continue;
+    const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
+    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((cnt + 3) / 4, B->max_grid));
+    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, d_vitems, picks[k].begin,
+                                        picks[k].begin + cnt, ctr + k, d_vparts, grid,
+                                        B->s[k % ns]));
+  }
+  for (int i = 1; i < ns; ++i) {
+    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+  }
+  std::vector<Partial> ref(vitems.size());
+  HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, vitems.size() * sizeof(Partial),
+                          hipMemcpyDeviceToHost, s0));
+  HIP_TRYB(hipEventRecord(B->ev1, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  HIP_TRYB(hipEventElapsedTime(&pass_ms, B->ev0, B->ev1));
+  for (size_t k = 0; k < ms.size(); ++k) {
+    const Entry& e = B->ents[ms[k]];
+    const Pick& pk = picks[k];
+    std::vector<double> err(pk.ids.size());
+    for (size_t j = 0; j < pk.ids.size(); ++j) {
+      const Partial& f = fp[e.first_item + pk.ids[j]];
+      const Partial& r = ref[pk.begin + j];
+      err[j] = (f.hi - r.hi) + (f.lo - r.lo);
+    }
+    perm_result& o = results[ms[k]];
+    o.err_est = perm::sampled_error(err, pk.weight, pk.certain, e.n, o.value);
+    o.certified = perm::is_certified(o) ? 1 : 0;
   }
   return PERM_OK;
 }
@@ -404,6 +502,41 @@ int perm_batch_add_sparse(perm_batch* B, int dtype, const void* mat, const int* 
   return add_prepared(B, nov, p, index);
 }
 
+int perm_batch_add_dense_many(perm_batch* B, int count, const int* dtypes,
+                              const void* const* mats, const int* novs, int threads,
+                              int* first_index) {
+  if (!B || count < 0 || (count > 0 && (!dtypes || !mats || !novs)))
+    return perm::fail(PERM_ERR_ARG, "null pointer");
+  std::vector<perm::Prepared> prep(count);
+  std::vector<int> status(count, PERM_OK);
+  std::vector<std::string> msg(count);
+  if (threads < 1) threads = 1;
+  threads = std::min(threads, std::max(1, count));
+  // host preparation on `threads` threads, matrices in fixed strides (each
+  // thread's status lives in its own thread-local slot; collected below)
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] {
+      for (int i = t; i < count; i += threads) {
+        if (novs[i] > perm::kMaxBatchN) {
+          status[i] = PERM_ERR_SIZE;
+          msg[i] = "batch matrices are limited to n <= " + std::to_string(perm::kMaxBatchN);
+          continue;
+        }
+        status[i] = perm::prepare_dense(dtypes[i], mats[i], novs[i], prep[i], perm::kMenuBatch);
+        if (status[i] != PERM_OK) msg[i] = perm::last_error();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (int i = 0; i < count; ++i)
+    if (status[i] != PERM_OK)
+      return perm::fail(status[i], "matrix " + std::to_string(i) + ": " + msg[i]);
+  if (first_index) *first_index = (int)B->ents.size();
+  for (int i = 0; i < count; ++i)
+    if (int s = add_prepared(B, novs[i], prep[i], nullptr)) return s;
+  return perm::ok();
+}
+
 int perm_batch_size(const perm_batch* B) { return B ? (int)B->ents.size() : 0; }
 
 int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
@@ -411,8 +544,15 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   if (!B->built)
     if (int s = build(B)) return s;
   HIP_TRYB(hipSetDevice(B->device));
+  if (B->ran && !B->graph)
+    if (int s = capture(B)) return s;
   HIP_TRYB(hipEventRecord(B->ev0, B->s[0]));
-  HIP_TRYB(hipGraphLaunch(B->graph, B->s[0]));
+  if (B->graph) {
+    HIP_TRYB(hipGraphLaunch(B->graph, B->s[0]));
+  } else {  // first run after a build: launched directly
+    if (int s = enqueue_run(B)) return s;
+    B->ran = true;
+  }
   HIP_TRYB(hipEventRecord(B->ev1, B->s[0]));
   HIP_TRYB(hipStreamSynchronize(B->s[0]));
   float ms = 0.f;
@@ -432,19 +572,37 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   // in the more accurate mode over the same items: double-double products
   // for integer matrices (same walk shape), double-double X for the others
   // (plain walk); one persistent kernel per matrix on the batch's streams,
-  // then the per-matrix reduction again.  At most two rounds: the mode only
-  // increases.
-  for (int round = 0; round < 2; ++round) {
+  // then the per-matrix reduction again.  At most three rounds: a sampled
+  // measurement, then passes whose mode only increases.
+  // A non-integer matrix of many items first has its FP64 error measured on
+  // a sample of its items (verify_batch); it gets the full pass only if that
+  // does not certify it.
+  std::vector<char> sampled(B->ents.size(), 0);
+  for (int round = 0; round < 3; ++round) {
     std::vector<std::pair<size_t, int>> esc;
+    std::vector<size_t> sample;
     for (size_t m = 0; m < B->ents.size(); ++m) {
-      if (B->ents[m].zero) continue;
+      const Entry& e = B->ents[m];
+      if (e.zero) continue;
       const int md = perm::escalation_mode(results[m]);
-      if (md >= 0) esc.push_back({m, md});
+      if (md < 0) continue;
+      if (md == 2 && results[m].mode == 0 && !sampled[m] && e.item_count >= perm::kVerifyMinUnits) {
+        sample.push_back(m);
+        sampled[m] = 1;
+      } else {
+        esc.push_back({m, md});
+      }
     }
-    if (esc.empty()) break;
+    if (esc.empty() && sample.empty()) break;
     float pass_ms = 0.f;
-    if (int s = escalate(B, esc, results, pass_ms)) return s;
-    total_ms += pass_ms;
+    if (!sample.empty()) {
+      if (int s = verify_batch(B, sample, results, pass_ms)) return s;
+      total_ms += pass_ms;
+    }
+    if (!esc.empty()) {
+      if (int s = escalate(B, esc, results, pass_ms)) return s;
+      total_ms += pass_ms;
+    }
   }
   if (device_ms) *device_ms = total_ms;
   for (size_t m = 0; m < B->ents.size(); ++m) results[m].kernel_ms = total_ms;

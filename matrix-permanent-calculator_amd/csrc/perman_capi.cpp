@@ -69,6 +69,7 @@ struct perm_plan {
   double product_bits = 1e9;  // integer matrices: log2 bound of the walk's product numerators
   bool zero = false;      // no perfect matching: the permanent is exactly 0
   bool borrowed = false;  // stream/events/memory belong to the device cache
+  uint32_t* ids = nullptr;  // device: room for kVerifyUnits workgroup indices
 };
 
 namespace {
@@ -97,18 +98,21 @@ DeviceCache& device_cache(int dev) {
 
 // Device layout of a plan: column store, x0, partial records, result record.
 struct PlanLayout {
-  size_t col, x, part, res;
-  size_t total() const { return col + x + part + res; }
+  size_t col, x, part, res, ids;
+  size_t total() const { return col + x + part + res + ids; }
 };
+
+using perm::kVerifyUnits;
 
 uint64_t blocks_of(uint64_t lanes) { return (lanes + 255) / 256; }
 
 PlanLayout plan_layout(const perm_plan* P, size_t col_doubles) {
   PlanLayout l;
   l.col = align_up(col_doubles * sizeof(double), 256);
-  l.x = align_up((size_t)P->n * sizeof(double), 256);
+  l.x = align_up(2 * (size_t)P->n * sizeof(double), 256);  // start vector and its low parts
   l.part = align_up(blocks_of(1ull << P->high_bits) * sizeof(perm::Partial), 256);
   l.res = align_up(sizeof(perm::Partial), 256);
+  l.ids = align_up(kVerifyUnits * sizeof(uint32_t), 256);
   return l;
 }
 
@@ -131,6 +135,7 @@ void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
   P->L.x0 = (const double*)(b + l.col);
   P->L.partials = (perm::Partial*)(b + l.col + l.x);
   P->L.result = (perm::Partial*)(b + l.col + l.x + l.part);
+  P->ids = (uint32_t*)(b + l.col + l.x + l.part + l.res);
 }
 
 // The FP64 kernel's walk shape from the host preparation (0: not restricted).
@@ -249,6 +254,66 @@ void fill_result(const perm_plan* P, const perm::Partial& r, int mode, perm_resu
   out->certified = perm::is_certified(*out) ? 1 : 0;
 }
 
+// Sampled verification of an FP64 result of a non-integer matrix whose
+// model estimate (perm::error_estimate) misses the accuracy target: the
+// error is measured instead of modelled.  The FP64 pass left one record
+// per workgroup; the error of the sum is the sum of the workgroups' errors,
+// independent from one workgroup to the next (each has its own roundings;
+// the shared x0 rounding is stated separately).  kVerifyUnits workgroups
+// are chosen with probability proportional to their modelled variance
+// z_i = Q_i + (2^(w-1) + h) Qr_i (systematic sampling; a workgroup whose
+// share exceeds one sampling interval is taken with certainty) and walked
+// again in mode 2 (double-double X and products: no drift), which gives
+// their FP64 errors E_i exactly.  V = sum over certain workgroups of E_i^2
+// + (Z_rest / K_rest) sum over sampled ones of E_i^2 / z_i estimates
+// sum_i E_i^2 without bias; err_est = 3 |scale| sqrt(V (1 + 3 sqrt(2 /
+// K_rest))) (a margin for the sampling spread) + 2 sqrt(n) u |value|.
+// The value itself is unchanged.  units_out: workgroups recomputed; ms_out:
+// device time.  Returns a PERM status; out->err_est / certified updated.
+int verify_sampled(perm_plan* P, perm_result* out, uint32_t& units_out, float& ms_out) {
+  units_out = 0;
+  ms_out = 0.f;
+  const uint64_t blocks = P->L.total_blocks;
+  std::vector<perm::Partial> rec(blocks);
+  HIP_TRY(hipMemcpyAsync(rec.data(), P->L.partials, blocks * sizeof(perm::Partial),
+                         hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  std::vector<double> z(blocks);
+  for (uint64_t i = 0; i < blocks; ++i)
+    z[i] = perm::unit_weight(rec[i].sq_sum, rec[i].sq_run, P->walk_bits, P->high_bits);
+  std::vector<uint32_t> ids;
+  std::vector<double> weight;
+  uint32_t certain = 0;
+  perm::pps_sample(z, perm::kVerifyUnits, ids, weight, certain);
+  const uint32_t count = (uint32_t)ids.size();
+  if (count == 0) return PERM_OK;  // nothing to measure
+  // walk the chosen workgroups in mode 2; their records go to partials[0, count)
+  HIP_TRY(hipMemcpyAsync(P->ids, ids.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         P->stream));
+  perm::DenseLaunch L = P->L;
+  L.mode = 2;
+  L.nu = L.nu1 = L.nu2 = P->n;
+  L.main.block_ids = P->ids;
+  L.main.block_count = count;
+  HIP_TRY(hipEventRecord(P->ev0, P->stream));
+  HIP_TRY(perm::launch_dense_pass_only(L, P->stream));
+  HIP_TRY(hipEventRecord(P->ev1, P->stream));
+  std::vector<perm::Partial> ref(count);
+  HIP_TRY(hipMemcpyAsync(ref.data(), P->L.partials, count * sizeof(perm::Partial),
+                         hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  HIP_TRY(hipEventElapsedTime(&ms_out, P->ev0, P->ev1));
+  std::vector<double> e(count);
+  for (uint32_t k = 0; k < count; ++k) {
+    const perm::Partial& f = rec[ids[k]];
+    e[k] = (f.hi - ref[k].hi) + (f.lo - ref[k].lo);
+  }
+  out->err_est = perm::sampled_error(e, weight, certain, P->n, out->value);
+  out->certified = perm::is_certified(*out) ? 1 : 0;
+  units_out = count;
+  return PERM_OK;
+}
+
 // mode < 0 (automatic): a matrix without a perfect matching is exactly 0
 // (no pass); otherwise the FP64 pass, then, while the result misses the
 // accuracy target (perm::escalation_mode), a pass in the more accurate mode:
@@ -269,9 +334,20 @@ int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
   const int first = mode < 0 ? 0 : mode;
   if (int s = run_pass(P, first, r, ms, first_main_ms)) return s;
   fill_result(P, r, first, out);
+  const double clock_ghz = 0.1 * r.pad / (double)std::max<uint64_t>(1, P->L.total_blocks);
   double total_ms = ms;
   const double first_subsets = (double)P->L.main.lanes * std::ldexp(1.0, P->L.main.walk_bits);
+  uint32_t verified = 0;
   if (mode < 0) {
+    // a non-integer result over many workgroups: measure its error on a
+    // sample first (verify_sampled); the full double-double X pass only if
+    // that does not certify it
+    if (first == 0 && !P->integral && perm::escalation_mode(*out) == 2 &&
+        P->L.total_blocks >= perm::kVerifyMinUnits) {
+      float vms = 0.f;
+      if (int s = verify_sampled(P, out, verified, vms)) return s;
+      total_ms += vms;
+    }
     // at most two more passes: the mode strictly increases
     for (int next = perm::escalation_mode(*out); next >= 0; next = perm::escalation_mode(*out)) {
       if (int s = run_pass(P, next, r, ms, main_ms)) return s;
@@ -279,9 +355,11 @@ int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
       total_ms += ms;
     }
   }
+  out->verified_units = (int)verified;
   out->kernel_ms = total_ms;
   out->main_ms = first_main_ms;
   out->main_subsets = first_subsets;
+  out->clock_ghz = clock_ghz;
   return ok();
 }
 
@@ -479,7 +557,7 @@ int run_prepared(int n, const Prepared& p, int mode, perm_result* out) {
 // host code the name `scratch` is only a stand-in for the symbol: its
 // device address on each GPU comes from hipGetSymbolAddress (kept in
 // NoallocDevice::ptr) and only that address is ever passed to a kernel or
-// a copy.  4 MiB fit the walk of every n <= 37 (2^24 lanes, 2^16 partial
+// a copy.  4 MiB fit the walk of every n <= 35 (2^24 lanes, 2^16 partial
 // records of 48 bytes); larger n are rejected with PERM_ERR_SIZE.
 constexpr size_t kScratchBytes = (size_t)4 << 20;
 static __device__ __align__(256) char scratch[kScratchBytes];
@@ -539,6 +617,44 @@ int perm_plan_sparse(int dtype, const void* mat, const int* cptrs, const int* ro
   if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p)) return s;
   return make_plan(nov, p, plan);
 }
+int perm_plan_geometry(const perm_plan* P, int* walk_bits, int* high_bits,
+                       unsigned long long* blocks) {
+  if (!P) return fail(PERM_ERR_ARG, "null plan");
+  if (walk_bits) *walk_bits = P->walk_bits;
+  if (high_bits) *high_bits = P->high_bits;
+  if (blocks) *blocks = blocks_of(1ull << P->high_bits);
+  return ok();
+}
+
+int perm_plan_run_blocks(perm_plan* P, int mode, unsigned long long block_begin,
+                         unsigned long long block_end, perm_result* out) {
+  if (!P || !out) return fail(PERM_ERR_ARG, "null pointer");
+  if (mode < 0 || mode > 2) return fail(PERM_ERR_ARG, "mode 0, 1 or 2");
+  const uint64_t blocks = blocks_of(1ull << P->high_bits);
+  if (block_begin >= block_end || block_end > blocks)
+    return fail(PERM_ERR_ARG, "block range outside [0, " + std::to_string(blocks) + ")");
+  HIP_TRY(hipSetDevice(P->device));
+  configure_passes(P, mode);
+  perm::DenseLaunch L = P->L;
+  L.main.block_begin = block_begin;
+  L.main.block_end = block_end;
+  HIP_TRY(hipEventRecord(P->ev0, P->stream));
+  HIP_TRY(perm::launch_dense_pass_only(L, P->stream));
+  HIP_TRY(hipEventRecord(P->ev1, P->stream));
+  HIP_TRY(perm::launch_reduce(L.partials + block_begin, block_end - block_begin, L.result,
+                              P->stream));
+  perm::Partial r;
+  HIP_TRY(hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+  fill_result(P, r, mode, out);
+  out->kernel_ms = out->main_ms = ms;
+  out->main_subsets = (double)(block_end - block_begin) * 256.0 * std::ldexp(1.0, P->walk_bits);
+  out->certified = 0;  // a part of the sum: the accuracy target applies to whole results
+  return ok();
+}
+
 int perm_plan_run(perm_plan* plan, perm_result* out) { return plan_run_impl(plan, -1, out); }
 int perm_plan_run_mode(perm_plan* plan, int mode, perm_result* out) {
   return plan_run_impl(plan, mode, out);
@@ -596,7 +712,7 @@ int perm_dense_noalloc(int dtype, const void* mat, int nov, int algo_id, perm_re
   if (need > kScratchBytes)
     return fail(PERM_ERR_SIZE, "nov=" + std::to_string(nov) + " needs " + std::to_string(need) +
                                    " bytes of device scratch; perm_dense_noalloc has " +
-                                   std::to_string(kScratchBytes) + " (n <= 37)");
+                                   std::to_string(kScratchBytes) + " (n <= 35)");
   bind_plan_memory(&P, D.ptr, p.cs.size());
   // pageable sources: plan_run_impl waits for the stream before returning,
   // and p outlives the call

@@ -205,8 +205,9 @@ void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector
     if (bc[l] >= 0) cp.push_back(bc[l]);
   for (int k = 0; k < B; ++k)
     if (k != bc[0] && k != bc[1] && k != bc[2]) cp.push_back(k);
-  std::vector<double> x2(n), c2(cols.size());
+  std::vector<double> x2(x0.size()), c2(cols.size());
   for (int j = 0; j < n; ++j) x2[j] = x0[rp[j]];
+  for (int j = 0; j + n < (int)x0.size(); ++j) x2[n + j] = x0[n + rp[j]];  // low parts
   for (int k = 0; k < B; ++k)
     for (int j = 0; j < n; ++j) c2[(size_t)k * n + j] = cols[(size_t)cp[k] * n + rp[j]];
   x0.swap(x2);
@@ -280,6 +281,63 @@ int escalation_mode(const perm_result& o) {
   return -1;
 }
 
+void pps_sample(const std::vector<double>& z, uint32_t K, std::vector<uint32_t>& ids,
+                std::vector<double>& weight, uint32_t& certain) {
+  ids.clear();
+  weight.clear();
+  certain = 0;
+  const uint64_t units = z.size();
+  double Z = 0.0;
+  for (double v : z) Z += v;
+  if (!(Z > 0.0) || units == 0) return;
+  K = (uint32_t)std::min<uint64_t>(K, units);
+  std::vector<uint32_t> order(units);
+  for (uint64_t i = 0; i < units; ++i) order[i] = (uint32_t)i;
+  std::partial_sort(order.begin(), order.begin() + K, order.end(), [&](uint32_t a, uint32_t b) {
+    return z[a] > z[b] || (z[a] == z[b] && a < b);
+  });
+  std::vector<char> taken(units, 0);
+  double z_rest = Z;
+  uint32_t k_rest = K;
+  for (uint32_t c = 0; c < K && k_rest > 0 && z[order[c]] * k_rest >= z_rest; ++c) {
+    ids.push_back(order[c]);
+    weight.push_back(1.0);
+    taken[order[c]] = 1;
+    z_rest -= z[order[c]];
+    --k_rest;
+  }
+  certain = (uint32_t)ids.size();
+  if (k_rest == 0 || !(z_rest > 0.0)) return;
+  // systematic sampling of the rest in index order (a fixed mid-interval
+  // start, so the choice is deterministic)
+  const double step = z_rest / k_rest;
+  double next = 0.5 * step, cum = 0.0;
+  for (uint64_t i = 0; i < units && ids.size() < K; ++i) {
+    if (taken[i] || !(z[i] > 0.0)) continue;
+    cum += z[i];
+    if (cum > next) {
+      ids.push_back((uint32_t)i);
+      weight.push_back(step / z[i]);
+      while (next < cum) next += step;
+    }
+  }
+}
+
+double unit_weight(double sq_sum, double sq_run, int walk_bits, int high_bits) {
+  return sq_sum + (std::ldexp(1.0, walk_bits - 1) + (double)high_bits) * sq_run;
+}
+
+double sampled_error(const std::vector<double>& e, const std::vector<double>& weight,
+                     uint32_t certain, int n, double value) {
+  double V = 0.0;
+  for (size_t k = 0; k < e.size(); ++k) V += weight[k] * e[k] * e[k];
+  const size_t sampled = e.size() - std::min<size_t>(certain, e.size());
+  const double margin = 1.0 + (sampled > 0 ? 3.0 * std::sqrt(2.0 / (double)sampled) : 0.0);
+  const double scale = std::fabs((double)(4 * (n & 1) - 2));
+  return 3.0 * scale * std::sqrt(V * margin) +
+         2.0 * std::sqrt((double)n) * std::ldexp(std::fabs(value), -53);
+}
+
 bool integral_matrix(const std::vector<double>& a) {
   for (double v : a)
     if (!(std::fabs(v) < 2147483648.0) || v != std::nearbyint(v)) return false;
@@ -299,8 +357,8 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p, const NestMenu
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
   p.integral = integral_matrix(a);
   p.zero = !pattern_has_matching(n, a);
-  p.x0.assign(n, 0.0);
-  perm::nw_start_vector(a.data(), n, p.x0.data());
+  p.x0.assign(2 * (size_t)n, 0.0);
+  perm::nw_start_vector(a.data(), n, p.x0.data(), p.x0.data() + n);
   std::vector<double> cols((size_t)(n - 1) * n);
   for (int k = 0; k < n - 1; ++k)
     for (int j = 0; j < n; ++j) cols[(size_t)k * n + j] = a[(size_t)j * n + k];
@@ -325,8 +383,8 @@ int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
     if (rows[e] < 0 || rows[e] >= n) return fail(PERM_ERR_ARG, "row index out of range");
   std::vector<double> a;
   if (!to_double(dtype, mat, (size_t)n * n, a)) return fail(PERM_ERR_ARG, "bad dtype");
-  x0.assign(n, 0.0);
-  perm::nw_start_vector(a.data(), n, x0.data());
+  x0.assign(2 * (size_t)n, 0.0);  // start vector, then its low parts
+  perm::nw_start_vector(a.data(), n, x0.data(), x0.data() + n);
   cols.assign((size_t)n * n, 0.0);
   for (int k = 0; k < n; ++k)
     for (int e = cptrs[k]; e < cptrs[k + 1]; ++e)

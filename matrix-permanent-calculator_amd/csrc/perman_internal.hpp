@@ -29,6 +29,8 @@ int check_n(int n);
 // [0, nu), columns 0-1 rows [0, nu1), columns 0-2 rows [0, nu2); n or 0
 // where not restricted), whether every entry is an integer.
 struct Prepared {
+  // x0: start vector (n doubles) followed by its double-double low parts
+  // (n more, read by the double-double X mode): 2n doubles, uploaded as one
   std::vector<double> x0, cs;
   int nu = 0, nu1 = 0, nu2 = 0;
   bool integral = false;
@@ -54,7 +56,8 @@ int prepare_dense(int dtype, const void* mat, int n, Prepared& p,
                   const NestMenu& menu = kMenuSingle);
 int prepare_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                    const void* cvals, int n, Prepared& p, const NestMenu& menu = kMenuSingle);
-// The sparse path's inputs before any reordering: x0 from the dense matrix,
+// The sparse path's inputs before any reordering: x0 from the dense matrix
+// (2n doubles: start vector, then its low parts),
 // cols[k*n + j] = CCS value of (j, k) for the walked columns k < n-1.
 int gather_sparse(int dtype, const void* mat, const int* cptrs, const int* rows,
                   const void* cvals, int n, std::vector<double>& x0, std::vector<double>& cols,
@@ -95,6 +98,29 @@ int escalation_mode(const perm_result& o);
 void cpu_lanes(const std::vector<double>& cs, const std::vector<double>& x0, int n,
                int walk_bits, int high_bits, uint64_t l0, uint64_t l1, int threads,
                double out[5]);
+
+// Sampled error measurement (the FP64 result of a non-integer matrix whose
+// modelled estimate misses the target).  The units of a walk (workgroups,
+// batch items) have independent errors; pps_sample draws up to K of them
+// with probability proportional to their modelled variance z_i (systematic
+// sampling; a unit whose share exceeds one sampling interval is taken with
+// certainty) and returns their indices, weights (1 / inclusion probability)
+// and how many were certain.
+void pps_sample(const std::vector<double>& z, uint32_t K, std::vector<uint32_t>& ids,
+                std::vector<double>& weight, uint32_t& certain);
+// Modelled variance weight of one unit's moments (sq_sum, sq_run) for a walk
+// of 2^walk_bits steps after high_bits start additions.
+double unit_weight(double sq_sum, double sq_run, int walk_bits, int high_bits);
+// err_est from the units' measured errors e_k (FP64 sum minus the mode 2 sum
+// of the same unit): 3 |scale| sqrt(V (1 + 3 sqrt(2 / sampled))) + 2 sqrt(n)
+// u |value|, V = sum_k weight_k e_k^2 (unbiased for the sum of all units'
+// squared errors).
+double sampled_error(const std::vector<double>& e, const std::vector<double>& weight,
+                     uint32_t certain, int n, double value);
+// Units recomputed per matrix, and the fewest units for which sampling is
+// used (below that the full double-double X pass is short anyway).
+constexpr uint32_t kVerifyUnits = 256;
+constexpr uint64_t kVerifyMinUnits = 512;
 
 // Split of an n-matrix into 64-lane batch items of about 2^item_log2
 // subsets (walk_bits + high_bits = n - 1); depends on n and item_log2 only.

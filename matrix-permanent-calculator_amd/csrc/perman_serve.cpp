@@ -4,12 +4,15 @@
 // Callers submit matrices from any thread and wait on (or poll) a ticket.
 // A scheduler thread owns the GPU work:
 //
-//  * admission: a request gets a device slot for its column store and start
-//    vector (a fixed arena, so admission is bounded by device memory) while
-//    the queued work stays under a cap;
+//  * submission copies the caller's matrix and returns a ticket; a pool of
+//    prep threads prepares it (start vector, column store, walk shape);
+//  * admission: a prepared request gets a device slot for its column store
+//    and start vector (a fixed arena, so admission is bounded by device
+//    memory) while the queued work stays under a cap;
 //  * iterations: each iteration takes 64-lane items (the batch kernels'
 //    items, ryser_batch.hip) from the admitted requests, shortest remaining
-//    work first, up to a work budget of a few milliseconds, and launches one
+//    work first (requests older than max_wait_ms ahead of that, oldest
+//    first), up to a work budget of a few milliseconds, and launches one
 //    persistent kernel per variant (n and walk shape) present over several
 //    streams;
 //    up to `inflight` iterations are queued on the GPU at once, so the next
@@ -58,27 +61,49 @@ using Clock = std::chrono::steady_clock;
 
 constexpr int kMaxStreams = 8;
 constexpr int kMaxInflight = 4;
+// Request mode of the sampled verification pass (mode 2 kernels on a sample
+// of the FP64 pass's items).
+constexpr int kVerifyPass = 3;
 // FP64 VALU lane-instructions per second of one MI355X at full issue
 // (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz): converts the iteration time
 // target into a work budget.
 constexpr double kLaneInstrPerSec = 256.0 * 4 * 16 * 2.4e9;
-// Slot of one resident matrix: column store (2(n-1)+1) n and x0 (n), for
-// n <= kMaxBatchN, rounded up to 256 bytes.
+// Slot of one resident matrix: column store (2(n-1)+1) n and x0 with its
+// low parts (2n), for n <= kMaxBatchN, rounded up to 256 bytes.
 constexpr size_t kSlotDoubles =
-    ((size_t)(2 * (perm::kMaxBatchN - 1) + 1) * perm::kMaxBatchN + perm::kMaxBatchN + 31) / 32 *
-    32;
+    ((size_t)(2 * (perm::kMaxBatchN - 1) + 1) * perm::kMaxBatchN + 2 * perm::kMaxBatchN + 31) /
+    32 * 32;
+
+// A submitted matrix as the caller gave it (copied), prepared by the pool.
+struct RawMatrix {
+  int dtype = PERM_DTYPE_F64;
+  bool sparse = false;
+  std::vector<char> mat, cvals;
+  std::vector<int> cptrs, rows;
+};
 
 struct Request {
   uint64_t id = 0;
+  RawMatrix raw;  // until prepared
   int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
-  int mode = 0;  // 0 FP64 items; 1, 2 a pass in a double-double mode (escalation)
+  int mode = 0;  // 0 FP64 items; 1, 2 a pass in a double-double mode (escalation);
+                 // kVerifyPass: sampled items in mode 2 measuring the FP64 error
   perm::NestShape shape{0, 0, 0};
   bool integral = false;
   perm::Prepared prep;
   int slot = -1;
   uint32_t items = 0, issued = 0, done = 0;
+  uint32_t walk_items = 0;  // items of the whole walk (items: of the current pass)
   double item_cost = 0.0;  // lane-instructions per item (scheduling only)
   std::vector<Partial> parts;
+  // sampled verification: the FP64 pass's result and item records, the
+  // sampled item indices, their weights and how many were certain
+  bool sampled = false;
+  perm_result fp_result{};
+  std::vector<Partial> fp_parts;
+  std::vector<uint32_t> vids;
+  std::vector<double> vweight;
+  uint32_t vcertain = 0;
   Clock::time_point t_submit, t_done;
   perm_result result{};
   int status = PERM_OK;
@@ -115,7 +140,11 @@ struct perm_server {
   std::thread worker;
   std::mutex mu;
   std::condition_variable cv_submit, cv_done;
-  std::deque<Request*> pending;                  // submitted, not admitted (mu)
+  std::deque<Request*> to_prep;                  // submitted, not prepared (mu)
+  std::deque<Request*> pending;                  // prepared, not admitted (mu)
+  std::condition_variable cv_prep;
+  std::vector<std::thread> preppers;
+  int prepping = 0;                              // requests taken by a prep thread (mu)
   std::unordered_map<uint64_t, Request*> table;  // every uncollected request (mu)
   uint64_t next_id = 1;
   bool stop = false;
@@ -182,39 +211,82 @@ double dd_subset_cost(int n, perm::NestShape s, int mode) {
   return half + 3.0 * half + 8.0;
 }
 
-// All items of r are done: sum its partials in item order and state the
-// error.  A result that misses the accuracy target (perm::escalation_mode)
-// goes back to the active list for a pass in the more accurate mode over the
-// same slot -- double-double products (same walk shape) for an integer
-// matrix, double-double X (plain walk) for any other -- and this returns
-// false; otherwise the slot is freed and r is ready to publish (true).
+// Queue r again for a pass in `mode` over `items` items (the verification's
+// sample or the whole walk), its matrix still in its slot.
+void requeue(perm_server* S, Request* r, int mode, uint32_t items) {
+  r->mode = mode;
+  const int kmode = mode == kVerifyPass ? 2 : mode;
+  if (kmode == 2) r->shape = perm::NestShape{r->n, r->n, r->n};
+  r->variant = perm::batch_variant(r->n, r->shape, kmode);
+  r->items = items;
+  r->issued = r->done = 0;
+  r->parts.assign(items, Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+  r->item_cost =
+      std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape, kmode);
+  S->queued_cost += (double)r->items * r->item_cost;
+  S->unissued_cost += (double)r->items * r->item_cost;
+  S->active.push_back(r);
+  S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
+}
+
+// All items of r's current pass are done.  FP64 and double-double passes:
+// sum its partials in item order and state the error.  A non-integer result
+// of many items whose modelled estimate misses the accuracy target first has
+// its error measured on a sample of its items (a kVerifyPass in mode 2, as
+// perm::pps_sample / perm::sampled_error define it); a result that still
+// misses the target, or misses it otherwise (perm::escalation_mode), goes
+// back to the active list for a pass in the more accurate mode over the same
+// slot -- double-double products (same walk shape) for an integer matrix,
+// double-double X (plain walk) for any other -- and this returns false;
+// otherwise the slot is freed and r is ready to publish (true).
 bool finalize(perm_server* S, Request* r) {
-  perm::HostDD s = {0.0, 0.0};
-  Partial tot{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (const Partial& p : r->parts) {
-    s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
-    tot.abs_sum += p.abs_sum;
-    tot.sq_sum += p.sq_sum;
-    tot.sq_run += p.sq_run;
-  }
-  tot.hi = s.hi;
-  tot.lo = s.lo;
-  perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
-                         r->mode, tot, r->result);
-  const int next = perm::escalation_mode(r->result);
-  if (next >= 0) {
-    r->mode = next;
-    if (next == 2) r->shape = perm::NestShape{r->n, r->n, r->n};
-    r->variant = perm::batch_variant(r->n, r->shape, next);
-    r->issued = r->done = 0;
-    std::fill(r->parts.begin(), r->parts.end(), Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
-    r->item_cost =
-        std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape, next);
-    S->queued_cost += (double)r->items * r->item_cost;
-    S->unissued_cost += (double)r->items * r->item_cost;
-    S->active.push_back(r);
-    S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
-    return false;
+  if (r->mode == kVerifyPass) {
+    std::vector<double> e(r->vids.size());
+    for (size_t k = 0; k < r->vids.size(); ++k) {
+      const Partial& f = r->fp_parts[r->vids[k]];
+      e[k] = (f.hi - r->parts[k].hi) + (f.lo - r->parts[k].lo);
+    }
+    r->result = r->fp_result;
+    r->result.err_est = perm::sampled_error(e, r->vweight, r->vcertain, r->n, r->result.value);
+    r->result.certified = perm::is_certified(r->result) ? 1 : 0;
+    r->fp_parts.clear();
+    r->fp_parts.shrink_to_fit();
+    if (!r->result.certified) {
+      requeue(S, r, 2, r->walk_items);
+      return false;
+    }
+  } else {
+    perm::HostDD s = {0.0, 0.0};
+    Partial tot{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (const Partial& p : r->parts) {
+      s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
+      tot.abs_sum += p.abs_sum;
+      tot.sq_sum += p.sq_sum;
+      tot.sq_run += p.sq_run;
+    }
+    tot.hi = s.hi;
+    tot.lo = s.lo;
+    perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
+                           r->mode, tot, r->result);
+    const int next = perm::escalation_mode(r->result);
+    if (next == 2 && r->mode == 0 && !r->sampled && r->walk_items >= perm::kVerifyMinUnits) {
+      std::vector<double> z(r->parts.size());
+      for (size_t i = 0; i < z.size(); ++i)
+        z[i] = perm::unit_weight(r->parts[i].sq_sum, r->parts[i].sq_run, r->walk_bits,
+                                 r->high_bits);
+      perm::pps_sample(z, perm::kVerifyUnits, r->vids, r->vweight, r->vcertain);
+      if (!r->vids.empty()) {
+        r->sampled = true;
+        r->fp_result = r->result;
+        r->fp_parts.swap(r->parts);
+        requeue(S, r, kVerifyPass, (uint32_t)r->vids.size());
+        return false;
+      }
+    }
+    if (next >= 0) {
+      requeue(S, r, next, r->walk_items);
+      return false;
+    }
   }
   r->parts.clear();
   r->parts.shrink_to_fit();
@@ -310,7 +382,15 @@ int issue(perm_server* S, bool& issued) {
     return PERM_OK;
   }
   S->unissued_cost = work;
-  std::sort(cand.begin(), cand.end(), [](const Request* a, const Request* b) {
+  // shortest remaining work first, except that requests waiting longer than
+  // max_wait_ms go ahead, oldest first (no large request starves behind a
+  // stream of small ones)
+  const Clock::time_point aged_before =
+      Clock::now() - std::chrono::microseconds((int64_t)(S->cfg.max_wait_ms * 1e3));
+  std::sort(cand.begin(), cand.end(), [&](const Request* a, const Request* b) {
+    const bool oa = a->t_submit < aged_before, ob = b->t_submit < aged_before;
+    if (oa != ob) return oa;
+    if (oa) return a->id < b->id;
     const double ca = a->remaining_cost(), cb = b->remaining_cost();
     return ca != cb ? ca < cb : a->id < b->id;
   });
@@ -350,8 +430,11 @@ int issue(perm_server* S, bool& issued) {
     I.h_mats[m] = {S->d_arena + (size_t)r->slot * kSlotDoubles,
                    S->d_arena + (size_t)r->slot * kSlotDoubles + r->prep.cs.size(),
                    1ull << r->high_bits, r->walk_bits, r->high_bits, r->variant, 0u, 0u, 0u};
-    for (uint32_t i = 0; i < g.count; ++i)
-      h_items[pos + i] = {(uint32_t)m, 0u, (uint64_t)(g.first + i) * perm::kItemLanes};
+    // the item's index in the walk (a verification pass runs the sampled ones)
+    for (uint32_t i = 0; i < g.count; ++i) {
+      const uint64_t item = r->mode == kVerifyPass ? r->vids[g.first + i] : g.first + i;
+      h_items[pos + i] = {(uint32_t)m, 0u, item * perm::kItemLanes};
+    }
     if (groups.empty() || groups.back().vid != r->variant)
       groups.push_back({r->variant, pos, pos + g.count});
     else
@@ -491,7 +574,9 @@ void scheduler_loop(perm_server* S) {
     }
     if (idle) {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (S->stop && S->pending.empty() && S->active.empty()) return;
+      if (S->stop && S->to_prep.empty() && S->prepping == 0 && S->pending.empty() &&
+          S->active.empty())
+        return;
       S->cv_submit.wait(lk, [&] { return S->stop || !S->pending.empty(); });
     }
   }
@@ -503,66 +588,135 @@ void scheduler_main(perm_server* S) {
   scheduler_loop(S);
 }
 
-int submit(perm_server* S, int n, perm::Prepared& p, unsigned long long* id) {
-  auto* r = new Request();
-  r->n = n;
+size_t dtype_size(int dtype) { return dtype == PERM_DTYPE_F64 ? 8 : 4; }
+
+// Prepare r from its raw input (a prep thread, no lock held): start vector,
+// column store, split, walk shape, variant.  Returns PERM_OK or an error
+// status (r->error set).
+int prepare_request(perm_server* S, Request* r) {
+  const RawMatrix& in = r->raw;
+  const int n = r->n;
+  perm::Prepared p;
+  int st = in.sparse ? perm::prepare_sparse(in.dtype, in.mat.data(), in.cptrs.data(),
+                                            in.rows.data(), in.cvals.data(), n, p,
+                                            perm::kMenuBatch)
+                     : perm::prepare_dense(in.dtype, in.mat.data(), n, p, perm::kMenuBatch);
+  if (st != PERM_OK) {
+    r->error = perm::last_error();
+    return st;
+  }
   r->integral = p.integral;
   perm::batch_split(n, S->cfg.item_log2, r->walk_bits, r->high_bits);
   const uint64_t lanes = 1ull << r->high_bits;
   r->items = (uint32_t)((lanes + perm::kItemLanes - 1) / perm::kItemLanes);
+  r->walk_items = r->items;
   // pair / nested kernels for requests with real work; tiny ones use the
   // plain variant (fewer kernels per iteration)
   r->shape = r->items >= 8 ? perm::batch_shape(n, {p.nu > 0 ? p.nu : n, p.nu1 > 0 ? p.nu1 : n,
                                                    p.nu2 > 0 ? p.nu2 : n})
                            : perm::NestShape{n, n, n};
   r->variant = perm::batch_variant(n, r->shape);
-  if (r->variant < 0) {
-    delete r;
-    return perm::fail(PERM_ERR_ARG, "no batch kernel for this n");
+  if (r->variant < 0 || perm::batch_variant(n, r->shape, 1) < 0 ||
+      perm::batch_variant(n, {n, n, n}, 2) < 0) {
+    r->error = "no batch kernel for this n";
+    return PERM_ERR_ARG;
   }
   r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
                  (double)perm::nest_cost(n, r->shape) / 8.0;
-  if (perm::batch_variant(n, r->shape, 1) < 0 || perm::batch_variant(n, {n, n, n}, 2) < 0) {
-    delete r;
-    return perm::fail(PERM_ERR_ARG, "no double-double batch kernel for this n");
-  }
-  const bool zero = p.zero;
+  if (p.zero) perm::fill_zero_result(n, r->integral, r->result);
   r->prep = std::move(p);
-  r->t_submit = Clock::now();
-  if (zero) {  // no perfect matching: exactly 0, finished at once
-    perm::fill_zero_result(n, r->integral, r->result);
-    r->t_done = r->t_submit;
-    r->finished = true;
-    r->prep = perm::Prepared();
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (S->fatal != PERM_OK) {
-      delete r;
-      return perm::fail(S->fatal, "server failed: " + S->fatal_msg);
+  r->raw = RawMatrix();
+  return PERM_OK;
+}
+
+// Prep thread: take submitted requests, prepare them, hand them to the
+// scheduler (or finish them: exactly 0, or a preparation error).
+void prep_loop(perm_server* S) {
+  perm::set_device(S->device);
+  for (;;) {
+    Request* r;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_prep.wait(lk, [&] { return S->stop || !S->to_prep.empty(); });
+      if (S->to_prep.empty()) return;  // stopping and drained
+      r = S->to_prep.front();
+      S->to_prep.pop_front();
+      S->prepping += 1;
     }
-    r->id = S->next_id++;
-    S->table[r->id] = r;
-    S->stats.submitted += 1;
-    S->stats.completed += 1;
-    *id = r->id;
-    return perm::ok();
+    const int st = prepare_request(S, r);
+    bool done_now = false;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->prepping -= 1;
+      if (st != PERM_OK || r->prep.zero) {  // finished at once
+        if (st != PERM_OK) r->status = st;
+        r->prep = perm::Prepared();
+        r->t_done = Clock::now();
+        r->finished = true;
+        S->stats.completed += 1;
+        done_now = true;
+      } else {
+        S->pending.push_back(r);
+      }
+    }
+    if (done_now) S->cv_done.notify_all();
+    else S->cv_submit.notify_one();
   }
+}
+
+// Copy a caller's matrix into a new request (no preparation yet).
+int make_request(int dtype, const void* mat, const int* cptrs, const int* rows,
+                 const void* cvals, int nov, Request*& out) {
+  if (!mat) return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (dtype < PERM_DTYPE_I32 || dtype > PERM_DTYPE_F64) return perm::fail(PERM_ERR_ARG, "bad dtype");
+  if (nov < 1) return perm::fail(PERM_ERR_ARG, "nov must be >= 1");
+  if (nov > perm::kMaxBatchN)
+    return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
+                                         std::to_string(perm::kMaxBatchN));
+  auto* r = new Request();
+  r->n = nov;
+  RawMatrix& m = r->raw;
+  m.dtype = dtype;
+  const size_t es = dtype_size(dtype);
+  m.mat.assign((const char*)mat, (const char*)mat + (size_t)nov * nov * es);
+  if (cptrs) {
+    if (cptrs[0] != 0 || cptrs[nov] < 0 || (cptrs[nov] > 0 && (!rows || !cvals))) {
+      delete r;
+      return perm::fail(PERM_ERR_ARG, "malformed cptrs");
+    }
+    m.sparse = true;
+    const int nnz = cptrs[nov];
+    m.cptrs.assign(cptrs, cptrs + nov + 1);
+    m.rows.assign(rows, rows + nnz);
+    m.cvals.assign((const char*)cvals, (const char*)cvals + (size_t)nnz * es);
+  }
+  out = r;
+  return PERM_OK;
+}
+
+// Register requests (one lock) and wake the prep pool.
+int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* ids) {
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    if (S->fatal != PERM_OK) {
-      delete r;
-      return perm::fail(S->fatal, "server failed: " + S->fatal_msg);
+    const char* why = S->fatal != PERM_OK ? "server failed: " : S->stop ? "server is shutting down" : nullptr;
+    if (why) {
+      for (int i = 0; i < count; ++i) delete rs[i];
+      return perm::fail(S->fatal != PERM_OK ? S->fatal : PERM_ERR_ARG,
+                        std::string(why) + (S->fatal != PERM_OK ? S->fatal_msg : ""));
     }
-    if (S->stop) {
-      delete r;
-      return perm::fail(PERM_ERR_ARG, "server is shutting down");
+    const Clock::time_point now = Clock::now();
+    for (int i = 0; i < count; ++i) {
+      Request* r = rs[i];
+      r->t_submit = now;
+      r->id = S->next_id++;
+      S->table[r->id] = r;
+      S->to_prep.push_back(r);
+      ids[i] = r->id;
     }
-    r->id = S->next_id++;
-    S->table[r->id] = r;
-    S->pending.push_back(r);
-    S->stats.submitted += 1;
+    S->stats.submitted += count;
   }
-  S->cv_submit.notify_one();
-  *id = r->id;
+  if (count == 1) S->cv_prep.notify_one();
+  else S->cv_prep.notify_all();
   return perm::ok();
 }
 
@@ -672,6 +826,8 @@ void perm_server_default_config(perm_server_config* c) {
   c->iteration_us = 8000.0;
   c->max_resident = 2048;
   c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)
+  c->prep_threads = 4;
+  c->max_wait_ms = 50.0;
 }
 
 int perm_server_create(const perm_server_config* cfg, perm_server** out) {
@@ -686,6 +842,8 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
     if (cfg->iteration_us > 0) c.iteration_us = cfg->iteration_us;
     if (cfg->max_resident > 0) c.max_resident = cfg->max_resident;
     if (cfg->max_queued_work > 0) c.max_queued_work = cfg->max_queued_work;
+    if (cfg->prep_threads > 0) c.prep_threads = std::min(cfg->prep_threads, 64);
+    if (cfg->max_wait_ms > 0) c.max_wait_ms = cfg->max_wait_ms;
   }
   if (c.item_log2 < 6 || c.item_log2 > 30) return perm::fail(PERM_ERR_ARG, "item_log2 in [6, 30]");
   if (c.streams > kMaxStreams) c.streams = kMaxStreams;
@@ -702,6 +860,7 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
     return perm::fail(st, msg);
   }
   S->worker = std::thread(scheduler_main, S);
+  for (int t = 0; t < c.prep_threads; ++t) S->preppers.emplace_back(prep_loop, S);
   *out = S;
   return perm::ok();
 }
@@ -709,24 +868,43 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
 int perm_server_submit_dense(perm_server* S, int dtype, const void* mat, int nov,
                              unsigned long long* id) {
   if (!S || !id) return perm::fail(PERM_ERR_ARG, "null pointer");
-  if (nov > perm::kMaxBatchN)
-    return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
-                                         std::to_string(perm::kMaxBatchN));
-  perm::Prepared p;
-  if (int s = perm::prepare_dense(dtype, mat, nov, p, perm::kMenuBatch)) return s;
-  return submit(S, nov, p, id);
+  Request* r = nullptr;
+  if (int s = make_request(dtype, mat, nullptr, nullptr, nullptr, nov, r)) return s;
+  return enqueue(S, &r, 1, id);
+}
+
+int perm_server_submit_many(perm_server* S, int count, const int* dtypes,
+                            const void* const* mats, const int* const* cptrs,
+                            const int* const* rows, const void* const* cvals, const int* novs,
+                            unsigned long long* ids) {
+  if (!S || count < 0 || (count > 0 && (!dtypes || !mats || !novs || !ids)))
+    return perm::fail(PERM_ERR_ARG, "null pointer");
+  if (cptrs && (!rows || !cvals)) return perm::fail(PERM_ERR_ARG, "null rows/cvals arrays");
+  std::vector<Request*> rs;
+  rs.reserve(count);
+  int st = PERM_OK;
+  for (int i = 0; i < count && st == PERM_OK; ++i) {
+    Request* r = nullptr;
+    const bool sp = cptrs && cptrs[i];
+    st = make_request(dtypes[i], mats[i], sp ? cptrs[i] : nullptr, sp ? rows[i] : nullptr,
+                      sp ? cvals[i] : nullptr, novs[i], r);
+    if (st == PERM_OK) rs.push_back(r);
+  }
+  const std::string err = st != PERM_OK ? perm::last_error() : "";
+  if (!rs.empty())
+    if (int s = enqueue(S, rs.data(), (int)rs.size(), ids)) return s;
+  if (st != PERM_OK)
+    return perm::fail(st, "matrix " + std::to_string(rs.size()) + ": " + err);
+  return perm::ok();
 }
 
 int perm_server_submit_sparse(perm_server* S, int dtype, const void* mat, const int* cptrs,
                               const int* rows, const void* cvals, int nov,
                               unsigned long long* id) {
-  if (!S || !id) return perm::fail(PERM_ERR_ARG, "null pointer");
-  if (nov > perm::kMaxBatchN)
-    return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
-                                         std::to_string(perm::kMaxBatchN));
-  perm::Prepared p;
-  if (int s = perm::prepare_sparse(dtype, mat, cptrs, rows, cvals, nov, p, perm::kMenuBatch)) return s;
-  return submit(S, nov, p, id);
+  if (!S || !id || !cptrs) return perm::fail(PERM_ERR_ARG, "null pointer");
+  Request* r = nullptr;
+  if (int s = make_request(dtype, mat, cptrs, rows, cvals, nov, r)) return s;
+  return enqueue(S, &r, 1, id);
 }
 
 int perm_server_wait(perm_server* S, unsigned long long id, perm_result* out,
@@ -793,6 +971,9 @@ void perm_server_destroy(perm_server* S) {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stop = true;
   }
+  S->cv_prep.notify_all();
+  for (auto& t : S->preppers)
+    if (t.joinable()) t.join();  // every submitted request is prepared (or failed)
   S->cv_submit.notify_all();
   if (S->worker.joinable()) S->worker.join();  // drains admitted and pending work
   (void)hipSetDevice(S->device);

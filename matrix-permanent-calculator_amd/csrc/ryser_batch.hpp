@@ -17,7 +17,7 @@ constexpr int kItemLanes = 64;
 // One matrix of a batch (device copy, 48 bytes).
 struct BatchMatrix {
   const double* colpm;  // column store (see ryser_dense_impl.hpp)
-  const double* x0;     // start vector
+  const double* x0;     // start vector (n doubles), then its low parts (n more)
   uint64_t lanes;       // 2^high_bits
   int32_t walk_bits;    // subsets per lane = 2^walk_bits
   int32_t high_bits;

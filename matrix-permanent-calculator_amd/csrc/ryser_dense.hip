@@ -23,15 +23,24 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const Partial* __r
                                                               uint64_t count,
                                                               Partial* __restrict__ out) {
   __shared__ Partial scratch[256 / kWave];
+  __shared__ double clk[256];
   DD s = {0.0, 0.0};
   Moments m = {0.0, 0.0, 0.0};
+  double c = 0.0;
   for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
     s = dd_add(s, DD{in[i].hi, in[i].lo});
     m.a += in[i].abs_sum;
     m.q += in[i].sq_sum;
     m.r += in[i].sq_run;
+    c += in[i].pad;
   }
-  block_reduce_store(s, m, scratch, out);
+  clk[threadIdx.x] = c;
+  block_reduce_store(s, m, scratch, out);  // (its __syncthreads orders clk)
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < (int)blockDim.x; ++i) t += clk[i];
+    out->pad = t;
+  }
 }
 
 int dense_unroll_bits(int n) { return n - 1 < 3 ? n - 1 : 3; }

@@ -383,7 +383,9 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     x[j] = x0[j];
-    xl[j] = 0.0;
+    // kDDAll starts from the double-double start vector (low parts after the
+    // n high ones); the other modes keep X in FP64
+    xl[j] = MODE == kDDAll ? x0[N + j] : 0.0;
   }
 
   // Closed-form start: add the high columns selected by the pattern.  Bits
@@ -443,9 +445,16 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
                         Partial* __restrict__ partials, uint64_t lane_base, uint64_t lanes,
-                        int n_walk_bits, int n_high_bits) {
+                        int n_walk_bits, int n_high_bits, const uint32_t* __restrict__ block_ids) {
   __shared__ Partial scratch[256 / kWave];
-  const uint64_t lane = lane_base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // shader-clock and 100 MHz real-time counters around the workgroup's work:
+  // their ratio is the clock the workgroup ran at (scalar reads, per wave)
+  const uint64_t c0 = __builtin_amdgcn_s_memtime();
+  const uint64_t r0 = __builtin_amdgcn_s_memrealtime();
+  // workgroup b walks lanes [lane_base + 256 b, +256); b = the listed index
+  // when a list is given (the record still goes to slot blockIdx.x)
+  const uint64_t b = block_ids ? (uint64_t)block_ids[blockIdx.x] : (uint64_t)blockIdx.x;
+  const uint64_t lane = lane_base + b * blockDim.x + threadIdx.x;
   const uint64_t pattern = lane < lanes ? lane : 0ull;  // high columns fixed by the lane id
   DD s;
   Moments m;
@@ -455,6 +464,12 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
     m = {0.0, 0.0, 0.0};
   }
   block_reduce_store(s, m, scratch, partials + blockIdx.x);
+  if (threadIdx.x == 0) {
+    const uint64_t c1 = __builtin_amdgcn_s_memtime();
+    const uint64_t r1 = __builtin_amdgcn_s_memrealtime();
+    // cycles per 10 ns tick; the grid reduction sums it (pad) over workgroups
+    partials[blockIdx.x].pad = r1 > r0 ? (double)(c1 - c0) / (double)(r1 - r0) : 0.0;
+  }
 }
 
 template <int N>
@@ -467,11 +482,19 @@ constexpr int unroll_bits_for() {
 template <int N, int MODE, class S>
 hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
+  if (P.lanes == 0) return hipSuccess;
+  if (P.walk_bits < R) return hipErrorInvalidValue;
+  if (P.block_ids) {  // listed workgroups (indices checked by the caller)
+    if (P.block_count == 0) return hipSuccess;
+    ryser_dense_kernel<N, R, MODE, S><<<dim3(P.block_count), dim3(256), 0, stream>>>(
+        L.colpm, L.x0, P.partials, 0, P.lanes, P.walk_bits, P.high_bits, P.block_ids);
+    return hipGetLastError();
+  }
   const uint64_t b0 = P.block_begin, b1 = P.block_end ? P.block_end : P.blocks;
-  if (P.lanes == 0 || b1 <= b0) return hipSuccess;
-  if (P.walk_bits < R || b1 > P.blocks) return hipErrorInvalidValue;
+  if (b1 <= b0) return hipSuccess;
+  if (b1 > P.blocks) return hipErrorInvalidValue;
   ryser_dense_kernel<N, R, MODE, S><<<dim3((unsigned)(b1 - b0)), dim3(256), 0, stream>>>(
-      L.colpm, L.x0, P.partials + b0, b0 * 256, P.lanes, P.walk_bits, P.high_bits);
+      L.colpm, L.x0, P.partials + b0, b0 * 256, P.lanes, P.walk_bits, P.high_bits, nullptr);
   return hipGetLastError();
 }
 

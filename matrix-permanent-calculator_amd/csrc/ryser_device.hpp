@@ -20,7 +20,9 @@ constexpr int kWave = 64;
 // statistical error estimate: sq_sum = sum of (sum of |terms| in the run)^2
 // (an upper bound on sum term^2: the product roundings) and sq_run = sum of
 // (signed sum of the run)^2 (the X drift: it moves a run's terms together,
-// so its effect follows the run's cancelled sum).  48 bytes.
+// so its effect follows the run's cancelled sum).  pad: the dense walk kernel
+// stores its workgroup's shader cycles per 10 ns real-time tick (summed by
+// the grid reduction: mean clock = 0.1 GHz * sum / workgroups).  48 bytes.
 struct Partial {
   double hi, lo, abs_sum, sq_sum, sq_run, pad;
 };

@@ -10,7 +10,8 @@
 
 namespace perm {
 
-// Start vector of the Nijenhuis-Wilf form, x0_j = a[j][n-1] - rs_j/2 with
+// Start vector of the Nijenhuis-Wilf form (and, in x0_lo when given, the
+// low part of its double-double value), x0_j = a[j][n-1] - rs_j/2 with
 // rs_j = sum_k a[j][k] (reference: gpu_exact_dense.cu:647-654; sparse
 // gpu_exact_sparse.cu:861-871, whose loop skips zero entries and so has the
 // same row sum).  The reference accumulates rs_j in plain double; here it is
@@ -20,7 +21,7 @@ namespace perm {
 // the result by about d_j * perm / a[j][n-1] -- with the plain row sum about
 // n u |perm| in total, here about sqrt(n) u |perm|.  Integer matrices get
 // the exact x0 either way.
-inline void nw_start_vector(const double* a, int n, double* x0) {
+inline void nw_start_vector(const double* a, int n, double* x0, double* x0_lo = nullptr) {
   for (int j = 0; j < n; ++j) {
     double hi = 0.0, lo = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -37,6 +38,9 @@ inline void nw_start_vector(const double* a, int n, double* x0) {
     const double bb = s - c;
     const double e = (c - (s - bb)) + (h - bb);
     x0[j] = s + (e - 0.5 * lo);
+    // the remainder, for kernels that keep X in double-double (mode 2): x0 +
+    // x0_lo is the start value to about u^2 (the row sum's compensation error)
+    if (x0_lo) x0_lo[j] = (s - x0[j]) + (e - 0.5 * lo);
   }
 }
 
@@ -61,10 +65,12 @@ inline void build_column_store(int n, ColFn col, std::vector<double>& out) {
 // and the drift of all lanes moves the result the same way, scaled by the
 // lane sums -- so shorter walks give a more accurate sum.  Measured at
 // n = 40 (FP64 against the double-double mode): 2^18-step walks 2.3e-10,
-// 2^12-step walks 6e-12 relative error, in the same time.  Default:
+// 2^12-step walks 6e-12 relative error, in the same time; the drift term of
+// the error estimate grows with the walk (2^(w-1) steps on average), and
+// 2^10-step walks halve it against 2^12 for ~2 % more lane starts.  Default:
 //   n <= 32: lanes = 2^min(n-1-R, max(n-11, 18)) (at most 2^21): walks of
 //            2^R steps up to n = 22, 2^10 steps from n = 29 on;
-//   n >= 33: 2^12 steps per lane, at most 2^28 lanes (2^20 workgroup
+//   n >= 33: 2^10 steps per lane, at most 2^30 lanes (2^22 workgroup
 //            partials), longer walks beyond n = 41.
 struct Split {
   int walk_bits;
@@ -82,8 +88,8 @@ inline Split choose_split(int n, int unroll_bits, int walk_override) {
     if (h > 21) h = 21;
     if (h < 0) h = 0;
   } else {
-    h = B - 12;
-    if (h > 28) h = 28;
+    h = B - 10;
+    if (h > 30) h = 30;
   }
   int w = B - h;
   if (walk_override > 0) {
@@ -126,7 +132,7 @@ inline Split choose_skip_split(int n, int unroll_bits, int walk_override) {
 //    grid sums are double-double);
 //  * rounding of the start vector x0 (non-integer matrices): n u |value|,
 //    every row's half-ulp error moving the result the same way (see
-//    nw_start_vector).
+//    nw_start_vector); mode 2 starts from the double-double x0 (n u^2).
 // With double-double products (mode 1) the first two terms shrink to
 // (n + 8 + 2^(w+1)) 2^-104 A; mode 2 (X in double-double too) is bounded by
 // four times that.
@@ -139,7 +145,8 @@ inline Split choose_skip_split(int n, int unroll_bits, int walk_override) {
 inline double error_statement(double abs_sum_scaled, double value, int n, int walk_bits,
                               int unroll_bits, bool integral, int mode) {
   const double u = std::ldexp(1.0, -53);
-  const double start = integral ? 0.0 : n * u * std::fabs(value);
+  // mode 2 starts from x0 in double-double (error ~u^2)
+  const double start = integral ? 0.0 : (mode == 2 ? n * u * u : n * u) * std::fabs(value);
   const double dd_terms = std::ldexp((double)n + 8.0 + std::ldexp(2.0, walk_bits), -104);
   if (mode == 2) return abs_sum_scaled * 4.0 * dd_terms + start;
   if (mode == 1) return abs_sum_scaled * dd_terms + start;
@@ -177,7 +184,7 @@ inline double error_estimate(double sq_scaled, double run_scaled, double value, 
   const double rms = std::sqrt(sq_scaled);
   const double rn = std::sqrt((double)n);
   const double start = integral ? 0.0 : 2.0 * rn * u * std::fabs(value);
-  if (mode == 2) return 3.0 * dd * rms + start;
+  if (mode == 2) return 3.0 * dd * rms + (integral ? 0.0 : 2.0 * rn * u * u * std::fabs(value));
   if (mode == 1) {
     const double drift =
         integral ? 0.0 : rn * std::sqrt(std::ldexp(1.0, walk_bits) + (double)high_bits) * u;

@@ -18,12 +18,16 @@ struct LanePass {
   int high_bits;        // walk_bits + high_bits = n - 1
   Partial* partials;    // device: `blocks` records
   uint64_t block_begin = 0, block_end = 0;  // workgroups to launch (end 0: all)
+  // Instead of a range: the listed workgroups (device array of block_count
+  // indices); the record of the i-th listed workgroup goes to partials[i].
+  const uint32_t* block_ids = nullptr;
+  uint32_t block_count = 0;
 };
 
 // One dense launch over all 2^(n-1) subsets of one matrix.
 struct DenseLaunch {
   const double* colpm;  // device: 2(n-1)+1 blocks of n doubles, [2k]=+a_k, [2k+1]=-a_k, zeros
-  const double* x0;     // device: n doubles, Nijenhuis-Wilf start vector
+  const double* x0;     // device: 2n doubles, Nijenhuis-Wilf start vector, then its low parts
   Partial* partials;    // device: base of all partial records (reduced in order)
   Partial* result;      // device: 1 record (grid sum)
   uint64_t total_blocks;  // partial records written by the passes below

@@ -41,7 +41,8 @@ class PermResultC(ctypes.Structure):
                 ("mode", ctypes.c_int), ("integral", ctypes.c_int), ("err_est", ctypes.c_double),
                 ("nu", ctypes.c_int), ("main_ms", ctypes.c_double),
                 ("main_subsets", ctypes.c_double), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
-                ("certified", ctypes.c_int), ("zero_pattern", ctypes.c_int)]
+                ("certified", ctypes.c_int), ("zero_pattern", ctypes.c_int),
+                ("clock_ghz", ctypes.c_double), ("verified_units", ctypes.c_int)]
 
 
 @dataclass
@@ -64,6 +65,8 @@ class PermResult:
     nu2: int = 0
     certified: int = 0
     zero_pattern: int = 0
+    clock_ghz: float = 0.0
+    verified_units: int = 0
 
     @property
     def stated(self) -> float:
@@ -76,7 +79,8 @@ class PermResult:
     def from_c(cls, r: PermResultC) -> "PermResult":
         return cls(r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.err_bound, r.kernel_ms, r.subsets,
                    r.n, r.mode, r.integral, r.err_est, r.nu, r.main_ms,
-                   r.main_subsets, r.nu1, r.nu2, r.certified, r.zero_pattern)
+                   r.main_subsets, r.nu1, r.nu2, r.certified, r.zero_pattern, r.clock_ghz,
+                   r.verified_units)
 
 
 class BatchTuningC(ctypes.Structure):
@@ -89,7 +93,8 @@ class ServerConfigC(ctypes.Structure):
     _fields_ = [("item_log2", ctypes.c_int), ("streams", ctypes.c_int),
                 ("inflight", ctypes.c_int), ("max_items", ctypes.c_int),
                 ("iteration_us", ctypes.c_double), ("max_resident", ctypes.c_int),
-                ("max_queued_work", ctypes.c_double)]
+                ("max_queued_work", ctypes.c_double), ("prep_threads", ctypes.c_int),
+                ("max_wait_ms", ctypes.c_double)]
 
 
 class ServerStatsC(ctypes.Structure):
@@ -104,6 +109,18 @@ class ServerStatsC(ctypes.Structure):
 PERM_PENDING = 5
 
 _LIB = None
+
+
+def _bind(L, name: str, argtypes, restype=ctypes.c_int):
+    """Declare an entry point's signature; an entry point this build of the
+    library lacks stays undeclared and fails when called (AttributeError),
+    the others keep working."""
+    try:
+        f = getattr(L, name)
+    except AttributeError:
+        return
+    f.argtypes = argtypes
+    f.restype = restype
 
 
 def lib():
@@ -122,13 +139,16 @@ def lib():
                                  ctypes.POINTER(PermResultC)]
         L.perm_sparse.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(PermResultC)]
-        L.perm_dense_noalloc.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
-                                         ctypes.POINTER(PermResultC)]
+        _bind(L, "perm_dense_noalloc", [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(PermResultC)])
         L.perm_plan_dense.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(vp)]
         L.perm_plan_sparse.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int,
                                        ctypes.POINTER(vp)]
         L.perm_plan_run.argtypes = [vp, ctypes.POINTER(PermResultC)]
         L.perm_plan_run_mode.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PermResultC)]
+        _bind(L, "perm_plan_geometry", [vp, ip, ip, ctypes.POINTER(ctypes.c_ulonglong)])
+        _bind(L, "perm_plan_run_blocks", [vp, ctypes.c_int, ctypes.c_ulonglong,
+                                          ctypes.c_ulonglong, ctypes.POINTER(PermResultC)])
         L.perm_plan_destroy.argtypes = [vp]
         L.perm_plan_destroy.restype = None
         for v in DENSE_VARIANTS:
@@ -145,6 +165,9 @@ def lib():
         L.perm_batch_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.perm_batch_add_dense.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ip]
         L.perm_batch_add_sparse.argtypes = [vp, ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ip]
+        _bind(L, "perm_batch_add_dense_many", [vp, ctypes.c_int, ip,
+                                               ctypes.POINTER(ctypes.c_void_p), ip,
+                                               ctypes.c_int, ip])
         L.perm_batch_size.argtypes = [vp]
         L.perm_batch_run.argtypes = [vp, ctypes.POINTER(PermResultC),
                                      ctypes.POINTER(ctypes.c_double)]
@@ -159,6 +182,9 @@ def lib():
         L.perm_server_submit_dense.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, u64p]
         L.perm_server_submit_sparse.argtypes = [vp, ctypes.c_int, vp, ip, ip, vp, ctypes.c_int,
                                                 u64p]
+        ipp = ctypes.POINTER(ip)
+        _bind(L, "perm_server_submit_many", [vp, ctypes.c_int, ip, ctypes.POINTER(vp), ipp, ipp,
+                                             ctypes.POINTER(vp), ip, u64p])
         L.perm_server_wait.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
         L.perm_server_poll.argtypes = [vp, ctypes.c_ulonglong, ctypes.POINTER(PermResultC), dp]
         L.perm_server_wait_many.argtypes = [vp, u64p, ctypes.c_int, ctypes.POINTER(PermResultC),
@@ -303,6 +329,20 @@ class Plan:
             _check(lib().perm_plan_run_mode(self._h, mode, ctypes.byref(r)))
         return PermResult.from_c(r)
 
+    def geometry(self) -> tuple:
+        """(walk_bits, high_bits, blocks) of the plan's walk split."""
+        w, h, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_ulonglong()
+        _check(lib().perm_plan_geometry(self._h, ctypes.byref(w), ctypes.byref(h),
+                                        ctypes.byref(b)))
+        return w.value, h.value, b.value
+
+    def run_blocks(self, block_begin: int, block_end: int, mode: int = 0) -> PermResult:
+        """The sum over workgroups [block_begin, block_end) in `mode`."""
+        r = PermResultC()
+        _check(lib().perm_plan_run_blocks(self._h, mode, block_begin, block_end,
+                                          ctypes.byref(r)))
+        return PermResult.from_c(r)
+
     def close(self):
         if self._h:
             lib().perm_plan_destroy(self._h)
@@ -339,6 +379,19 @@ class Batch:
                                             ctypes.byref(idx))
         _check(st)
         return idx.value
+
+    def add_many(self, mats, threads: int = 8) -> int:
+        """Add dense matrices in one call (host preparation on `threads`
+        threads); returns the index of the first."""
+        arrs = [_mat(m) for m in mats]  # the library copies what it keeps
+        k = len(arrs)
+        dts = np.array([dt for _, dt in arrs], dtype=np.int32)
+        ns = np.array([a.shape[0] for a, _ in arrs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * k)(*[a.ctypes.data for a, _ in arrs])
+        first = ctypes.c_int()
+        _check(lib().perm_batch_add_dense_many(self._h, k, _ip(dts), ptrs, _ip(ns), threads,
+                                               ctypes.byref(first)))
+        return first.value
 
     def set(self, item_log2: int = 0, streams: int = 0, max_grid: int = 0):
         _check(lib().perm_batch_set(self._h, item_log2, streams, max_grid))
@@ -514,6 +567,22 @@ class Server:
                                                 ctypes.byref(tid))
         _check(st)  # the matrix is copied before submit returns
         return tid.value
+
+    def submit_many(self, reqs) -> list:
+        """Submit prepared Requests in one call (perm_server_submit_many);
+        returns their tickets."""
+        k = len(reqs)
+        vp = ctypes.c_void_p
+        dts = (ctypes.c_int * k)(*[q.dt for q in reqs])
+        ns = (ctypes.c_int * k)(*[q.n for q in reqs])
+        mats = (vp * k)(*[q.a.ctypes.data for q in reqs])
+        ip = ctypes.POINTER(ctypes.c_int)
+        cps = (ip * k)(*[_ip(q.cptrs) if q.sparse else None for q in reqs])
+        rws = (ip * k)(*[_ip(q.rows) if q.sparse else None for q in reqs])
+        cvs = (vp * k)(*[q.cvals.ctypes.data if q.sparse else None for q in reqs])
+        ids = (ctypes.c_ulonglong * k)()
+        _check(lib().perm_server_submit_many(self._h, k, dts, mats, cps, rws, cvs, ns, ids))
+        return list(ids)
 
     def submit_request(self, req: Request) -> int:
         tid = ctypes.c_ulonglong()

@@ -70,6 +70,9 @@ def lib():
             f = getattr(L, f"orc_gpu_dense_floatx_{sfx}")
             f.restype = ctypes.c_double
             f.argtypes = [ptr, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_lane_range_dd.restype = None
+        L.orc_lane_range_dd.argtypes = [_dp, ctypes.c_int, ctypes.c_int, c_i64, c_i64,
+                                        ctypes.c_int, _dp, _dp]
         L.orc_ryser_u128.restype = None
         L.orc_ryser_u128.argtypes = [ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
@@ -314,6 +317,18 @@ def exact_permanent(mat) -> Fraction:
     if x > prod // 2:
         x -= prod
     return Fraction(x, 1 << e)
+
+
+def lane_range_dd(mat, walk_bits: int, lane_begin: int, lane_end: int,
+                  threads: int = 8) -> tuple[float, float]:
+    """Double-double sum (no scale) over the subsets of lanes
+    [lane_begin, lane_end) of the GPU walk's aligned split in natural column
+    order (orc_lane_range_dd)."""
+    a = np.ascontiguousarray(mat, dtype=np.float64)
+    hi, lo = ctypes.c_double(), ctypes.c_double()
+    lib().orc_lane_range_dd(_ptr(a, ctypes.c_double), a.shape[0], walk_bits, lane_begin,
+                            lane_end, threads, ctypes.byref(hi), ctypes.byref(lo))
+    return hi.value, lo.value
 
 
 def exact_integer_permanent(mat, threads: int = 8) -> int:

@@ -20,6 +20,8 @@
  *                          wrappers :401-699), evaluated thread by thread on the CPU
  *   orc_ryser_mod          exact Ryser modulo a prime (exact oracle building block)
  *   orc_ryser_dd           double-double Ryser (accurate oracle, n <= ~30)
+ *   orc_lane_range_dd      double-double sum over the subsets of a range of
+ *                          lanes of the GPU walk's aligned split (natural order)
  *   orc_ryser_u128         exact Ryser of an integer matrix in 128-bit
  *                          wrap-around arithmetic (exact when |perm| < 2^127)
  *
@@ -579,4 +581,62 @@ void orc_ryser_u128(const i64* a, int n, int threads, uint64_t* out_lo, uint64_t
   if (n & 1) sum = (u128)0 - sum; /* (-1)^n */
   *out_lo = (uint64_t)sum;
   *out_hi = (uint64_t)(sum >> 64);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Double-double sum over the subsets the GPU walk assigns to lanes
+ * [lane_begin, lane_end) of its aligned split, in natural column order: lane
+ * l fixes columns w..n-2 by the bits of l and walks columns 0..w-1 in Gray
+ * order; the sum is sum (-1)^|S| prod_j x_j(S) with x(S) = x0 + sum_{k in S}
+ * a_k and x0_j = a[j][n-1] - rowsum_j / 2 in double-double from the matrix
+ * (no scale factor).  It restates the Nijenhuis-Wilf sum of the reference's
+ * range evaluator (gpu_exact_dense.cu:6-69) over the lane decomposition of
+ * the GPU kernels, with X, products and sum in double-double.            */
+void orc_lane_range_dd(const double* mat, int n, int w, i64 lane_begin, i64 lane_end,
+                       int threads, double* out_hi, double* out_lo) {
+  dd x0[64];
+  dd colt[64 * 64];
+  for (int j = 0; j < n; j++) {
+    dd rs = dd_from(0.0);
+    for (int k = 0; k < n; k++) rs = dd_add(rs, dd_from(mat[j * n + k]));
+    dd half = {rs.hi * 0.5, rs.lo * 0.5};
+    x0[j] = dd_add(dd_from(mat[j * n + n - 1]), dd_neg(half));
+  }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) colt[k * n + j] = dd_from(mat[j * n + k]);
+  const i64 lanes = lane_end - lane_begin;
+  if (threads < 1) threads = 1;
+  dd* part = (dd*)calloc(lanes > 0 ? lanes : 1, sizeof(dd));
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (i64 li = 0; li < lanes; li++) {
+    const i64 l = lane_begin + li;
+    dd x[64];
+    memcpy(x, x0, sizeof(dd) * n);
+    int parity = 0;
+    for (int b = 0; w + b < n - 1; b++)
+      if ((l >> b) & 1LL) {
+        parity ^= 1;
+        for (int j = 0; j < n; j++) x[j] = dd_add(x[j], colt[(w + b) * n + j]);
+      }
+    dd acc = dd_from(0.0);
+    i64 g = 0;
+    for (i64 t = 0; t < ((i64)1 << w); t++) {
+      if (t > 0) {
+        int k = __builtin_ctzll((unsigned long long)t);
+        g ^= 1LL << k;
+        int plus = (g >> k) & 1LL;
+        for (int j = 0; j < n; j++)
+          x[j] = dd_add(x[j], plus ? colt[k * n + j] : dd_neg(colt[k * n + j]));
+      }
+      dd prod = x[0];
+      for (int j = 1; j < n; j++) prod = dd_mul(prod, x[j]);
+      acc = dd_add(acc, ((t & 1LL) ^ parity) ? dd_neg(prod) : prod);
+    }
+    part[li] = acc;
+  }
+  dd p = dd_from(0.0);
+  for (i64 li = 0; li < lanes; li++) p = dd_add(p, part[li]);
+  free(part);
+  *out_hi = p.hi;
+  *out_lo = p.lo;
 }

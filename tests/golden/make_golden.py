@@ -85,6 +85,10 @@ def cache_cases():
         cases.append(synthetic_matrix(n, d, seed=900 + n))
     # test_gpu_accuracy.py::test_every_split_within_stated_error
     cases.append(synthetic_matrix(24, 0.5, seed=24))
+    # test_gpu_accuracy.py::test_error_estimate_is_tight
+    for n in (20, 22, 24, 26):
+        for k, d in enumerate((0.2, 0.5, 0.9)):
+            cases.append(synthetic_matrix(n, d, seed=4000 + 10 * n + k))
     # test_gpu_nested.py::test_integer_matrices_exact_under_every_shape
     rng = np.random.default_rng(3)
     for n in (24, 28):
@@ -113,12 +117,21 @@ def _cache_entry(a):
 
 
 def cache():
-    out = {}
+    """Write oracle_cache.json; entries already there are kept (the key is a
+    hash of the matrix), so adding cases costs only their own oracle runs."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import matrix_key  # noqa: E402
+
+    path = os.path.join(OUT, "oracle_cache.json")
+    old = json.load(open(path)) if os.path.exists(path) else {}
+    cases = cache_cases()
+    todo = [a for a in cases if matrix_key(a) not in old]
+    out = {matrix_key(a): old[matrix_key(a)] for a in cases if matrix_key(a) in old}
     with ProcessPoolExecutor(max_workers=8) as pool:
-        for key, rec in pool.map(_cache_entry, cache_cases()):
+        for key, rec in pool.map(_cache_entry, todo):
             out[key] = rec
             print(key, rec.get("exact", rec.get("dd_hi")), flush=True)
-    with open(os.path.join(OUT, "oracle_cache.json"), "w") as fh:
+    with open(path, "w") as fh:
         json.dump(out, fh, indent=0, sort_keys=True)
 
 

@@ -83,3 +83,28 @@ def test_default_split_n36_vs_double_double():
     print(f"n=36 fast={fast.value!r} dd={ref.value!r} rel={rel(fast.value, ref.value):.3e}")
     assert rel(fast.value, ref.value) < 1e-11
     assert abs(fast.value - ref.value) <= fast.err_est
+
+
+def test_sampled_error_measurement_on_the_single_matrix_path():
+    """A non-integer matrix whose modelled FP64 estimate misses 1e-10 (n = 34,
+    density 0.2) has its error measured on 256 workgroups recomputed in the
+    double-double X mode: the result stays the FP64 one, is certified, and
+    is within 1e-10 of the full double-double X pass."""
+    rng = np.random.default_rng(34003)
+    while True:
+        mask = rng.random((34, 34)) < 0.2
+        if mask.any(0).all() and mask.any(1).all():
+            break
+    a = np.where(mask, 1.0 - rng.random((34, 34)), 0.0)
+    plan = P.Plan(a)
+    fp = plan.run(0)
+    auto = plan.run()
+    ref = plan.run(2)
+    plan.close()
+    print(f"model {fp.err_est / abs(fp.value):.2e} measured {auto.err_est / abs(auto.value):.2e} "
+          f"actual {abs(fp.value - ref.value) / abs(ref.value):.2e} units {auto.verified_units}")
+    assert fp.err_est > 1e-10 * abs(fp.value)  # the model alone does not certify it
+    assert auto.mode == 0 and auto.verified_units > 0 and auto.certified
+    assert auto.value == fp.value
+    assert abs(auto.value - ref.value) <= auto.err_est
+    assert abs(auto.value - ref.value) <= 1e-10 * abs(ref.value)

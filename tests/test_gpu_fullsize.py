@@ -193,3 +193,33 @@ def test_largest_sizes_exact_tilings():
     print(f"skipper n=48: {s.value!r} walked {s.main_subsets / s.subsets:.2e} "
           f"kernel_ms={s.kernel_ms:.1f}")
     assert s.value == float(O.domino_tilings(8, 12))
+
+
+def test_n40_block_ranges_against_cpu_double_double():
+    """The C2 matrix (n = 40) pinned independently of the GPU's own modes:
+    workgroup ranges of the natural-order walk (first, middle, last) summed
+    by the GPU in FP64 and in mode 2 against a CPU double-double evaluation
+    of exactly the same subsets (oracle.lane_range_dd; 2^18 subsets per
+    workgroup).  Mode 2 agrees to 1e-13 of the range's sum of |terms|; FP64
+    within its stated error (the range's err_est is scaled by the same
+    moments)."""
+    import os
+
+    a = P.synthetic_matrix(40, 0.5, 2)
+    os.environ["PERMAN_PAIR"] = "0"  # natural walk order (no row/column reordering)
+    try:
+        plan = P.Plan(a)
+    finally:
+        del os.environ["PERMAN_PAIR"]
+    w, h, blocks = plan.geometry()
+    assert (w, h, blocks) == (10, 29, 1 << 21)
+    for b0 in (0, blocks // 2 + 12345, blocks - 1):
+        fp = plan.run_blocks(b0, b0 + 1, mode=0)
+        dd = plan.run_blocks(b0, b0 + 1, mode=2)
+        hi, lo = O.lane_range_dd(a, w, 256 * b0, 256 * (b0 + 1), 8)
+        ref = hi + lo
+        a_sum = fp.abs_sum / 2.0  # |scale| = 2
+        print(f"block {b0}: fp={fp.sum_hi!r} dd={dd.sum_hi!r} cpu={ref!r} |terms|={a_sum:.3e}")
+        assert abs((dd.sum_hi + dd.sum_lo) - ref) <= 1e-13 * a_sum, b0
+        assert abs((fp.sum_hi + fp.sum_lo) - ref) <= fp.err_est / 2.0 + 1e-13 * a_sum, b0
+    plan.close()

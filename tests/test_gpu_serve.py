@@ -5,6 +5,7 @@
 * a request gives the same bits alone and in a concurrent mix, whatever
   iterations its items were packed into;
 * poll/wait/stats semantics, rejection of bad requests, shutdown drains."""
+import ctypes
 import threading
 
 import numpy as np
@@ -152,3 +153,82 @@ def test_server_exact_pass_runs_as_batch_items():
             assert r.value == float(ex) and r.err_bound == 0.0, (m.shape, r, ex)
     print("dd passes:", st["dd_passes"], [r.mode for r, _ in res])
     assert st["dd_passes"] == sum(r.mode == 1 for r, _ in res)
+
+
+def _sparse_ill_conditioned(n, seed):
+    """Density 0.2, U(0,1] entries, every row and column nonzero: sums whose
+    cancellation makes the modelled FP64 error estimate exceed 1e-10 at
+    n >= 28 (numpy's default generator, seeded)."""
+    rng = np.random.default_rng(seed)
+    while True:
+        mask = rng.random((n, n)) < 0.2
+        if mask.any(0).all() and mask.any(1).all():
+            break
+    return np.where(mask, 1.0 - rng.random((n, n)), 0.0)
+
+
+def test_server_two_waiters_on_one_ticket():
+    """Two threads wait on the same ticket: one collects the result, the
+    other is told the ticket was collected by another caller (the waiter
+    looks the ticket up again after every wake-up, never a freed request)."""
+    m = P.synthetic_matrix(28, 0.5, seed=4343)
+    with P.Server() as srv:
+        t = srv.submit(m)
+        out = [None, None]
+
+        def waiter(k):
+            try:
+                out[k] = srv.wait(t)[0]
+            except P.PermError as e:
+                out[k] = e
+
+        ths = [threading.Thread(target=waiter, args=(k,)) for k in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    got = [x for x in out if isinstance(x, P.PermResult)]
+    err = [x for x in out if isinstance(x, P.PermError)]
+    assert len(got) == 1 and len(err) == 1, out
+    assert "collected" in str(err[0]) or "unknown" in str(err[0])
+
+
+def test_server_submit_many_same_bits_and_errors_at_wait():
+    """Bulk submission gives the same bits as one submission at a time
+    (dense and sparse requests mixed); a malformed CCS passes the cheap
+    submission checks and is reported by the wait of that request."""
+    mats = [(P.synthetic_matrix(14 + i % 14, 0.45, seed=950 + i), i % 3 == 1) for i in range(30)]
+    reqs = [P.Request(m, sp) for m, sp in mats]
+    with P.Server() as srv:
+        one = [srv.wait(srv.submit_request(q))[0] for q in reqs]
+        many = srv.wait_many(srv.submit_many(reqs))
+        for a, (b, _) in zip(one, many):
+            assert (a.value, a.sum_hi, a.sum_lo) == (b.value, b.sum_hi, b.sum_lo)
+        bad = P.synthetic_matrix(12, 0.5, seed=3)
+        cptrs, rows, cvals = P.compress(bad)
+        rows = rows.copy()
+        rows[0] = 99  # row index out of range
+        tid = ctypes.c_ulonglong()
+        st = P.lib().perm_server_submit_sparse(srv._h, 2, bad.ctypes.data, P._ip(cptrs),
+                                               P._ip(rows), cvals.ctypes.data, 12,
+                                               ctypes.byref(tid))
+        assert st == 0
+        with pytest.raises(P.PermError, match="row index"):
+            srv.wait(tid.value)
+
+
+def test_server_sampled_error_measurement():
+    """Non-integer requests of many items whose modelled error estimate
+    misses 1e-10 have their error measured on a sample of items in the
+    double-double X mode (dd_passes counts it) and come out certified and
+    within 1e-10 of the single-matrix path."""
+    mats = [_sparse_ill_conditioned(28, 28020), _sparse_ill_conditioned(30, 30020),
+            P.synthetic_matrix(30, 0.5, seed=1530)]
+    with P.Server() as srv:
+        res = srv.wait_many([srv.submit(m) for m in mats])
+        st = srv.stats()
+    for m, (r, _) in zip(mats, res):
+        one = P.dense(m)
+        assert r.certified, r
+        assert abs(r.value - one.value) <= 1e-10 * abs(one.value), (m.shape, r, one)
+    print("dd passes:", st["dd_passes"], [(r.mode, r.err_est / abs(r.value)) for r, _ in res])

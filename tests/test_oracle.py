@@ -276,3 +276,24 @@ def test_c4_mix_fixture_pins():
             assert abs((hi + lo) - (rec["hi"] + rec["lo"])) <= 1e-20 * abs(hi)
             checked_dd += 1
     assert checked_exact == 40 and checked_dd == 10
+
+
+def test_lane_range_oracle_sums_to_the_permanent():
+    """oracle.lane_range_dd (the GPU walk's lane decomposition, natural
+    order): over all lanes it gives the exact permanent (times the scale
+    1/(4(n&1)-2)) for every walk length, and ranges add up."""
+    import permanent_amd as P
+
+    for n in (6, 9, 12):
+        a = P.synthetic_matrix(n, 0.6, seed=n)
+        sc = 4 * (n & 1) - 2
+        ex = float(O.exact_permanent(a))
+        for w in range(1, n - 1):
+            h = n - 1 - w
+            hi, lo = O.lane_range_dd(a, w, 0, 1 << h, 4)
+            assert abs(sc * (hi + lo) - ex) <= 1e-14 * abs(ex), (n, w)
+            if h >= 2:
+                m = (1 << h) // 3
+                h1, l1 = O.lane_range_dd(a, w, 0, m, 4)
+                h2, l2 = O.lane_range_dd(a, w, m, 1 << h, 4)
+                assert abs(sc * (h1 + l1 + h2 + l2) - ex) <= 1e-14 * abs(ex), (n, w, m)

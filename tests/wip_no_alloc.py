@@ -50,7 +50,7 @@ def test_noalloc_integer_and_zero_pattern():
 
 
 def test_noalloc_rejects_sizes_beyond_its_scratch():
-    a = P.synthetic_matrix(38, 0.5, seed=73)
+    a = P.synthetic_matrix(36, 0.5, seed=73)
     st, _ = noalloc(a)
     assert st == 2  # PERM_ERR_SIZE
     assert "scratch" in P.lib().perm_last_error().decode()
