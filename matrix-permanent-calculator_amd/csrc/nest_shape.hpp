@@ -4,8 +4,14 @@
 
 namespace perm {
 
-// Largest n the dense kernels are instantiated for (x lives in 2n VGPRs).
-constexpr int kMaxDenseN = 48;
+// Largest n the dense, sparse and skipping kernels are instantiated for (x
+// lives in 2n VGPRs), as the reference's kernels and CPU paths accept
+// (n <= 63: 64-bit subset masks).
+constexpr int kMaxDenseN = 63;
+// Largest n with pair and nested walk shapes; above it the plain walk only
+// (2^48 subsets and more: the compiled shape menu would multiply the kernels
+// for orders a walk of every subset cannot finish anyway).
+constexpr int kMaxNestN = 48;
 
 // Smallest n with pair (zero-skipping) variants; below it the whole
 // permanent takes microseconds.
@@ -26,7 +32,8 @@ struct NestShape {
 // rows add zeros); the host picks the covering shape of least work.
 constexpr int kNestShapesPerK0 = 5;
 constexpr NestShape nest_menu_shape(int n, int k0, int j) {
-  if (n < kMinPairN || k0 < 4 || k0 > n - 4 || k0 % 4 != 0 || j < 0 || j >= kNestShapesPerK0)
+  if (n < kMinPairN || n > kMaxNestN || k0 < 4 || k0 > n - 4 || k0 % 4 != 0 || j < 0 ||
+      j >= kNestShapesPerK0)
     return {0, 0, 0};
   if (j == 0) return {k0, n, n};
   if (n < kMinNestN) return {0, 0, 0};

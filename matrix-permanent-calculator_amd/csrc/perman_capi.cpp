@@ -70,6 +70,15 @@ struct perm_plan {
   bool zero = false;      // no perfect matching: the permanent is exactly 0
   bool borrowed = false;  // stream/events/memory belong to the device cache
   uint32_t* ids = nullptr;  // device: room for kVerifyUnits workgroup indices
+  // Workgroup records (and their variance weights): inside `mem` for a
+  // borrowed plan; for an owning plan a separate arena of rec_cap records,
+  // allocated when a run first needs it (a block-range run only needs
+  // records for its range).
+  void* rec_base = nullptr;
+  uint64_t rec_cap = 0;
+  double* zdev = nullptr;   // device: one variance weight per workgroup record
+  perm::Partial* vrec = nullptr;  // device: 2 kVerifyUnits records (FP64, then mode 2)
+  std::vector<double> zhost;      // host copy of zdev (kept between runs)
 };
 
 namespace {
@@ -96,10 +105,13 @@ DeviceCache& device_cache(int dev) {
   return *table[dev];
 }
 
-// Device layout of a plan: column store, x0, partial records, result record.
+// Device layout of a plan: column store, x0, result record, sample ids and
+// records, then the workgroup records and their weights (the last two are a
+// separate arena in owning plans).
 struct PlanLayout {
-  size_t col, x, part, res, ids;
-  size_t total() const { return col + x + part + res + ids; }
+  size_t col, x, res, ids, vrec, part, z;
+  size_t fixed() const { return col + x + res + ids + vrec; }
+  size_t total() const { return fixed() + part + z; }
 };
 
 using perm::kVerifyUnits;
@@ -113,6 +125,8 @@ PlanLayout plan_layout(const perm_plan* P, size_t col_doubles) {
   l.part = align_up(blocks_of(1ull << P->high_bits) * sizeof(perm::Partial), 256);
   l.res = align_up(sizeof(perm::Partial), 256);
   l.ids = align_up(kVerifyUnits * sizeof(uint32_t), 256);
+  l.z = align_up(blocks_of(1ull << P->high_bits) * sizeof(double), 256);
+  l.vrec = align_up(2 * kVerifyUnits * sizeof(perm::Partial), 256);
   return l;
 }
 
@@ -128,14 +142,40 @@ size_t layout_plan(perm_plan* P, int n, size_t col_doubles) {
   return plan_layout(P, col_doubles).total();
 }
 
-void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles) {
+// Bind the fixed regions at `base`; with `records`, also the workgroup
+// records of all 2^h lanes and their weights right after them.
+void bind_plan_memory(perm_plan* P, void* base, size_t col_doubles, bool records = true) {
   const PlanLayout l = plan_layout(P, col_doubles);
   char* b = (char*)base;
   P->L.colpm = (const double*)b;
   P->L.x0 = (const double*)(b + l.col);
-  P->L.partials = (perm::Partial*)(b + l.col + l.x);
-  P->L.result = (perm::Partial*)(b + l.col + l.x + l.part);
-  P->ids = (uint32_t*)(b + l.col + l.x + l.part + l.res);
+  P->L.result = (perm::Partial*)(b + l.col + l.x);
+  P->ids = (uint32_t*)(b + l.col + l.x + l.res);
+  P->vrec = (perm::Partial*)(b + l.col + l.x + l.res + l.ids);
+  if (records) {
+    P->L.partials = (perm::Partial*)(b + l.fixed());
+    P->zdev = (double*)(b + l.fixed() + l.part);
+  }
+}
+
+// Room for `count` workgroup records (and weights) in an owning plan's
+// record arena (grow-only); a borrowed plan always has all of them.
+int ensure_records(perm_plan* P, uint64_t count) {
+  if (P->borrowed || count <= P->rec_cap) return PERM_OK;
+  if (P->rec_base) {
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    HIP_TRY(hipFree(P->rec_base));
+    P->rec_base = nullptr;
+    P->rec_cap = 0;
+    P->L.partials = nullptr;
+    P->zdev = nullptr;
+  }
+  const size_t part = align_up(count * sizeof(perm::Partial), 256);
+  HIP_TRY(hipMalloc(&P->rec_base, part + count * sizeof(double)));
+  P->rec_cap = count;
+  P->L.partials = (perm::Partial*)P->rec_base;
+  P->zdev = (double*)((char*)P->rec_base + part);
+  return PERM_OK;
 }
 
 // The FP64 kernel's walk shape from the host preparation (0: not restricted).
@@ -186,7 +226,8 @@ int make_plan(int n, const Prepared& prep, perm_plan** out) {
   P->device = perm::current_device();
   set_shape(P, n, prep);
   P->integral = integral;
-  P->mem.bytes = layout_plan(P, n, colstore.size());
+  layout_plan(P, n, colstore.size());
+  P->mem.bytes = plan_layout(P, colstore.size()).fixed();  // records: ensure_records
   auto cleanup = [&](int code, const std::string& msg) {
     perm_plan_destroy(P);
     return fail(code, msg);
@@ -199,7 +240,7 @@ int make_plan(int n, const Prepared& prep, perm_plan** out) {
     return cleanup(PERM_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
   if ((e = hipMalloc(&P->mem.base, P->mem.bytes)) != hipSuccess)
     return cleanup(PERM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-  bind_plan_memory(P, P->mem.base, colstore.size());
+  bind_plan_memory(P, P->mem.base, colstore.size(), false);
   if (upload_plan(P, colstore, x0) != PERM_OK ||
       (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
     std::string msg = *perm::last_error() ? std::string(perm::last_error()) : hipGetErrorString(e);
@@ -274,39 +315,44 @@ int verify_sampled(perm_plan* P, perm_result* out, uint32_t& units_out, float& m
   units_out = 0;
   ms_out = 0.f;
   const uint64_t blocks = P->L.total_blocks;
-  std::vector<perm::Partial> rec(blocks);
-  HIP_TRY(hipMemcpyAsync(rec.data(), P->L.partials, blocks * sizeof(perm::Partial),
+  // variance weights of the FP64 pass's workgroup records, formed on the
+  // device (8 bytes per record come back instead of the record)
+  const double run_weight = std::ldexp(1.0, P->walk_bits - 1) + (double)P->high_bits;
+  HIP_TRY(perm::launch_unit_weights(P->L.partials, blocks, run_weight, P->zdev, P->stream));
+  P->zhost.resize(blocks);
+  HIP_TRY(hipMemcpyAsync(P->zhost.data(), P->zdev, blocks * sizeof(double),
                          hipMemcpyDeviceToHost, P->stream));
   HIP_TRY(hipStreamSynchronize(P->stream));
-  std::vector<double> z(blocks);
-  for (uint64_t i = 0; i < blocks; ++i)
-    z[i] = perm::unit_weight(rec[i].sq_sum, rec[i].sq_run, P->walk_bits, P->high_bits);
   std::vector<uint32_t> ids;
   std::vector<double> weight;
   uint32_t certain = 0;
-  perm::pps_sample(z, perm::kVerifyUnits, ids, weight, certain);
+  perm::pps_sample(P->zhost, perm::kVerifyUnits, ids, weight, certain);
   const uint32_t count = (uint32_t)ids.size();
   if (count == 0) return PERM_OK;  // nothing to measure
-  // walk the chosen workgroups in mode 2; their records go to partials[0, count)
+  // the chosen FP64 records go to vrec[0, count), their mode 2 walks to
+  // vrec[kVerifyUnits, +count)
   HIP_TRY(hipMemcpyAsync(P->ids, ids.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice,
                          P->stream));
+  HIP_TRY(perm::launch_gather_partials(P->L.partials, P->ids, count, P->vrec, P->stream));
   perm::DenseLaunch L = P->L;
   L.mode = 2;
   L.nu = L.nu1 = L.nu2 = P->n;
   L.main.block_ids = P->ids;
   L.main.block_count = count;
+  L.main.partials = P->vrec + kVerifyUnits;
   HIP_TRY(hipEventRecord(P->ev0, P->stream));
   HIP_TRY(perm::launch_dense_pass_only(L, P->stream));
   HIP_TRY(hipEventRecord(P->ev1, P->stream));
-  std::vector<perm::Partial> ref(count);
-  HIP_TRY(hipMemcpyAsync(ref.data(), P->L.partials, count * sizeof(perm::Partial),
+  std::vector<perm::Partial> rec(2 * (size_t)kVerifyUnits);
+  HIP_TRY(hipMemcpyAsync(rec.data(), P->vrec, rec.size() * sizeof(perm::Partial),
                          hipMemcpyDeviceToHost, P->stream));
   HIP_TRY(hipStreamSynchronize(P->stream));
   HIP_TRY(hipEventElapsedTime(&ms_out, P->ev0, P->ev1));
   std::vector<double> e(count);
   for (uint32_t k = 0; k < count; ++k) {
-    const perm::Partial& f = rec[ids[k]];
-    e[k] = (f.hi - ref[k].hi) + (f.lo - ref[k].lo);
+    const perm::Partial& f = rec[k];
+    const perm::Partial& r = rec[kVerifyUnits + k];
+    e[k] = (f.hi - r.hi) + (f.lo - r.lo);
   }
   out->err_est = perm::sampled_error(e, weight, certain, P->n, out->value);
   out->certified = perm::is_certified(*out) ? 1 : 0;
@@ -329,6 +375,7 @@ int plan_run_impl(perm_plan* P, int mode, perm_result* out) {
     return ok();
   }
   HIP_TRY(hipSetDevice(P->device));
+  if (int s = ensure_records(P, blocks_of(1ull << P->high_bits))) return s;
   perm::Partial r;
   float ms = 0.f, main_ms = 0.f, first_main_ms = 0.f;
   const int first = mode < 0 ? 0 : mode;
@@ -634,15 +681,16 @@ int perm_plan_run_blocks(perm_plan* P, int mode, unsigned long long block_begin,
   if (block_begin >= block_end || block_end > blocks)
     return fail(PERM_ERR_ARG, "block range outside [0, " + std::to_string(blocks) + ")");
   HIP_TRY(hipSetDevice(P->device));
+  if (int s = ensure_records(P, block_end - block_begin)) return s;
   configure_passes(P, mode);
   perm::DenseLaunch L = P->L;
   L.main.block_begin = block_begin;
   L.main.block_end = block_end;
+  L.main.record_begin = block_begin;  // the range's records from partials[0]
   HIP_TRY(hipEventRecord(P->ev0, P->stream));
   HIP_TRY(perm::launch_dense_pass_only(L, P->stream));
   HIP_TRY(hipEventRecord(P->ev1, P->stream));
-  HIP_TRY(perm::launch_reduce(L.partials + block_begin, block_end - block_begin, L.result,
-                              P->stream));
+  HIP_TRY(perm::launch_reduce(L.partials, block_end - block_begin, L.result, P->stream));
   perm::Partial r;
   HIP_TRY(hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, P->stream));
   HIP_TRY(hipStreamSynchronize(P->stream));
@@ -664,6 +712,7 @@ void perm_plan_destroy(perm_plan* P) {
   (void)hipSetDevice(P->device);
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   if (P->mem.base) (void)hipFree(P->mem.base);
+  if (P->rec_base) (void)hipFree(P->rec_base);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
   if (P->ev1) (void)hipEventDestroy(P->ev1);
   if (P->evm) (void)hipEventDestroy(P->evm);

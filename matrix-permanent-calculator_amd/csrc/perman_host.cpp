@@ -110,7 +110,7 @@ int select_device() {
 void nest_order(int n, perm::NestMenu menu, std::vector<double>& x0, std::vector<double>& cols,
                 int& nu, int& nu1, int& nu2) {
   nu = nu1 = nu2 = n;
-  if (n < perm::kMinPairN || env_int("PERMAN_PAIR", 1) == 0) return;
+  if (n < perm::kMinPairN || n > perm::kMaxNestN || env_int("PERMAN_PAIR", 1) == 0) return;
   if (env_int("PERMAN_NEST", 1) == 0) menu.mask &= 1u;
   const bool nested = (menu.mask & ~1u) != 0 && n >= perm::kMinNestN;
   const int B = n - 1;
@@ -287,38 +287,88 @@ void pps_sample(const std::vector<double>& z, uint32_t K, std::vector<uint32_t>&
   weight.clear();
   certain = 0;
   const uint64_t units = z.size();
-  double Z = 0.0;
-  for (double v : z) Z += v;
-  if (!(Z > 0.0) || units == 0) return;
+  if (units == 0 || K == 0) return;
   K = (uint32_t)std::min<uint64_t>(K, units);
-  std::vector<uint32_t> order(units);
-  for (uint64_t i = 0; i < units; ++i) order[i] = (uint32_t)i;
-  std::partial_sort(order.begin(), order.begin() + K, order.end(), [&](uint32_t a, uint32_t b) {
-    return z[a] > z[b] || (z[a] == z[b] && a < b);
-  });
-  std::vector<char> taken(units, 0);
+  // One pass over the units in blocks of 1024: the block sums (four
+  // interleaved partial sums) and the K largest units (z descending, index
+  // ascending among equal z), kept in a heap whose top is the least of them.
+  constexpr uint64_t kBlock = 1024;
+  const uint64_t nblocks = (units + kBlock - 1) / kBlock;
+  std::vector<double> bsum(nblocks);
+  auto better = [&](uint32_t a, uint32_t b) { return z[a] > z[b] || (z[a] == z[b] && a < b); };
+  std::vector<uint32_t> top;
+  top.reserve(K);
+  for (uint64_t i = 0; i < K; ++i) {
+    top.push_back((uint32_t)i);
+    std::push_heap(top.begin(), top.end(), better);
+  }
+  double least = z[top.front()];  // a later unit enters only with a larger z
+  double Z = 0.0;
+  for (uint64_t b = 0; b < nblocks; ++b) {
+    const uint64_t i0 = b * kBlock, i1 = std::min(units, i0 + kBlock);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t i = i0; i < i1; ++i) {
+      const double v = z[i];
+      acc[i & 3] += v;
+      if (i >= K && v > least) {
+        std::pop_heap(top.begin(), top.end(), better);
+        top.back() = (uint32_t)i;
+        std::push_heap(top.begin(), top.end(), better);
+        least = z[top.front()];
+      }
+    }
+    bsum[b] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    Z += bsum[b];
+  }
+  if (!(Z > 0.0)) return;
+  if (K == units) {  // every unit, each with certainty
+    for (uint64_t i = 0; i < units; ++i) {
+      ids.push_back((uint32_t)i);
+      weight.push_back(1.0);
+    }
+    certain = (uint32_t)units;
+    return;
+  }
+  std::sort(top.begin(), top.end(), better);
+  // certain units: while the largest remaining one holds a whole sampling
+  // interval of the rest
   double z_rest = Z;
   uint32_t k_rest = K;
-  for (uint32_t c = 0; c < K && k_rest > 0 && z[order[c]] * k_rest >= z_rest; ++c) {
-    ids.push_back(order[c]);
+  for (uint32_t c = 0; c < K && k_rest > 0 && z[top[c]] * k_rest >= z_rest; ++c) {
+    ids.push_back(top[c]);
     weight.push_back(1.0);
-    taken[order[c]] = 1;
-    z_rest -= z[order[c]];
+    z_rest -= z[top[c]];
     --k_rest;
   }
   certain = (uint32_t)ids.size();
   if (k_rest == 0 || !(z_rest > 0.0)) return;
+  std::vector<uint32_t> taken(ids.begin(), ids.end());
+  std::sort(taken.begin(), taken.end());
   // systematic sampling of the rest in index order (a fixed mid-interval
-  // start, so the choice is deterministic)
+  // start, so the choice is deterministic); a block without a certain unit
+  // that the next sampling point lies beyond is passed by its sum
   const double step = z_rest / k_rest;
   double next = 0.5 * step, cum = 0.0;
-  for (uint64_t i = 0; i < units && ids.size() < K; ++i) {
-    if (taken[i] || !(z[i] > 0.0)) continue;
-    cum += z[i];
-    if (cum > next) {
-      ids.push_back((uint32_t)i);
-      weight.push_back(step / z[i]);
-      while (next < cum) next += step;
+  size_t t = 0;
+  for (uint64_t b = 0; b < nblocks && ids.size() < K; ++b) {
+    const uint64_t i0 = b * kBlock, i1 = std::min(units, i0 + kBlock);
+    const bool has_taken = t < taken.size() && taken[t] < i1;
+    if (!has_taken && cum + bsum[b] < next) {
+      cum += bsum[b];
+      continue;
+    }
+    for (uint64_t i = i0; i < i1 && ids.size() < K; ++i) {
+      if (t < taken.size() && taken[t] == i) {
+        ++t;
+        continue;
+      }
+      if (!(z[i] > 0.0)) continue;
+      cum += z[i];
+      if (cum > next) {
+        ids.push_back((uint32_t)i);
+        weight.push_back(step / z[i]);
+        while (next < cum) next += step;
+      }
     }
   }
 }

@@ -407,9 +407,11 @@ __device__ __forceinline__ void lane_walk(const double* __restrict__ colpm,
     for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
     add_column<N, MODE, N>(x, xl, cv);
   }
+  // (readfirstlane returns int: each half goes back through uint32_t, or a
+  // set bit 31 -- pattern bit 37 -- would sign-extend over the upper half)
   const uint64_t wave_bits =
-      ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 38)) << 32) |
-      (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 6));
+      ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 38)) << 32) |
+      (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(pattern >> 6));
   for (int b = lane_bits; b < n_high_bits; ++b) {
     if (!((wave_bits >> (b - 6)) & 1)) continue;
     parity ^= 1;
@@ -478,7 +480,8 @@ constexpr int unroll_bits_for() {
 }
 
 // Workgroups [block_begin, block_end) of pass P (all of them when
-// block_end == 0); partial of workgroup b goes to P.partials[b].
+// block_end == 0); partial of workgroup b goes to
+// P.partials[b - P.record_begin].
 template <int N, int MODE, class S>
 hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
@@ -492,9 +495,11 @@ hipError_t launch_pass(const DenseLaunch& L, const LanePass& P, hipStream_t stre
   }
   const uint64_t b0 = P.block_begin, b1 = P.block_end ? P.block_end : P.blocks;
   if (b1 <= b0) return hipSuccess;
-  if (b1 > P.blocks) return hipErrorInvalidValue;
+  if (b1 > P.blocks || P.record_begin > b0 || b1 - b0 > 0xffffffffull)
+    return hipErrorInvalidValue;
   ryser_dense_kernel<N, R, MODE, S><<<dim3((unsigned)(b1 - b0)), dim3(256), 0, stream>>>(
-      L.colpm, L.x0, P.partials + b0, b0 * 256, P.lanes, P.walk_bits, P.high_bits, nullptr);
+      L.colpm, L.x0, P.partials + (b0 - P.record_begin), b0 * 256, P.lanes, P.walk_bits,
+      P.high_bits, nullptr);
   return hipGetLastError();
 }
 
@@ -546,7 +551,7 @@ hipError_t launch_nested(const DenseLaunch& L, hipStream_t stream, hipEvent_t ma
 template <int N, int MODE>
 hipError_t launch_shaped(const DenseLaunch& L, hipStream_t stream, hipEvent_t main_done) {
   if (L.nu == N || L.nu <= 0) return launch_mode<N, MODE, Nest<N, N, N>>(L, stream, main_done);
-  if constexpr (N >= kMinPairN) {
+  if constexpr (N >= kMinPairN && N <= kMaxNestN) {
     if (L.nu % 4 == 0 && L.nu < N)
       return launch_nested<N, MODE>(L, stream, main_done,
                                     std::make_integer_sequence<int, (N - 1) / 4>{});

@@ -104,12 +104,14 @@ inline Split choose_split(int n, int unroll_bits, int walk_override) {
 }
 
 // Split of the skipping kernel: long walks (the skips are aligned runs
-// inside a lane's walk) -- lanes = 2^min(21, n-11).  Integer matrices only,
-// where X is exact and there is no drift.
+// inside a lane's walk) -- lanes = 2^min(21, n-11), but at least 2^(n-32)
+// (a walk counts its steps in 32 bits).  Integer matrices only, where X is
+// exact and there is no drift.
 inline Split choose_skip_split(int n, int unroll_bits, int walk_override) {
   const int B = n - 1;
   int h = B - 10;
   if (h > 21) h = 21;
+  if (h < B - 31) h = B - 31;  // walks of at most 2^31 steps (32-bit step counter)
   if (h < 0) h = 0;
   int w = B - h;
   if (walk_override > 0) {

@@ -18,6 +18,7 @@ struct LanePass {
   int high_bits;        // walk_bits + high_bits = n - 1
   Partial* partials;    // device: `blocks` records
   uint64_t block_begin = 0, block_end = 0;  // workgroups to launch (end 0: all)
+  uint64_t record_begin = 0;  // the record of workgroup b goes to partials[b - record_begin]
   // Instead of a range: the listed workgroups (device array of block_count
   // indices); the record of the i-th listed workgroup goes to partials[i].
   const uint32_t* block_ids = nullptr;
@@ -67,6 +68,13 @@ hipError_t launch_dense_pass_only(const DenseLaunch& L, hipStream_t stream);
 // The grid reduction alone: the DD sum of in[0, count) into *out (the same
 // order as launch_dense's).
 hipError_t launch_reduce(const Partial* in, uint64_t count, Partial* out, hipStream_t stream);
+// z[i] = rec[i].sq_sum + run_weight * rec[i].sq_run for i < units: the
+// modelled variance weights of a walk's workgroup records.
+hipError_t launch_unit_weights(const Partial* rec, uint64_t units, double run_weight, double* z,
+                               hipStream_t stream);
+// dst[k] = src[ids[k]] for k < count (ids checked by the caller).
+hipError_t launch_gather_partials(const Partial* src, const uint32_t* ids, uint32_t count,
+                                  Partial* dst, hipStream_t stream);
 // Unrolled inner steps per group (log2) used for a given n.
 int dense_unroll_bits(int n);
 
