@@ -186,33 +186,52 @@ def batched_leg(P, count: int, seed: int):
     """Batched mixed-size stream (4096 matrices, n uniform in [12, 28],
     density uniform in [0.2, 0.8], nonzeros U(0,1], seed 5): one resident
     batch replayed as a graph, against one drop-in call per matrix (the
-    reference's usage: host setup, upload, launch, download per matrix)."""
+    reference's usage: host setup, upload, launch, download per matrix).
+    Both sides are timed around their C calls only: the matrices' C
+    arguments are built beforehand and the results land in C arrays
+    (converted to Python objects after the timed region)."""
+    import ctypes
+
     mats = P.mixed_batch(count, 12, 28, 0.2, 0.8, seed)
     subsets = sum(2.0 ** (m.shape[0] - 1) for m in mats)
+    L = P.lib()
+    inputs = P.BatchInputs(mats)
+    res_c = (P.PermResultC * count)()
+    ms = ctypes.c_double()
+    # end to end: create, prepare (host threads), plan + upload + launch once
     t0 = time.perf_counter()
     b = P.Batch()
-    b.add_many(mats, threads=cpu_share())  # host preparation on the CPU share's threads
-    res, _ = b.run()  # builds (plan items, upload) and launches; no graph yet
+    b.add_prepared(inputs, threads=cpu_share())
+    P._check(L.perm_batch_run(b._h, res_c, ctypes.byref(ms)))
     t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     tuned = b.autotune(reps=2)  # picks the work split on this batch
     t_tune = time.perf_counter() - t0
-    res, _ = b.run()  # rebuild with the chosen split
+    P._check(L.perm_batch_run(b._h, res_c, ctypes.byref(ms)))  # rebuild with the chosen split
     reps = 3
     t0 = time.perf_counter()
     dev = []
     for _ in range(reps):
-        res, ms = b.run()
-        dev.append(ms)
+        P._check(L.perm_batch_run(b._h, res_c, ctypes.byref(ms)))
+        dev.append(ms.value)
     t_batch = (time.perf_counter() - t0) / reps
     b.close()
+    res = [P.PermResult.from_c(r) for r in res_c]
+    # plain: one perm_dense call per matrix (ID 4), arguments prebuilt
+    args = [(P._DT[m.dtype], m.ctypes.data, m.shape[0]) for m in inputs.mats]
+    one = P.PermResultC()
+    plain_c = (P.PermResultC * count)()
     t0 = time.perf_counter()
-    plain = [P.dense(m) for m in mats]
+    for i, (dt, ptr, n) in enumerate(args):
+        P._check(L.perm_dense(dt, ptr, n, 4, ctypes.byref(one)))
+        plain_c[i] = one
     t_plain = time.perf_counter() - t0
+    plain = [P.PermResult.from_c(r) for r in plain_c]
     worst, worst_err = _diff_vs_plain(res, plain)
     oracle_err = _vs_c4_fixture(mats, res)
     return {
         "workload": f"{count} matrices, n~U{{12..28}}, density~U[0.2,0.8], U(0,1], seed {seed}",
+        "timing": "C calls only (arguments prebuilt, results into C arrays) on both sides",
         "subsets": subsets,
         "batched_perms_per_s": count / t_batch,
         "batched_subsets_per_s": subsets / t_batch,
@@ -270,69 +289,64 @@ def _vs_c4_fixture(mats, res):
 def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
     """Served stream: `count` requests, n in [16, 36] with
     P(n) ~ 2^(-(n-16)/decay), density U[0.1, 0.9], half 0/1 and half FP64, a
-    third through the sparse path.  Capacity: every request submitted at once
-    (closed loop).  Then an open loop of Poisson arrivals at `load` x
-    capacity; p50/p99 of submit-to-result latency.  Plain: one drop-in call
-    per request."""
-    import threading
+    third through the sparse path.  The server's work split is first
+    autotuned on a sample of the same mix (another seed).  Capacity: every
+    request submitted at once (closed loop).  Then an open loop of Poisson
+    arrivals at `load` x capacity driven from C (perm_server_drive); p50/p99
+    of submit-to-result latency.  Plain: one perm_dense / perm_sparse call
+    per request.  Every side is timed around its C calls only (arguments
+    prebuilt, results into C arrays)."""
+    import ctypes
 
     import numpy as np
 
-    # requests arrive as arrays (CCS built for the sparse ones) before any timing
     reqs = [P.Request(m, sp) for m, sp in P.served_workload(count, seed, decay=decay)]
     work = sum(2.0 * q.a.shape[0] * 2.0 ** (q.a.shape[0] - 1) for q in reqs)
-    with P.Server() as srv:
-        for q in reqs[:32]:  # warm the kernels of the common sizes
-            srv.wait(srv.submit_request(q))
+    batch = P.RequestBatch(reqs)
+    sample = [P.Request(m, sp)
+              for m, sp in P.served_workload(min(count, 1000), seed + 100, decay=decay)]
+    t0 = time.perf_counter()
+    cfg, tuned_rate = P.autotune_server(sample)
+    t_tune = time.perf_counter() - t0
+    progress(f"  autotuned in {t_tune:.1f} s: {cfg}")
+    with P.Server(**cfg) as srv:
+        srv.wait_many(srv.submit_many(reqs[:32]))  # warm the host paths and the kernel
         t0 = time.perf_counter()
-        tickets = srv.submit_many(reqs)  # one call; preparation runs in the server's pool
-        closed = srv.wait_many(tickets)
+        ids = srv.submit_batch(batch)  # one call; preparation runs in the server's pool
+        res_c, _ = srv.wait_many_raw(ids)
         t_closed = time.perf_counter() - t0
         stats = srv.stats()
+    closed = [P.PermResult.from_c(r) for r in res_c]
     capacity = count / t_closed
+    progress(f"  closed loop {capacity:.0f} req/s")
     rate = load * capacity
     gaps = -np.log(1.0 - P.uniforms((seed << 20) + (1 << 18), count)) / rate
     arrivals = np.cumsum(gaps)
-    lat = [None] * count
-    with P.Server() as srv:
-        done = threading.Event()
-        queue = []
-        cv = threading.Condition()
-
-        def collector():
-            k = 0
-            while k < count:
-                with cv:
-                    while len(queue) <= k:
-                        cv.wait()
-                    t = queue[k]
-                lat[k] = srv.wait(t)[1]
-                k += 1
-            done.set()
-
-        th = threading.Thread(target=collector)
-        th.start()
-        t0 = time.perf_counter()
-        for i, q in enumerate(reqs):
-            delay = arrivals[i] - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
-            t = srv.submit_request(q)
-            with cv:
-                queue.append(t)
-                cv.notify()
-        done.wait()
-        th.join()
-        t_open = time.perf_counter() - t0
-    lat_ms = np.array(lat)
+    with P.Server(**cfg) as srv:
+        _, lat_c, t_open = srv.drive(batch, arrivals)
+    lat_ms = np.array(list(lat_c))
+    progress(f"  open loop done in {t_open:.1f} s")
+    plain_c = (P.PermResultC * count)()
+    one = P.PermResultC()
+    L = P.lib()
     t0 = time.perf_counter()
-    plain = [P.solve(q) for q in reqs]
+    for i, q in enumerate(reqs):
+        if q.sparse:
+            P._check(L.perm_sparse(*q._args, 4, ctypes.byref(one)))
+        else:
+            P._check(L.perm_dense(*q._args, 4, ctypes.byref(one)))
+        plain_c[i] = one
     t_plain = time.perf_counter() - t0
-    worst, worst_err = _diff_vs_plain([r for r, _ in closed], plain)
+    plain = [P.PermResult.from_c(r) for r in plain_c]
+    worst, worst_err = _diff_vs_plain(closed, plain)
     return {
         "workload": f"{count} requests, n in [16,36] with P(n)~2^(-(n-16)/{decay:g}), density "
                     f"U[0.1,0.9], half 0/1 half FP64, 1/3 sparse path, seed {seed}",
+        "timing": "C calls only (arguments prebuilt, results into C arrays)",
         "algorithmic_flops": work,
+        "autotuned_config": cfg,
+        "autotune_s": t_tune,
+        "autotune_sample_req_per_s": tuned_rate,
         "capacity_req_per_s": capacity,
         "capacity_tflops": work / t_closed / 1e12,
         "offered_load": load,
@@ -345,10 +359,15 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
         "capacity_vs_plain": t_plain / t_closed,
         "max_rel_diff_vs_plain": worst,
         "max_diff_over_stated_error": worst_err,
-        "certified_fraction": sum(r.certified for r, _ in closed) / len(closed),
-        "passes": {f"mode{m}": sum(r.mode == m for r, _ in closed) for m in (0, 1, 2)},
+        "certified_fraction": sum(r.certified for r in closed) / len(closed),
+        "passes": {f"mode{m}": sum(r.mode == m for r in closed) for m in (0, 1, 2)},
         "server_stats_closed_loop": stats,
     }
+
+
+def progress(msg: str):
+    """A progress line on stderr (the result is the one JSON line on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -373,6 +392,7 @@ def main():
 
     import permanent_amd as P
 
+    progress("dense n=40 steps")
     mat = P.synthetic_matrix(N, DENSITY, SEED)
     plan = P.Plan(mat)
     for _ in range(args.warmup):
@@ -450,21 +470,28 @@ def main():
         },
     }
     if not args.no_mixed:
+        progress("0/1 n=40 leg")
         out["binary_n40"] = binary_leg(P, 2)
+    progress("sparse leg")
     out["sparse"] = sparse_leg(P, 3)
     with affinity_kept():
         out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
     if not args.no_mixed:
+        progress("batched leg")
         out["batched"] = batched_leg(P, args.batch_count, 5)
         if args.serve_count > 0:
+            progress("served leg (decay 4)")
             out["served"] = served_leg(P, args.serve_count, args.serve_load, 6)
             # the other reading of the configuration's size law, P(n) ~ 2^-(n-16):
             # mostly n = 16..18, per-request overhead dominates
+            progress("served leg (decay 1)")
             out["served_small_n"] = served_leg(P, args.serve_count, args.serve_load, 7,
                                                decay=1.0)
     if not args.no_cpu:
+        progress("CPU baseline")
         with affinity_kept():
             out["cpu_baseline"] = cpu_baseline(mat, args.cpu_sample_log2, args.cpu_threads)
+    progress("done")
     print(json.dumps(out), flush=True)
 
 

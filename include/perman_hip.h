@@ -367,43 +367,50 @@ double perm_matlab_calculate_return(const int* mat, int algorithm, int nt, int x
 
 /* ---- server: continuous batching of requests (extension) ---------------
  * No counterpart in the reference (its entry points compute one permanent
- * per call).  Requests (n <= 40, dense or sparse) are submitted from any
- * thread and return a ticket at once: the submission only copies the matrix;
- * a pool of host threads prepares it (start vector, column store, walk
- * shape) and a scheduler thread packs the 64-lane items of all admitted
- * requests into iterations of a few milliseconds of GPU work, shortest
- * remaining work first (a request older than max_wait_ms goes ahead), and
- * sums each request's items in a fixed order: a result is bitwise the same
- * alone or in any mix.  Argument errors found while preparing (a malformed
- * CCS) are reported by the wait/poll of that request.
- * Admission is bounded by a device arena of resident matrices and by the
- * queued work.  perm_server_wait blocks until the request is done and
- * frees its ticket; perm_server_poll returns PERM_PENDING until then.  In a
- * served result, kernel_ms holds the request's latency (submit to result,
- * also returned in *latency_ms).  Integer results are exact under the same
- * rule as perm_dense (rounded when the stated error is below 1/2, else a
- * double-double product pass). */
+ * per call; its nearest structure is the chunk queue of its multi-GPU
+ * wrappers, gpu_exact_dense.cu:776-904).  Requests (n <= 40, dense or
+ * sparse) are submitted from any thread and return a ticket at once: the
+ * submission only copies the matrix; a pool of host threads prepares it
+ * (start vector, column store, walk shape, padded to a served order that is
+ * a multiple of 4) and a scheduler thread places it in a device arena and
+ * publishes it to ONE persistent kernel, launched as a captured hipGraph,
+ * that polls a ring of pass descriptors in pinned host memory.  The
+ * kernel's waves claim 64-lane items in publication order; the wave that
+ * finishes a request's last item sums its item records on the device in
+ * item order and writes the sum to host memory: a result is bitwise the
+ * same alone or in any mix.  Argument errors found while preparing (a
+ * malformed CCS) are reported by the wait/poll of that request.
+ * Admission is bounded by the resident requests, the arena and the queued
+ * work.  perm_server_wait blocks until the request is done and frees its
+ * ticket; perm_server_poll returns PERM_PENDING until then.  In a served
+ * result, kernel_ms holds the request's latency (submit to result, also
+ * returned in *latency_ms).  Every result meets the accuracy rule of
+ * perm_dense (integer results exact; others certified to 1e-10 by the
+ * model, a sampled measurement or a double-double pass). */
 typedef struct perm_server perm_server;
 typedef struct perm_server_config {
-  int item_log2;          /* subsets per item, log2 (default 18) */
-  int streams;            /* compute streams (default 4, at most 8) */
-  int inflight;           /* iterations queued on the GPU (default 2, at most 4) */
-  int max_items;          /* items per iteration (default 32768) */
-  double iteration_us;    /* work per iteration at full FP64 issue rate, microseconds (8000) */
-  int max_resident;       /* matrices resident on the device (default 2048) */
+  int item_log2;          /* subsets per item, log2 (default 16) */
+  int grid;               /* workgroups of the persistent kernel (default 1024: 4 waves per SIMD) */
+  int max_items;          /* most items per pass (default 4096: more lane groups per item above) */
+  double idle_us;         /* a launch of the persistent kernel ends after this long without work (2000) */
+  int max_resident;       /* requests resident on the device (default 4096) */
   double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */
   int prep_threads;       /* host threads preparing submitted matrices (default 4) */
-  double max_wait_ms;     /* age after which a request is scheduled ahead of the
-                             shortest-remaining-work order (default 50) */
+  double arena_mb;        /* device arena for matrices and item records, MiB (default 256) */
 } perm_server_config;
 typedef struct perm_server_stats {
-  long long submitted, admitted, completed, iterations, items, pending;
-  long long uploads;      /* host-to-device copies of admitted matrices */
-  double admit_ms;        /* scheduler time admitting and uploading requests */
-  double issue_ms;        /* scheduler time building and launching iterations */
-  double retire_ms;       /* scheduler time summing and publishing results */
-  double gpu_wait_ms;     /* scheduler time blocked on the GPU */
-  double gpu_ms;          /* summed GPU span of the iterations (upload to result copy) */
+  long long submitted, admitted, completed;
+  long long launches;     /* launches (graph replays) of the persistent kernel */
+  long long items;        /* items published */
+  long long pending;      /* prepared requests not yet admitted */
+  long long uploads;      /* host-to-device copies of arena blocks */
+  long long stops;        /* launches stopped so that arena space could be reused */
+  long long laps;         /* times the arena's block ring wrapped to its start */
+  double admit_ms;        /* scheduler time admitting, staging and copying requests */
+  double publish_ms;      /* scheduler time publishing passes and launching */
+  double retire_ms;       /* scheduler time turning completed passes into results */
+  double stop_ms;         /* scheduler time waiting for stopped launches */
+  double gpu_ms;          /* summed duration of the kernel's launches */
   long long dd_passes;    /* passes in a double-double mode: escalations and sampled error measurements */
 } perm_server_stats;
 void perm_server_default_config(perm_server_config* cfg);
@@ -435,6 +442,24 @@ int perm_server_stats_get(perm_server* server, perm_server_stats* out);
 /* Finishes every submitted request, then frees the server (uncollected
  * results are dropped). */
 void perm_server_destroy(perm_server* server);
+/* Open-loop driver (load testing): submits request i (arguments as
+ * perm_server_submit_many) at arrival_s[i] seconds after the call from a
+ * thread of its own, and collects every result in order on the calling
+ * thread: out[i], latency_ms[i] (submit to result; may be NULL), *wall_s
+ * the time from the call until the last result (may be NULL). */
+int perm_server_drive(perm_server* server, int count, const int* dtypes,
+                      const void* const* mats, const int* const* cptrs, const int* const* rows,
+                      const void* const* cvals, const int* novs, const double* arrival_s,
+                      perm_result* out, double* latency_ms, double* wall_s);
+/* Autotuner of the served work split: closed-loop trials of the sample
+ * (arguments as perm_server_submit_many) over item_log2 {12..20} and the
+ * kernel's grid {256..2048} by coordinate descent from *base (NULL:
+ * defaults); *best receives the fastest configuration and
+ * *best_req_per_s its throughput.  Each trial creates its own server. */
+int perm_server_autotune(const perm_server_config* base, int count, const int* dtypes,
+                         const void* const* mats, const int* const* cptrs, const int* const* rows,
+                         const void* const* cvals, const int* novs, perm_server_config* best,
+                         double* best_req_per_s);
 
 /* ---- CPU path of `perman -c` (parallel_perman64, algo.h:663; and
  * parallel_perman64_sparse, algo.h:569).  Native OpenMP Ryser with FP64 X

@@ -1,35 +1,49 @@
 // perman_serve.cpp -- a permanent server on one GPU: continuous batching of
-// requests of mixed n (C ABI in include/perman_hip.h, perm_server_*).
+// requests of mixed n (C ABI in include/perman_hip.h, perm_server_*) on one
+// persistent kernel (ryser_serve.hip) launched as a captured graph.
 //
 // Callers submit matrices from any thread and wait on (or poll) a ticket.
-// A scheduler thread owns the GPU work:
 //
-//  * submission copies the caller's matrix and returns a ticket; a pool of
-//    prep threads prepares it (start vector, column store, walk shape);
-//  * admission: a prepared request gets a device slot for its column store
-//    and start vector (a fixed arena, so admission is bounded by device
-//    memory) while the queued work stays under a cap;
-//  * iterations: each iteration takes 64-lane items (the batch kernels'
-//    items, ryser_batch.hip) from the admitted requests, shortest remaining
-//    work first (requests older than max_wait_ms ahead of that, oldest
-//    first), up to a work budget of a few milliseconds, and launches one
-//    persistent kernel per variant (n and walk shape) present over several
-//    streams;
-//    up to `inflight` iterations are queued on the GPU at once, so the next
-//    one is ready when the previous drains;
-//  * completion: an iteration's item partials come back to pinned memory
-//    and are stored per request by item index; a request whose items are
-//    all done is summed on the host in item order (double-double) -- so its
-//    result depends only on the matrix, bitwise the same alone or in any mix
-//    and whatever iterations its items ran in;
-//  * a result that misses the accuracy target goes back to the active list
-//    for a pass in a double-double mode (integer matrices: double-double
-//    products, exact below 2^53; others: double-double X), its matrix still
-//    in its arena slot;
-//  * a matrix whose nonzero pattern has no perfect matching is exactly 0 and
-//    finishes at submission.
-//
-// Iterations change every time, so they are launched directly (no graph).
+//  * Submission copies the caller's matrix; a pool of prep threads prepares
+//    it: start vector, column store and walk shape (kMenuBatch), padded to
+//    the served order N = serve_padded_n(n) (rows of x equal to 1 that no
+//    Gray column touches; n < 8 embedded as A (+) I), the walk split and the
+//    item split (about 2^item_log2 subsets per item, at most max_items items
+//    per pass, several 64-lane groups per item above that).
+//  * Admission (scheduler thread): a prepared request gets one block of the
+//    device arena for its whole life -- a header slot for each of its (at
+//    most three) passes, room for sampled item ids, its column store and
+//    start vector, one record per item (reused by every pass) -- while
+//    resident requests stay under max_resident and queued work under
+//    max_queued_work.  Blocks are taken from a ring in order and freed in
+//    order; since a later pass never needs new space, the oldest request
+//    always finishes and frees the ring's front.  Blocks are written into a
+//    pinned mirror of the arena and go up by one asynchronous copy per run of
+//    new blocks (a later pass uploads its header, and ids, alone).
+//  * Publication: once a round's copy has completed, each pass's header
+//    address goes into the ring of the pinned control block and `tail`
+//    advances.  The kernel's waves claim items in publication order, so no
+//    request waits behind later ones.  If no launch is live, the scheduler
+//    replays the graph of the persistent kernel.
+//  * Completion: the wave that finishes a pass's last item writes the
+//    pass's sum (its records summed in item order, so a result depends only
+//    on its matrix: bitwise the same alone or in any mix) into a completion
+//    record in pinned memory.  The scheduler turns it into a result with the
+//    stated error; a result that misses the accuracy target gets another
+//    pass over the same matrix (double-double products for an integer
+//    matrix; for other matrices first a sampled measurement of the FP64
+//    error -- up to 256 items walked again in double-double X -- then, if
+//    needed, the double-double X pass), else the request is published to its
+//    waiters.
+//  * Cache rule of the persistent kernel: a launch reads headers and
+//    matrices through caches that do not see device-memory writes made by
+//    copies after the launch began, and its waves may scan any pass from the
+//    oldest one in flight when it began.  So arena space is reused only if it
+//    was freed before the live launch began; otherwise the live launch is
+//    stopped first (about once per lap of the arena ring) and the next one
+//    starts with clean caches.
+//  * A matrix whose nonzero pattern has no perfect matching is exactly 0 and
+//    finishes at preparation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,9 +51,10 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <deque>
-#include <queue>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -51,28 +66,28 @@
 #include "ryser_batch.hpp"
 #include "ryser_host.hpp"
 #include "ryser_kernels.hpp"
+#include "ryser_serve.hpp"
 
 namespace {
 
-using perm::BatchItem;
-using perm::BatchMatrix;
 using perm::Partial;
+using perm::ServeControl;
+using perm::ServeDone;
+using perm::ServeHeader;
+using perm::ServeState;
 using Clock = std::chrono::steady_clock;
 
-constexpr int kMaxStreams = 8;
-constexpr int kMaxInflight = 4;
-// Request mode of the sampled verification pass (mode 2 kernels on a sample
-// of the FP64 pass's items).
+// Pass kinds: 0 FP64, 1 double-double products, 2 double-double X;
+// kVerifyPass: sampled items of the FP64 pass walked in mode 2.
 constexpr int kVerifyPass = 3;
-// FP64 VALU lane-instructions per second of one MI355X at full issue
-// (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz): converts the iteration time
-// target into a work budget.
-constexpr double kLaneInstrPerSec = 256.0 * 4 * 16 * 2.4e9;
-// Slot of one resident matrix: column store (2(n-1)+1) n and x0 with its
-// low parts (2n), for n <= kMaxBatchN, rounded up to 256 bytes.
-constexpr size_t kSlotDoubles =
-    ((size_t)(2 * (perm::kMaxBatchN - 1) + 1) * perm::kMaxBatchN + 2 * perm::kMaxBatchN + 31) /
-    32 * 32;
+// Request block: kPassSlots header slots, the sampled ids, the column store
+// and start vector, the item records.
+constexpr size_t kHeaderBytes = 256;
+constexpr int kPassSlots = 3;  // FP64; then double-double products, or verification then X
+constexpr size_t kIdsBytes = (perm::kVerifyUnits * sizeof(uint32_t) + 255) / 256 * 256;
+constexpr size_t kDataOff = kPassSlots * kHeaderBytes + kIdsBytes;
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
 // A submitted matrix as the caller gave it (copied), prepared by the pool.
 struct RawMatrix {
@@ -82,22 +97,42 @@ struct RawMatrix {
   std::vector<int> cptrs, rows;
 };
 
+struct Request;
+
+// One pass of a request: a header (and records) in one arena block.
+struct Pass {
+  Request* r = nullptr;
+  int kind = 0;          // 0, 1, 2, kVerifyPass
+  uint32_t items = 0;
+  uint64_t seq = 0;      // publication sequence
+  uint64_t tag = 0;      // the completion record carries tag + 1
+  uint32_t done_slot = 0;
+  bool first = false;    // the request's first pass (uploads its whole block)
+  int slot = 0;          // header slot in the request's block
+  uint64_t up_v = 0;     // virtual arena position and size of the upload
+  size_t up_bytes = 0;
+};
+
 struct Request {
   uint64_t id = 0;
-  RawMatrix raw;  // until prepared
-  int n = 0, variant = -1, walk_bits = 0, high_bits = 0;
-  int mode = 0;  // 0 FP64 items; 1, 2 a pass in a double-double mode (escalation);
-                 // kVerifyPass: sampled items in mode 2 measuring the FP64 error
-  perm::NestShape shape{0, 0, 0};
+  RawMatrix raw;   // until prepared
+  int n = 0;       // caller's order
+  int n_eff = 0;   // order walked: n, or 8 for an embedded n < 8
+  int n_pad = 0;   // served kernel order
+  int walk_bits = 0, high_bits = 0;
+  uint32_t walk_items = 0, groups = 0;
+  perm::NestShape shape{0, 0, 0};    // walk shape, padded-order terms
+  perm::NestShape shape_n{0, 0, 0};  // the same shape in terms of n_eff (reported)
   bool integral = false;
-  perm::Prepared prep;
-  int slot = -1;
-  uint32_t items = 0, issued = 0, done = 0;
-  uint32_t walk_items = 0;  // items of the whole walk (items: of the current pass)
-  double item_cost = 0.0;  // lane-instructions per item (scheduling only)
-  std::vector<Partial> parts;
-  // sampled verification: the FP64 pass's result and item records, the
-  // sampled item indices, their weights and how many were certain
+  double product_bits = 1e9;
+  double cost = 0.0;     // FP64 lane-instructions of the FP64 pass (admission)
+  std::vector<double> cs, x0;  // padded column store and start vector (until uploaded)
+  uint64_t vblock = 0;   // virtual arena position of the request's block
+  size_t block_bytes = 0;
+  size_t block = 0;      // block index in allocation order
+  size_t cs_doubles = 0;
+  int passes = 0;        // passes staged so far
+  bool resident = false;
   bool sampled = false;
   perm_result fp_result{};
   std::vector<Partial> fp_parts;
@@ -109,25 +144,30 @@ struct Request {
   int status = PERM_OK;
   std::string error;
   bool finished = false;
-  double remaining_cost() const { return (double)(items - issued) * item_cost; }
 };
 
-struct Segment {
-  Request* r;
-  uint32_t first, count, pos;  // items [first, first+count) of r at iteration position pos
+// One persistent kernel instance: its stream, graph, the events around its
+// last launch, and where the arena's live data began when it started.
+struct KernelSlot {
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool launched = false;  // a launch was made and not yet accounted
+  uint64_t launch_vfront = 0;
 };
 
-struct Iteration {
-  BatchMatrix* h_mats = nullptr;
-  Partial* h_parts = nullptr;
-  BatchMatrix* d_mats = nullptr;
-  Partial* d_parts = nullptr;
-  uint32_t* d_counters = nullptr;
-  void* dmem = nullptr;
-  void* hmem = nullptr;
-  hipEvent_t start = nullptr, done = nullptr;  // timed: the iteration's GPU span
-  std::vector<Segment> segs;
-  bool busy = false;
+// Arena block: virtual positions grow forever; the physical offset is the
+// position modulo the arena size (a block never straddles the end).
+struct Block {
+  uint64_t vstart = 0, size = 0;
+  bool live = true;
+};
+
+// One upload round: passes staged together, and the event after their copies.
+struct Round {
+  std::vector<Pass*> passes;
+  hipEvent_t ev = nullptr;
 };
 
 }  // namespace
@@ -135,14 +175,11 @@ struct Iteration {
 struct perm_server {
   perm_server_config cfg{};
   int device = 0;
-  size_t max_items = 0;
-  double iter_cost = 0.0;
   std::thread worker;
   std::mutex mu;
-  std::condition_variable cv_submit, cv_done;
+  std::condition_variable cv_submit, cv_done, cv_prep;
   std::deque<Request*> to_prep;                  // submitted, not prepared (mu)
   std::deque<Request*> pending;                  // prepared, not admitted (mu)
-  std::condition_variable cv_prep;
   std::vector<std::thread> preppers;
   int prepping = 0;                              // requests taken by a prep thread (mu)
   std::unordered_map<uint64_t, Request*> table;  // every uncollected request (mu)
@@ -151,26 +188,29 @@ struct perm_server {
   int fatal = PERM_OK;  // scheduler error (mu): every later call fails with it
   std::string fatal_msg;
   perm_server_stats stats{};  // (mu)
-  // scheduler-owned
-  std::vector<Request*> active;
-  // free arena slots, smallest first (requests admitted together get
-  // neighbouring slots, uploaded by one copy)
-  std::priority_queue<int, std::vector<int>, std::greater<int>> free_slots;
+  // ---- scheduler-owned ----
+  char* d_arena = nullptr;
+  char* h_arena = nullptr;     // pinned mirror (upload staging at the same offsets)
+  uint64_t arena_bytes = 0;
+  std::deque<Block> blocks;    // blocks in allocation order, from index block_base
+  size_t block_base = 0;
+  uint64_t vhead = 0;          // virtual position of the next block
+  ServeControl* ctl = nullptr;  // pinned, coherent
+  ServeDone* done = nullptr;    // pinned, coherent, kServeRing records
+  ServeState* d_state = nullptr;
+  uint64_t next_seq = 0, next_tag = 1, next_done = 0;
+  int resident = 0;
   double queued_cost = 0.0;
-  // work of the admitted items not yet issued (the sum of remaining_cost()
-  // over the active list), kept as items are admitted, requeued and issued
-  double unissued_cost = 0.0;
-  // scheduler time per phase (ms) and upload copies: written by the
-  // scheduler only, read by perm_server_stats_get
-  std::atomic<double> t_admit{0.0}, t_issue{0.0}, t_retire{0.0}, t_wait{0.0}, t_gpu{0.0};
-  std::atomic<long long> n_uploads{0};
-  std::atomic<long long> n_dd{0};  // requests given a double-double pass
-  double* d_arena = nullptr;
-  double* h_arena = nullptr;  // pinned mirror of the arena (upload staging)
-  Iteration it[kMaxInflight];
-  int head = 0, n_busy = 0;  // oldest busy iteration, number busy
-  hipStream_t s0 = nullptr, s[kMaxStreams] = {};
-  hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams] = {};
+  std::vector<Request*> active;   // admitted, not finished
+  std::vector<Pass*> to_stage;    // written to the mirror, not copied yet
+  std::deque<Round> rounds;       // copies in flight, in order
+  std::vector<hipEvent_t> free_events;
+  std::vector<Pass*> inflight;    // published, not completed
+  hipStream_t s_copy = nullptr;
+  KernelSlot K[2];  // [0] the main kernel, [1] the double-double X kernel
+  std::atomic<double> t_admit{0.0}, t_publish{0.0}, t_retire{0.0}, t_stop{0.0}, t_gpu{0.0};
+  std::atomic<long long> n_uploads{0}, n_dd{0}, n_launches{0}, n_items{0}, n_stops{0};
+  std::atomic<uint64_t> vhead_seen{0};  // vhead, for the statistics
 };
 
 namespace {
@@ -182,11 +222,303 @@ namespace {
       return perm::fail(PERM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
   } while (0)
 
-size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+// PERMAN_SERVE_TRACE=1: one stderr line per scheduler event (diagnostics).
+bool trace_on() {
+  static const bool on = perm::env_int("PERMAN_SERVE_TRACE", 0) != 0;
+  return on;
+}
+#define SERVE_TRACE(...)                                                                   \
+  do {                                                                                     \
+    if (trace_on()) {                                                                      \
+      std::fprintf(stderr, "[serve %.3f ms] ",                                             \
+                   std::chrono::duration<double, std::milli>(                              \
+                       Clock::now().time_since_epoch()).count());                          \
+      std::fprintf(stderr, __VA_ARGS__);                                                   \
+      std::fprintf(stderr, "\n");                                                          \
+    }                                                                                      \
+  } while (0)
 
+void add_ms(std::atomic<double>& acc, Clock::time_point t0) {
+  const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+  acc.store(acc.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
+}
+void bump(std::atomic<long long>& c, long long v = 1) {
+  c.store(c.load(std::memory_order_relaxed) + v, std::memory_order_relaxed);
+}
 
-// Publish a retired iteration's finished requests under one lock, one wake-up.
-void publish_all(perm_server* S, const std::vector<Request*>& rs) {
+// FP64 VALU lane-instructions per subset of a pass (admission estimate).
+double subset_cost(int n_pad, perm::NestShape s, int kind) {
+  if (kind == 2 || kind == kVerifyPass) return 11.0 * n_pad + 8.0;
+  const double fp64 = (double)perm::nest_cost(n_pad, s) / 8.0;
+  if (kind == 0) return fp64;
+  const double half = (fp64 - 2.0) / 2.0;  // adds = multiplies
+  return half + 3.0 * half + 8.0;
+}
+
+double pass_cost(const Request* r, int kind, uint32_t items) {
+  return (double)items * r->groups * std::ldexp((double)perm::kWave, r->walk_bits) *
+         subset_cost(r->n_pad, r->shape, kind);
+}
+
+// ---- the persistent kernels ---------------------------------------------
+
+bool is_dd(int kind) { return kind == 2 || kind == kVerifyPass; }
+
+bool launch_live(perm_server* S, int k) {
+  return S->K[k].launched && hipEventQuery(S->K[k].e1) == hipErrorNotReady;
+}
+
+// Account an ended launch's GPU time.
+void account_launch(perm_server* S, int k) {
+  KernelSlot& K = S->K[k];
+  if (!K.launched) return;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, K.e0, K.e1) == hipSuccess)
+    S->t_gpu.store(S->t_gpu.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
+  K.launched = false;
+}
+
+uint64_t vfront(const perm_server* S) {
+  return S->blocks.empty() ? S->vhead : S->blocks.front().vstart;
+}
+
+// Replay the graph of kernel k.  Its waves start scanning at the oldest
+// pass of its kind in flight (written before the launch).
+int launch(perm_server* S, int k) {
+  KernelSlot& K = S->K[k];
+  account_launch(S, k);
+  uint64_t head = S->next_seq;
+  for (const Pass* p : S->inflight)
+    if (is_dd(p->kind) == (k == 1)) head = std::min(head, p->seq);
+  __atomic_store_n(k == 1 ? &S->ctl->head_dd : &S->ctl->head, head, __ATOMIC_RELEASE);
+  K.launch_vfront = vfront(S);
+  HIP_TRYS(hipEventRecord(K.e0, K.s));
+  // the device copy of the stop flag starts clear (ordered before the
+  // kernel on its stream; outside the captured graph).  No launch is stopping
+  // here: stop_kernels waits for the launches it stops.
+  HIP_TRYS(hipMemsetAsync(&S->d_state->stop, 0, sizeof(uint64_t), K.s));
+  HIP_TRYS(hipGraphLaunch(K.exec, K.s));
+  HIP_TRYS(hipEventRecord(K.e1, K.s));
+  K.launched = true;
+  bump(S->n_launches);
+  SERVE_TRACE("launch kernel %d head %llu", k, (unsigned long long)head);
+  return PERM_OK;
+}
+
+// Stop the live launches (waves finish their current item, then leave).
+int stop_kernels(perm_server* S) {
+  if (!S->K[0].launched && !S->K[1].launched) return PERM_OK;
+  const Clock::time_point t0 = Clock::now();
+  __atomic_store_n(&S->ctl->stop, 1ull, __ATOMIC_RELEASE);
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2; ++k)
+    if (S->K[k].launched && e == hipSuccess) e = hipEventSynchronize(S->K[k].e1);
+  __atomic_store_n(&S->ctl->stop, 0ull, __ATOMIC_RELEASE);
+  if (e != hipSuccess)
+    return perm::fail(PERM_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+  account_launch(S, 0);
+  account_launch(S, 1);
+  bump(S->n_stops);
+  SERVE_TRACE("stopped the kernels");
+  add_ms(S->t_stop, t0);
+  return PERM_OK;
+}
+
+// Launch the kernels the passes in flight need.  The double-double X kernel
+// (1 wave per SIMD) is launched before the main one (4 waves per SIMD takes
+// every register file it is given): a live main launch is stopped first and
+// launched again after it.
+int ensure_kernels(perm_server* S) {
+  bool main_work = false, dd_work = false;
+  for (const Pass* p : S->inflight) (is_dd(p->kind) ? dd_work : main_work) = true;
+  if (dd_work && !launch_live(S, 1)) {
+    if (launch_live(S, 0))
+      if (int s = stop_kernels(S)) return s;
+    if (int s = launch(S, 1)) return s;
+  }
+  if (main_work && !launch_live(S, 0))
+    if (int s = launch(S, 0)) return s;
+  return PERM_OK;
+}
+
+// ---- arena ring -----------------------------------------------------------
+
+// Reserve a block of `size` bytes (a multiple of 256): its virtual position,
+// or UINT64_MAX when the ring has no room.  Space that the live launch may
+// have read is reused only after stopping that launch.
+int arena_alloc(perm_server* S, uint64_t size, uint64_t& vpos) {
+  vpos = UINT64_MAX;
+  const uint64_t A = S->arena_bytes;
+  if (size > A) return PERM_OK;
+  uint64_t v = S->vhead;
+  if (v % A + size > A) v = (v / A + 1) * A;  // blocks never straddle the end
+  if (v + size - vfront(S) > A) return PERM_OK;  // would overwrite live data
+  // the block reuses space last used at positions [v - A, v + size - A): a
+  // live launch that began before that space was freed may have read it
+  if (v + size > A)
+    for (int k = 0; k < 2; ++k)
+      if (v + size - A > S->K[k].launch_vfront && launch_live(S, k)) {
+        if (int s = stop_kernels(S)) return s;
+        break;
+      }
+  S->vhead = v + size;
+  S->vhead_seen.store(S->vhead, std::memory_order_relaxed);
+  S->blocks.push_back({v, size, true});
+  vpos = v;
+  return PERM_OK;
+}
+
+void arena_free(perm_server* S, size_t block) {
+  S->blocks[block - S->block_base].live = false;
+  while (!S->blocks.empty() && !S->blocks.front().live) {
+    S->blocks.pop_front();
+    ++S->block_base;
+  }
+}
+
+// ---- passes ---------------------------------------------------------------
+
+// Stage pass `kind` of r.  The first pass reserves the request's block and
+// writes header 0, the column store and the start vector to the mirror;
+// a later pass writes header slot r->passes (and the sampled ids of a
+// verification pass).  Returns nullptr (st = PERM_OK) when the ring has no
+// room for a first pass now.
+Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
+  st = PERM_OK;
+  const bool first = !r->resident;
+  const uint32_t items = kind == kVerifyPass ? (uint32_t)r->vids.size() : r->walk_items;
+  const int vmode = kind == kVerifyPass ? 2 : kind;
+  const perm::NestShape vshape =
+      vmode == 2 ? perm::NestShape{r->n_pad, r->n_pad, r->n_pad} : r->shape;
+  const int variant = perm::serve_variant(r->n_pad, vshape, vmode);
+  if (variant < 0 || r->passes >= kPassSlots || items > r->walk_items ||
+      (kind == kVerifyPass && items > perm::kVerifyUnits)) {  // not expected
+    st = perm::fail(PERM_ERR_ARG, "no served kernel variant or pass slot for this request");
+    return nullptr;
+  }
+  if (first) {
+    const size_t data_bytes = align256((r->cs.size() + r->x0.size()) * sizeof(double));
+    const size_t bytes = kDataOff + data_bytes + align256((size_t)r->walk_items * sizeof(Partial));
+    uint64_t v = 0;
+    if ((st = arena_alloc(S, bytes, v)) != PERM_OK || v == UINT64_MAX) return nullptr;
+    r->vblock = v;
+    r->block_bytes = bytes;
+    r->block = S->block_base + S->blocks.size() - 1;
+    r->cs_doubles = r->cs.size();
+    r->resident = true;
+    char* h = S->h_arena + (size_t)(v % S->arena_bytes);
+    std::memcpy(h + kDataOff, r->cs.data(), r->cs.size() * sizeof(double));
+    std::memcpy(h + kDataOff + r->cs.size() * sizeof(double), r->x0.data(),
+                r->x0.size() * sizeof(double));
+    std::vector<double>().swap(r->cs);
+    std::vector<double>().swap(r->x0);
+  }
+  const size_t off = (size_t)(r->vblock % S->arena_bytes);
+  char* h = S->h_arena + off;
+  char* d = S->d_arena + off;
+  const size_t data_bytes = align256(r->cs_doubles * sizeof(double) + 2 * r->n_pad * sizeof(double));
+  if (kind == kVerifyPass)
+    std::memcpy(h + kPassSlots * kHeaderBytes, r->vids.data(), items * sizeof(uint32_t));
+  const int slot = r->passes++;
+  auto* p = new Pass();
+  p->r = r;
+  p->kind = kind;
+  p->items = items;
+  p->first = first;
+  p->slot = slot;
+  p->done_slot = (uint32_t)(S->next_done++ & (perm::kServeRing - 1));
+  p->tag = S->next_tag++;
+  S->done[p->done_slot].tag = 0;
+  // upload: the whole block (first pass: the records need no upload but a
+  // run of new blocks goes up as one copy), else this header slot through
+  // the ids area
+  p->up_v = first ? r->vblock : r->vblock + slot * kHeaderBytes;
+  p->up_bytes = first ? r->block_bytes : kDataOff - slot * kHeaderBytes;
+  ServeHeader hd;
+  std::memset(&hd, 0, sizeof(hd));
+  hd.colpm = (const double*)(d + kDataOff);
+  hd.x0 = hd.colpm + r->cs_doubles;
+  hd.parts = (Partial*)(d + kDataOff + data_bytes);
+  hd.ids = kind == kVerifyPass ? (const uint32_t*)(d + kPassSlots * kHeaderBytes) : nullptr;
+  hd.lanes = 1ull << r->high_bits;
+  hd.walk_bits = r->walk_bits;
+  hd.high_bits = r->high_bits;
+  hd.variant = variant | (vmode == 2 ? (int32_t)perm::kServeDDFlag : 0);
+  hd.items = items;
+  hd.groups = r->groups;
+  hd.done_slot = p->done_slot;
+  hd.tag = p->tag;
+  std::memcpy(h + slot * kHeaderBytes, &hd, sizeof(hd));
+  S->queued_cost += pass_cost(r, kind, items);
+  S->to_stage.push_back(p);
+  return p;
+}
+
+// Copy the staged passes to the device -- first passes as runs of adjacent
+// new blocks, one copy per run; a later pass's header (and ids) alone -- and
+// queue the round behind its event.
+int upload_round(perm_server* S) {
+  if (S->to_stage.empty()) return PERM_OK;
+  Round R;
+  R.passes.swap(S->to_stage);
+  const uint64_t A = S->arena_bytes;
+  for (size_t i = 0; i < R.passes.size();) {
+    size_t j = i;
+    while (R.passes[i]->first && j + 1 < R.passes.size() && R.passes[j + 1]->first &&
+           R.passes[j + 1]->up_v == R.passes[j]->up_v + R.passes[j]->up_bytes &&
+           R.passes[j + 1]->up_v % A != 0)
+      ++j;
+    const size_t first = (size_t)(R.passes[i]->up_v % A);
+    const size_t bytes =
+        (size_t)(R.passes[j]->up_v - R.passes[i]->up_v) + R.passes[j]->up_bytes;
+    HIP_TRYS(hipMemcpyAsync(S->d_arena + first, S->h_arena + first, bytes,
+                            hipMemcpyHostToDevice, S->s_copy));
+    bump(S->n_uploads);
+    i = j + 1;
+  }
+  if (S->free_events.empty()) {
+    hipEvent_t e;
+    HIP_TRYS(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    S->free_events.push_back(e);
+  }
+  R.ev = S->free_events.back();
+  S->free_events.pop_back();
+  HIP_TRYS(hipEventRecord(R.ev, S->s_copy));
+  SERVE_TRACE("upload round of %zu passes", R.passes.size());
+  S->rounds.push_back(std::move(R));
+  return PERM_OK;
+}
+
+// Publish every round whose copy has completed (in order); launch the
+// kernel when work is in flight and no launch is live.
+int publish(perm_server* S, bool& progressed) {
+  while (!S->rounds.empty()) {
+    Round& R = S->rounds.front();
+    const hipError_t q = hipEventQuery(R.ev);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess)
+      return perm::fail(PERM_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+    for (Pass* p : R.passes) {
+      p->seq = S->next_seq++;
+      const size_t hoff = (size_t)(p->r->vblock % S->arena_bytes) + (size_t)p->slot * kHeaderBytes;
+      S->ctl->ring[p->seq & (perm::kServeRing - 1)] = (uint64_t)(uintptr_t)(S->d_arena + hoff);
+      S->inflight.push_back(p);
+      bump(S->n_items, p->items);
+    }
+    // the ring entries are visible before the tail that covers them
+    __atomic_store_n(&S->ctl->tail, S->next_seq, __ATOMIC_RELEASE);
+    SERVE_TRACE("published %zu passes, tail %llu", R.passes.size(),
+                (unsigned long long)S->next_seq);
+    S->free_events.push_back(R.ev);
+    S->rounds.pop_front();
+    progressed = true;
+  }
+  return ensure_kernels(S);
+}
+
+// ---- results ----------------------------------------------------------------
+
+void publish_finished(perm_server* S, const std::vector<Request*>& rs) {
   if (rs.empty()) return;
   const Clock::time_point now = Clock::now();
   {
@@ -200,385 +532,216 @@ void publish_all(perm_server* S, const std::vector<Request*>& rs) {
   S->cv_done.notify_all();
 }
 
-// FP64 VALU lane-instructions per subset of a pass in a double-double mode
-// (scheduling estimate only).  Mode 1 over walk shape s: the FP64 adds,
-// three instructions per TwoProd multiply, about 8 for the TwoSum
-// accumulation.  Mode 2 (plain walk): a TwoSum per column addition (7), a
-// double-double multiply step per factor (4), the accumulation (8).
-double dd_subset_cost(int n, perm::NestShape s, int mode) {
-  if (mode == 2) return 11.0 * n + 8.0;
-  const double half = (perm::nest_cost(n, s) / 8.0 - 2.0) / 2.0;  // adds = multiplies
-  return half + 3.0 * half + 8.0;
+// A pass's item records (sampled error measurement).
+int read_parts(perm_server* S, const Pass* p, std::vector<Partial>& out) {
+  const Request* r = p->r;
+  const size_t data_bytes = align256(r->cs_doubles * sizeof(double) + 2 * r->n_pad * sizeof(double));
+  const Partial* parts =
+      (const Partial*)(S->d_arena + (size_t)(r->vblock % S->arena_bytes) + kDataOff + data_bytes);
+  out.resize(p->items);
+  HIP_TRYS(hipMemcpyAsync(out.data(), parts, p->items * sizeof(Partial),
+                          hipMemcpyDeviceToHost, S->s_copy));
+  HIP_TRYS(hipStreamSynchronize(S->s_copy));
+  return PERM_OK;
 }
 
-// Queue r again for a pass in `mode` over `items` items (the verification's
-// sample or the whole walk), its matrix still in its slot.
-void requeue(perm_server* S, Request* r, int mode, uint32_t items) {
-  r->mode = mode;
-  const int kmode = mode == kVerifyPass ? 2 : mode;
-  if (kmode == 2) r->shape = perm::NestShape{r->n, r->n, r->n};
-  r->variant = perm::batch_variant(r->n, r->shape, kmode);
-  r->items = items;
-  r->issued = r->done = 0;
-  r->parts.assign(items, Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
-  r->item_cost =
-      std::ldexp((double)perm::kItemLanes, r->walk_bits) * dd_subset_cost(r->n, r->shape, kmode);
-  S->queued_cost += (double)r->items * r->item_cost;
-  S->unissued_cost += (double)r->items * r->item_cost;
-  S->active.push_back(r);
-  S->n_dd.store(S->n_dd.load(std::memory_order_relaxed) + 1, std::memory_order_relaxed);
-}
-
-// All items of r's current pass are done.  FP64 and double-double passes:
-// sum its partials in item order and state the error.  A non-integer result
-// of many items whose modelled estimate misses the accuracy target first has
-// its error measured on a sample of its items (a kVerifyPass in mode 2, as
-// perm::pps_sample / perm::sampled_error define it); a result that still
-// misses the target, or misses it otherwise (perm::escalation_mode), goes
-// back to the active list for a pass in the more accurate mode over the same
-// slot -- double-double products (same walk shape) for an integer matrix,
-// double-double X (plain walk) for any other -- and this returns false;
-// otherwise the slot is freed and r is ready to publish (true).
-bool finalize(perm_server* S, Request* r) {
-  if (r->mode == kVerifyPass) {
+// Pass p of r is complete, its sum in rec.  Returns the kind of the next
+// pass r needs, or -1 when r is finished (r->result filled).
+int finalize(perm_server* S, Request* r, const Pass* p, const Partial& rec, int& st) {
+  st = PERM_OK;
+  const int n = r->n_eff;
+  if (p->kind == kVerifyPass) {
+    std::vector<Partial> ver;
+    if ((st = read_parts(S, p, ver)) != PERM_OK) return -1;
     std::vector<double> e(r->vids.size());
     for (size_t k = 0; k < r->vids.size(); ++k) {
       const Partial& f = r->fp_parts[r->vids[k]];
-      e[k] = (f.hi - r->parts[k].hi) + (f.lo - r->parts[k].lo);
+      e[k] = (f.hi - ver[k].hi) + (f.lo - ver[k].lo);
     }
     r->result = r->fp_result;
-    r->result.err_est = perm::sampled_error(e, r->vweight, r->vcertain, r->n, r->result.value);
+    r->result.err_est = perm::sampled_error(e, r->vweight, r->vcertain, n, r->result.value);
     r->result.certified = perm::is_certified(r->result) ? 1 : 0;
-    r->fp_parts.clear();
-    r->fp_parts.shrink_to_fit();
-    if (!r->result.certified) {
-      requeue(S, r, 2, r->walk_items);
-      return false;
+    std::vector<Partial>().swap(r->fp_parts);
+    return r->result.certified ? -1 : 2;
+  }
+  perm::fill_item_result(n, r->walk_bits, r->integral, r->product_bits, r->shape_n, p->kind, rec,
+                         r->result);
+  const int next = perm::escalation_mode(r->result);
+  if (next == 2 && p->kind == 0 && !r->sampled && r->walk_items >= perm::kVerifyMinUnits) {
+    std::vector<Partial> parts;
+    if ((st = read_parts(S, p, parts)) != PERM_OK) return -1;
+    std::vector<double> z(parts.size());
+    for (size_t i = 0; i < z.size(); ++i)
+      z[i] = perm::unit_weight(parts[i].sq_sum, parts[i].sq_run, r->walk_bits, r->high_bits);
+    perm::pps_sample(z, perm::kVerifyUnits, r->vids, r->vweight, r->vcertain);
+    if (!r->vids.empty()) {
+      r->sampled = true;
+      r->fp_result = r->result;
+      r->fp_parts.swap(parts);
+      return kVerifyPass;
     }
-  } else {
-    perm::HostDD s = {0.0, 0.0};
-    Partial tot{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (const Partial& p : r->parts) {
-      s = perm::host_dd_add(s, perm::HostDD{p.hi, p.lo});
-      tot.abs_sum += p.abs_sum;
-      tot.sq_sum += p.sq_sum;
-      tot.sq_run += p.sq_run;
+  }
+  return next;
+}
+
+// Completed passes: results, further passes, publication to waiters.
+int retire(perm_server* S, bool& progressed) {
+  std::vector<Request*> finished;
+  size_t keep = 0;
+  int st = PERM_OK;
+  for (size_t i = 0; i < S->inflight.size(); ++i) {
+    Pass* p = S->inflight[i];
+    ServeDone& d = S->done[p->done_slot];
+    if (st != PERM_OK || __atomic_load_n(&d.tag, __ATOMIC_ACQUIRE) != p->tag + 1) {
+      S->inflight[keep++] = p;
+      continue;
     }
-    tot.hi = s.hi;
-    tot.lo = s.lo;
-    perm::fill_item_result(r->n, r->walk_bits, r->integral, r->prep.product_bits, r->shape,
-                           r->mode, tot, r->result);
-    const int next = perm::escalation_mode(r->result);
-    if (next == 2 && r->mode == 0 && !r->sampled && r->walk_items >= perm::kVerifyMinUnits) {
-      std::vector<double> z(r->parts.size());
-      for (size_t i = 0; i < z.size(); ++i)
-        z[i] = perm::unit_weight(r->parts[i].sq_sum, r->parts[i].sq_run, r->walk_bits,
-                                 r->high_bits);
-      perm::pps_sample(z, perm::kVerifyUnits, r->vids, r->vweight, r->vcertain);
-      if (!r->vids.empty()) {
-        r->sampled = true;
-        r->fp_result = r->result;
-        r->fp_parts.swap(r->parts);
-        requeue(S, r, kVerifyPass, (uint32_t)r->vids.size());
-        return false;
-      }
-    }
+    Partial rec = d.sum;
+    rec.pad = 0.0;
+    Request* r = p->r;
+    SERVE_TRACE("pass %llu (request %llu, kind %d) complete", (unsigned long long)p->seq,
+                (unsigned long long)r->id, p->kind);
+    S->queued_cost -= pass_cost(r, p->kind, p->items);
+    const int next = finalize(S, r, p, rec, st);
+    delete p;
+    if (st != PERM_OK) continue;  // reported below, after the list is consistent
     if (next >= 0) {
-      requeue(S, r, next, r->walk_items);
-      return false;
+      // the next pass lives in the request's block: it never waits for space
+      bump(S->n_dd);
+      make_pass(S, r, next, st);
+    } else {
+      r->result.n = r->n;
+      r->result.subsets = std::ldexp(1.0, r->n - 1);
+      arena_free(S, r->block);
+      r->resident = false;
+      S->resident -= 1;
+      std::vector<uint32_t>().swap(r->vids);
+      std::vector<double>().swap(r->vweight);
+      finished.push_back(r);
     }
+    progressed = true;
   }
-  r->parts.clear();
-  r->parts.shrink_to_fit();
-  S->free_slots.push(r->slot);
-  r->slot = -1;
-  r->prep = perm::Prepared();
-  return true;
+  S->inflight.resize(keep);
+  publish_finished(S, finished);
+  return st;
 }
 
-// Copy newly admitted requests into their arena slots: each into the pinned
-// mirror, then one asynchronous copy per run of nearby slots on s0, ahead of
-// any kernel that reads them.  A run may span up to kUploadGapSlots slots
-// that are not fresh: a resident slot's mirror holds the bytes its device
-// copy already has, a free slot's content is unused, so copying them over is
-// harmless and cheaper than another copy call (a slot is ~26 KB).
-constexpr int kUploadGapSlots = 4;
-
-int upload(perm_server* S, std::vector<Request*>& fresh) {
-  std::sort(fresh.begin(), fresh.end(),
-            [](const Request* a, const Request* b) { return a->slot < b->slot; });
-  for (size_t i = 0; i < fresh.size();) {
-    size_t j = i;
-    for (;;) {
-      Request* r = fresh[j];
-      double* h = S->h_arena + (size_t)r->slot * kSlotDoubles;
-      std::memcpy(h, r->prep.cs.data(), r->prep.cs.size() * sizeof(double));
-      std::memcpy(h + r->prep.cs.size(), r->prep.x0.data(), r->prep.x0.size() * sizeof(double));
-      if (j + 1 == fresh.size() || fresh[j + 1]->slot > r->slot + 1 + kUploadGapSlots) break;
-      ++j;
-    }
-    const Request* last = fresh[j];
-    const size_t first = (size_t)fresh[i]->slot * kSlotDoubles;
-    const size_t end = (size_t)last->slot * kSlotDoubles + last->prep.cs.size() + last->prep.x0.size();
-    HIP_TRYS(hipMemcpyAsync(S->d_arena + first, S->h_arena + first, (end - first) * sizeof(double),
-                            hipMemcpyHostToDevice, S->s0));
-    S->n_uploads.store(S->n_uploads.load(std::memory_order_relaxed) + 1,
-                       std::memory_order_relaxed);
-    i = j + 1;
-  }
-  return PERM_OK;
-}
-
-// Admit pending requests in arrival order while slots are free and queued
-// work stays under max_queued_work (the first one always, on an idle server);
-// the pending queue is taken under one lock.
-int admit(perm_server* S) {
-  std::vector<Request*> fresh;
+// Admit pending requests (arrival order) while resident requests, queued
+// work and arena space allow.
+int admit(perm_server* S, bool& progressed) {
+  std::vector<Request*> take;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    size_t slots = S->free_slots.size();
     double queued = S->queued_cost;
-    while (!S->pending.empty() && slots > 0) {
+    int resident = S->resident;
+    while (!S->pending.empty() && resident < S->cfg.max_resident) {
       Request* c = S->pending.front();
-      const double cost = (double)c->items * c->item_cost;
-      if (!(S->active.empty() && fresh.empty()) && queued + cost > S->cfg.max_queued_work) break;
+      if (resident > 0 && queued + c->cost > S->cfg.max_queued_work) break;
       S->pending.pop_front();
-      fresh.push_back(c);
-      queued += cost;
-      --slots;
+      take.push_back(c);
+      queued += c->cost;
+      ++resident;
     }
-    S->stats.admitted += (long long)fresh.size();
   }
-  if (fresh.empty()) return PERM_OK;
-  for (Request* r : fresh) {
-    r->slot = S->free_slots.top();
-    S->free_slots.pop();
-    r->parts.assign(r->items, Partial{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
-    S->queued_cost += (double)r->items * r->item_cost;
-    S->unissued_cost += (double)r->items * r->item_cost;
+  size_t k = 0;
+  for (; k < take.size(); ++k) {
+    int st = PERM_OK;
+    Pass* p = make_pass(S, take[k], 0, st);
+    if (st != PERM_OK) return st;
+    if (!p) break;
+    S->resident += 1;
+    S->active.push_back(take[k]);
   }
-  if (int st = upload(S, fresh)) return st;
-  for (Request* r : fresh) S->active.push_back(r);
+  if (k < take.size()) {  // no arena room: back to the front, in order
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (size_t i = take.size(); i-- > k;) S->pending.push_front(take[i]);
+  }
+  if (k > 0) {
+    progressed = true;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.admitted += (long long)k;
+  }
   return PERM_OK;
 }
 
-// Choose the next iteration's items and launch it on a free context.
-int issue(perm_server* S, bool& issued) {
-  issued = false;
-  if (S->n_busy >= S->cfg.inflight) return PERM_OK;
-  // continuous batching: while an iteration runs, let small work accumulate
-  // into the next one instead of paying an iteration's launch cost for it
-  // (checked on the running total, before any scan of the active list)
-  if (S->n_busy > 0 && S->unissued_cost < S->iter_cost / 16.0) return PERM_OK;
-  std::vector<Request*> cand;
-  double work = 0.0;
-  for (Request* r : S->active)
-    if (r->issued < r->items) {
-      cand.push_back(r);
-      work += r->remaining_cost();
-    }
-  if (cand.empty()) {
-    S->unissued_cost = 0.0;  // drop the rounding drift of the running total
-    return PERM_OK;
-  }
-  S->unissued_cost = work;
-  // shortest remaining work first, except that requests waiting longer than
-  // max_wait_ms go ahead, oldest first (no large request starves behind a
-  // stream of small ones)
-  const Clock::time_point aged_before =
-      Clock::now() - std::chrono::microseconds((int64_t)(S->cfg.max_wait_ms * 1e3));
-  std::sort(cand.begin(), cand.end(), [&](const Request* a, const Request* b) {
-    const bool oa = a->t_submit < aged_before, ob = b->t_submit < aged_before;
-    if (oa != ob) return oa;
-    if (oa) return a->id < b->id;
-    const double ca = a->remaining_cost(), cb = b->remaining_cost();
-    return ca != cb ? ca < cb : a->id < b->id;
-  });
-  Iteration& I = S->it[(S->head + S->n_busy) % S->cfg.inflight];
-  I.segs.clear();
-  double budget = S->iter_cost;
-  size_t cap = S->max_items;
-  for (Request* r : cand) {
-    if (budget <= 0.0 || cap == 0) break;
-    uint64_t take = r->items - r->issued;
-    const uint64_t by_budget = (uint64_t)std::ceil(budget / r->item_cost);
-    take = std::min<uint64_t>(take, std::max<uint64_t>(1, by_budget));
-    take = std::min<uint64_t>(take, cap);
-    I.segs.push_back({r, r->issued, (uint32_t)take, 0});
-    r->issued += (uint32_t)take;
-    budget -= (double)take * r->item_cost;
-    S->unissued_cost -= (double)take * r->item_cost;
-    cap -= take;
-  }
-  // group by variant: segments of one variant are contiguous in the item list
-  std::stable_sort(I.segs.begin(), I.segs.end(), [](const Segment& a, const Segment& b) {
-    return a.r->variant < b.r->variant;
-  });
-  struct Group {
-    int vid;
-    uint32_t begin, end;
-  };
-  std::vector<Group> groups;
-  uint32_t pos = 0;
-  // staging: matrix records, then (from the next 256-byte boundary) items
-  const size_t mats_bytes = align256(I.segs.size() * sizeof(BatchMatrix));
-  BatchItem* h_items = (BatchItem*)((char*)I.h_mats + mats_bytes);
-  for (size_t m = 0; m < I.segs.size(); ++m) {
-    Segment& g = I.segs[m];
-    Request* r = g.r;
-    g.pos = pos;
-    I.h_mats[m] = {S->d_arena + (size_t)r->slot * kSlotDoubles,
-                   S->d_arena + (size_t)r->slot * kSlotDoubles + r->prep.cs.size(),
-                   1ull << r->high_bits, r->walk_bits, r->high_bits, r->variant, 0u, 0u, 0u};
-    // the item's index in the walk (a verification pass runs the sampled ones)
-    for (uint32_t i = 0; i < g.count; ++i) {
-      const uint64_t item = r->mode == kVerifyPass ? r->vids[g.first + i] : g.first + i;
-      h_items[pos + i] = {(uint32_t)m, 0u, item * perm::kItemLanes};
-    }
-    if (groups.empty() || groups.back().vid != r->variant)
-      groups.push_back({r->variant, pos, pos + g.count});
-    else
-      groups.back().end = pos + g.count;
-    pos += g.count;
-  }
-  const uint32_t n_items = pos;
-  hipStream_t s0 = S->s0;
-  // one upload: the matrix records, then the items
-  BatchItem* d_items = (BatchItem*)((char*)I.d_mats + mats_bytes);
-  HIP_TRYS(hipEventRecord(I.start, s0));
-  HIP_TRYS(hipMemcpyAsync(I.d_mats, I.h_mats, mats_bytes + n_items * sizeof(BatchItem),
-                          hipMemcpyHostToDevice, s0));
-  HIP_TRYS(perm::launch_batch_reset(I.d_counters, (uint32_t)groups.size(), s0));
-  // variants run concurrently on the streams they need (one group: on s0)
-  const int ns = (int)std::min<size_t>(groups.size() > 1 ? groups.size() : 0,
-                                       (size_t)std::max(1, std::min(S->cfg.streams, kMaxStreams)));
-  if (ns > 0) {
-    HIP_TRYS(hipEventRecord(S->ev_fork, s0));
-    for (int i = 0; i < ns; ++i) HIP_TRYS(hipStreamWaitEvent(S->s[i], S->ev_fork, 0));
-  }
-  for (size_t gi = 0; gi < groups.size(); ++gi) {
-    const Group& G = groups[gi];
-    const uint32_t waves = G.end - G.begin;
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, 2048u));
-    HIP_TRYS(perm::launch_batch_variant(G.vid, I.d_mats, d_items, G.begin, G.end,
-                                        I.d_counters + gi, I.d_parts, grid,
-                                        ns > 0 ? S->s[gi % ns] : s0));
-  }
-  for (int i = 0; i < ns; ++i) {
-    HIP_TRYS(hipEventRecord(S->ev_join[i], S->s[i]));
-    HIP_TRYS(hipStreamWaitEvent(s0, S->ev_join[i], 0));
-  }
-  HIP_TRYS(hipMemcpyAsync(I.h_parts, I.d_parts, n_items * sizeof(Partial),
-                          hipMemcpyDeviceToHost, s0));
-  HIP_TRYS(hipEventRecord(I.done, s0));
-  I.busy = true;
-  S->n_busy += 1;
-  issued = true;
-  std::lock_guard<std::mutex> lk(S->mu);
-  S->stats.iterations += 1;
-  S->stats.items += n_items;
-  return PERM_OK;
-}
-
-// Retire the oldest iteration if it is done (wait = block until it is).
-int retire(perm_server* S, bool wait, bool& retired) {
-  retired = false;
-  if (S->n_busy == 0) return PERM_OK;
-  Iteration& I = S->it[S->head];
-  if (wait) {
-    HIP_TRYS(hipEventSynchronize(I.done));
-  } else {
-    const hipError_t q = hipEventQuery(I.done);
-    if (q == hipErrorNotReady) return PERM_OK;
-    if (q != hipSuccess)
-      return perm::fail(PERM_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
-  }
-  float gpu_ms = 0.f;
-  if (hipEventElapsedTime(&gpu_ms, I.start, I.done) == hipSuccess)
-    S->t_gpu.store(S->t_gpu.load(std::memory_order_relaxed) + gpu_ms, std::memory_order_relaxed);
-  std::vector<Request*> completed, finished;
-  for (const Segment& g : I.segs) {
-    Request* r = g.r;
-    std::copy(I.h_parts + g.pos, I.h_parts + g.pos + g.count, r->parts.begin() + g.first);
-    r->done += g.count;
-    S->queued_cost -= (double)g.count * r->item_cost;
-    if (r->done == r->items) completed.push_back(r);
-  }
-  if (!completed.empty())  // drop them from the active list in one pass, first
-    S->active.erase(std::remove_if(S->active.begin(), S->active.end(),
-                                   [](const Request* r) { return r->done == r->items; }),
-                    S->active.end());
-  for (Request* r : completed)
-    if (finalize(S, r)) finished.push_back(r);
-  publish_all(S, finished);
-  I.segs.clear();
-  I.busy = false;
-  S->head = (S->head + 1) % S->cfg.inflight;
-  S->n_busy -= 1;
-  retired = true;
-  return PERM_OK;
-}
-
-void add_ms(std::atomic<double>& acc, Clock::time_point t0) {
-  const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-  acc.store(acc.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
-}
-
-int scheduler_step(perm_server* S, bool& idle) {
-  idle = false;
+int scheduler_step(perm_server* S, bool& progressed) {
   Clock::time_point t0 = Clock::now();
-  if (int st = admit(S)) return st;
+  if (int st = admit(S, progressed)) return st;
+  if (int st = upload_round(S)) return st;
   add_ms(S->t_admit, t0);
   t0 = Clock::now();
-  bool issued = false, retired = false;
-  if (int st = issue(S, issued)) return st;
-  add_ms(S->t_issue, t0);
+  if (int st = publish(S, progressed)) return st;
+  add_ms(S->t_publish, t0);
   t0 = Clock::now();
-  if (int st = retire(S, false, retired)) return st;
+  if (int st = retire(S, progressed)) return st;
   add_ms(S->t_retire, t0);
-  if (issued || retired) return PERM_OK;
-  if (S->n_busy >= S->cfg.inflight) {  // GPU queue full
-    t0 = Clock::now();
-    const int st = retire(S, true, retired);
-    add_ms(S->t_wait, t0);
-    return st;
-  }
-  if (S->n_busy > 0) {
-    // nothing to issue: wait briefly for the GPU or a new request
-    std::unique_lock<std::mutex> lk(S->mu);
-    S->cv_submit.wait_for(lk, std::chrono::microseconds(100),
-                          [&] { return S->stop || !S->pending.empty(); });
-    return PERM_OK;
-  }
-  idle = true;
   return PERM_OK;
+}
+
+// Diagnostics (PERMAN_SERVE_TRACE): every 50 ms while passes are in flight,
+// the device-side state and the oldest pass's counters.
+void trace_device(perm_server* S) {
+  static Clock::time_point last;
+  const Clock::time_point now = Clock::now();
+  if (now - last < std::chrono::milliseconds(50)) return;
+  last = now;
+  uint64_t st[8] = {};
+  uint32_t ctr[2] = {};
+  const Pass* p = S->inflight.front();
+  const char* hdr = S->d_arena + (size_t)(p->r->vblock % S->arena_bytes) +
+                    (size_t)p->slot * kHeaderBytes + offsetof(ServeHeader, next_item);
+  if (hipMemcpyAsync(st, S->d_state, sizeof(st), hipMemcpyDeviceToHost, S->s_copy) != hipSuccess ||
+      hipMemcpyAsync(ctr, hdr, sizeof(ctr), hipMemcpyDeviceToHost, S->s_copy) != hipSuccess ||
+      hipStreamSynchronize(S->s_copy) != hipSuccess)
+    return;
+  uint64_t rt = 0;
+  SERVE_TRACE("device: head %llu head_dd %llu pub %llu poll %llu activity %llu stop %llu | "
+              "pass %llu claimed %u done %u of %u | host tail %llu live %d%d",
+              (unsigned long long)st[0], (unsigned long long)st[1], (unsigned long long)st[2],
+              (unsigned long long)st[3], (unsigned long long)st[4], (unsigned long long)st[5],
+              (unsigned long long)p->seq, ctr[0], ctr[1], p->items,
+              (unsigned long long)S->ctl->tail, (int)launch_live(S, 0), (int)launch_live(S, 1));
+  (void)rt;
+}
+
+bool drained(perm_server* S) {
+  return S->inflight.empty() && S->rounds.empty() && S->to_stage.empty();
+}
+
+void fail_all(perm_server* S, int st) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->fatal = st;
+  S->fatal_msg = perm::last_error();
+  for (auto& kv : S->table)
+    if (!kv.second->finished) {
+      kv.second->status = st;
+      kv.second->error = S->fatal_msg;
+      kv.second->finished = true;
+    }
+  S->cv_done.notify_all();
 }
 
 void scheduler_loop(perm_server* S) {
   for (;;) {
-    bool idle = false;
-    const int st = scheduler_step(S, idle);
+    bool progressed = false;
+    const int st = scheduler_step(S, progressed);
     if (st != PERM_OK) {
       // a HIP failure leaves the device state unknown: fail everything
-      std::lock_guard<std::mutex> lk(S->mu);
-      S->fatal = st;
-      S->fatal_msg = perm::last_error();
-      for (auto& kv : S->table)
-        if (!kv.second->finished) {
-          kv.second->status = st;
-          kv.second->error = S->fatal_msg;
-          kv.second->finished = true;
-        }
-      S->cv_done.notify_all();
+      fail_all(S, st);
       return;
     }
-    if (idle) {
-      std::unique_lock<std::mutex> lk(S->mu);
-      if (S->stop && S->to_prep.empty() && S->prepping == 0 && S->pending.empty() &&
-          S->active.empty())
-        return;
-      S->cv_submit.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+    if (progressed) continue;
+    if (trace_on() && !S->inflight.empty()) trace_device(S);
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (!drained(S)) {
+      // work on the GPU or copies in flight: look again shortly (or at once
+      // on a new request)
+      S->cv_submit.wait_for(lk, std::chrono::microseconds(20),
+                            [&] { return S->stop || !S->pending.empty(); });
+      continue;
     }
+    if (S->stop && S->to_prep.empty() && S->prepping == 0 && S->pending.empty()) return;
+    S->cv_submit.wait(lk, [&] { return S->stop || !S->pending.empty(); });
   }
 }
 
@@ -586,16 +749,69 @@ void scheduler_main(perm_server* S) {
   perm::set_device(S->device);
   (void)hipSetDevice(S->device);
   scheduler_loop(S);
+  // every request is finished: end the live launches
+  (void)stop_kernels(S);
 }
 
 size_t dtype_size(int dtype) { return dtype == PERM_DTYPE_F64 ? 8 : 4; }
 
+// A raw matrix of order n < 8 embedded as A (+) I of order 8 (same
+// permanent), dense and (if sparse) CCS, in the raw dtype.
+void embed_identity(RawMatrix& m, int n) {
+  constexpr int E = perm::kServeMinN;
+  const size_t es = dtype_size(m.dtype);
+  std::vector<char> mat((size_t)E * E * es, 0);
+  auto put_one = [&](char* dst) {
+    if (m.dtype == PERM_DTYPE_F64) {
+      const double one = 1.0;
+      std::memcpy(dst, &one, 8);
+    } else if (m.dtype == PERM_DTYPE_F32) {
+      const float one = 1.0f;
+      std::memcpy(dst, &one, 4);
+    } else {
+      const int one = 1;
+      std::memcpy(dst, &one, 4);
+    }
+  };
+  for (int i = 0; i < n; ++i)
+    std::memcpy(mat.data() + (size_t)i * E * es, m.mat.data() + (size_t)i * n * es, n * es);
+  for (int i = n; i < E; ++i) put_one(mat.data() + ((size_t)i * E + i) * es);
+  m.mat.swap(mat);
+  if (m.sparse) {
+    std::vector<int> cptrs(E + 1), rows;
+    std::vector<char> cvals;
+    for (int c = 0; c < E; ++c) {
+      cptrs[c] = (int)rows.size();
+      if (c < n) {
+        for (int e = m.cptrs[c]; e < m.cptrs[c + 1]; ++e) {
+          rows.push_back(m.rows[e]);
+          cvals.insert(cvals.end(), m.cvals.begin() + (size_t)e * es,
+                       m.cvals.begin() + (size_t)(e + 1) * es);
+        }
+      } else {
+        rows.push_back(c);
+        cvals.resize(cvals.size() + es);
+        put_one(cvals.data() + cvals.size() - es);
+      }
+    }
+    cptrs[E] = (int)rows.size();
+    m.cptrs.swap(cptrs);
+    m.rows.swap(rows);
+    m.cvals.swap(cvals);
+  }
+}
+
 // Prepare r from its raw input (a prep thread, no lock held): start vector,
-// column store, split, walk shape, variant.  Returns PERM_OK or an error
-// status (r->error set).
+// column store and walk shape, padded to the served order; walk and item
+// split.  Returns PERM_OK or an error status (r->error set).
 int prepare_request(perm_server* S, Request* r) {
-  const RawMatrix& in = r->raw;
-  const int n = r->n;
+  RawMatrix& in = r->raw;
+  int n = r->n;
+  if (n < perm::kServeMinN) {
+    embed_identity(in, n);
+    n = perm::kServeMinN;
+  }
+  r->n_eff = n;
   perm::Prepared p;
   int st = in.sparse ? perm::prepare_sparse(in.dtype, in.mat.data(), in.cptrs.data(),
                                             in.rows.data(), in.cvals.data(), n, p,
@@ -606,25 +822,37 @@ int prepare_request(perm_server* S, Request* r) {
     return st;
   }
   r->integral = p.integral;
+  r->product_bits = p.product_bits;
+  if (p.zero) {
+    perm::fill_zero_result(r->n, r->integral, r->result);
+    r->raw = RawMatrix();
+    return PERM_OK;
+  }
+  const int N = perm::serve_padded_n(n);
+  r->n_pad = N;
+  // walk shape: column l < 3 touches rows [0, k_l); rows from n on are the
+  // padding (untouched, like the matrix's own rows beyond k2)
+  const int k0 = p.nu > 0 && p.nu < n ? p.nu : n, k1 = p.nu1 > 0 && p.nu1 < n ? p.nu1 : n,
+            k2 = p.nu2 > 0 && p.nu2 < n ? p.nu2 : n;
+  r->shape_n = {k0, k1, k2};
+  r->shape = {k0 < n ? k0 : N, k1 < n ? k1 : N, k2 < n ? k2 : N};
+  if (perm::serve_variant(N, r->shape, 0) < 0) r->shape = {N, N, N};  // (not expected)
   perm::batch_split(n, S->cfg.item_log2, r->walk_bits, r->high_bits);
   const uint64_t lanes = 1ull << r->high_bits;
-  r->items = (uint32_t)((lanes + perm::kItemLanes - 1) / perm::kItemLanes);
-  r->walk_items = r->items;
-  // pair / nested kernels for requests with real work; tiny ones use the
-  // plain variant (fewer kernels per iteration)
-  r->shape = r->items >= 8 ? perm::batch_shape(n, {p.nu > 0 ? p.nu : n, p.nu1 > 0 ? p.nu1 : n,
-                                                   p.nu2 > 0 ? p.nu2 : n})
-                           : perm::NestShape{n, n, n};
-  r->variant = perm::batch_variant(n, r->shape);
-  if (r->variant < 0 || perm::batch_variant(n, r->shape, 1) < 0 ||
-      perm::batch_variant(n, {n, n, n}, 2) < 0) {
-    r->error = "no batch kernel for this n";
-    return PERM_ERR_ARG;
-  }
-  r->item_cost = std::ldexp((double)perm::kItemLanes, r->walk_bits) *
-                 (double)perm::nest_cost(n, r->shape) / 8.0;
-  if (p.zero) perm::fill_zero_result(n, r->integral, r->result);
-  r->prep = std::move(p);
+  const uint64_t groups64 = (lanes + perm::kWave - 1) / perm::kWave;
+  const uint64_t max_items = (uint64_t)std::max(1, S->cfg.max_items);
+  const uint64_t items0 = std::min<uint64_t>(groups64, max_items);
+  r->groups = (uint32_t)((groups64 + items0 - 1) / items0);
+  r->walk_items = (uint32_t)((groups64 + r->groups - 1) / r->groups);
+  // padded column store: block b of n values -> block b of N values
+  const size_t blocks_n = (size_t)(2 * (n - 1));
+  r->cs.assign((size_t)(2 * (N - 1) + 1) * N, 0.0);
+  for (size_t b = 0; b < blocks_n; ++b)
+    std::copy(p.cs.begin() + b * n, p.cs.begin() + (b + 1) * n, r->cs.begin() + b * N);
+  r->x0.assign(2 * (size_t)N, 0.0);
+  for (int j = 0; j < N; ++j) r->x0[j] = j < n ? p.x0[j] : 1.0;
+  for (int j = 0; j < n; ++j) r->x0[N + j] = p.x0[n + j];
+  r->cost = pass_cost(r, 0, r->walk_items);
   r->raw = RawMatrix();
   return PERM_OK;
 }
@@ -648,9 +876,8 @@ void prep_loop(perm_server* S) {
     {
       std::lock_guard<std::mutex> lk(S->mu);
       S->prepping -= 1;
-      if (st != PERM_OK || r->prep.zero) {  // finished at once
+      if (st != PERM_OK || r->n_pad == 0) {  // finished at once (error, or exactly 0)
         if (st != PERM_OK) r->status = st;
-        r->prep = perm::Prepared();
         r->t_done = Clock::now();
         r->finished = true;
         S->stats.completed += 1;
@@ -670,9 +897,9 @@ int make_request(int dtype, const void* mat, const int* cptrs, const int* rows,
   if (!mat) return perm::fail(PERM_ERR_ARG, "null pointer");
   if (dtype < PERM_DTYPE_I32 || dtype > PERM_DTYPE_F64) return perm::fail(PERM_ERR_ARG, "bad dtype");
   if (nov < 1) return perm::fail(PERM_ERR_ARG, "nov must be >= 1");
-  if (nov > perm::kMaxBatchN)
+  if (nov > perm::kServeMaxN)
     return perm::fail(PERM_ERR_SIZE, "server requests are limited to n <= " +
-                                         std::to_string(perm::kMaxBatchN));
+                                         std::to_string(perm::kServeMaxN));
   auto* r = new Request();
   r->n = nov;
   RawMatrix& m = r->raw;
@@ -760,55 +987,61 @@ int wait_collect(perm_server* S, std::unique_lock<std::mutex>& lk, uint64_t id, 
 }
 
 void release(perm_server* S) {
-  for (auto& I : S->it) {
-    if (I.dmem) (void)hipFree(I.dmem);
-    if (I.hmem) (void)hipHostFree(I.hmem);
-    if (I.done) (void)hipEventDestroy(I.done);
-    if (I.start) (void)hipEventDestroy(I.start);
+  for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
+  for (Round& R : S->rounds) {
+    if (R.ev) (void)hipEventDestroy(R.ev);
+    for (Pass* p : R.passes) delete p;
   }
+  for (Pass* p : S->to_stage) delete p;
+  for (Pass* p : S->inflight) delete p;
+  for (KernelSlot& K : S->K) {
+    if (K.exec) (void)hipGraphExecDestroy(K.exec);
+    if (K.graph) (void)hipGraphDestroy(K.graph);
+    if (K.e0) (void)hipEventDestroy(K.e0);
+    if (K.e1) (void)hipEventDestroy(K.e1);
+    if (K.s) (void)hipStreamDestroy(K.s);
+  }
+  if (S->s_copy) (void)hipStreamDestroy(S->s_copy);
+  if (S->d_state) (void)hipFree(S->d_state);
   if (S->d_arena) (void)hipFree(S->d_arena);
   if (S->h_arena) (void)hipHostFree(S->h_arena);
-  for (int i = 0; i < kMaxStreams; ++i) {
-    if (S->s[i]) (void)hipStreamDestroy(S->s[i]);
-    if (S->ev_join[i]) (void)hipEventDestroy(S->ev_join[i]);
-  }
-  if (S->ev_fork) (void)hipEventDestroy(S->ev_fork);
-  if (S->s0) (void)hipStreamDestroy(S->s0);
+  if (S->ctl) (void)hipHostFree(S->ctl);
+  if (S->done) (void)hipHostFree(S->done);
   for (auto& kv : S->table) delete kv.second;
   delete S;
 }
 
 int setup(perm_server* S) {
-  perm_server_config& c = S->cfg;
-  const size_t slots = (size_t)c.max_resident;
-  HIP_TRYS(hipMalloc((void**)&S->d_arena, slots * kSlotDoubles * sizeof(double)));
-  HIP_TRYS(hipHostMalloc((void**)&S->h_arena, slots * kSlotDoubles * sizeof(double),
-                         hipHostMallocDefault));
-  for (size_t i = 0; i < slots; ++i) S->free_slots.push((int)i);
-  HIP_TRYS(hipStreamCreateWithFlags(&S->s0, hipStreamNonBlocking));
-  for (int i = 0; i < kMaxStreams; ++i) {
-    HIP_TRYS(hipStreamCreateWithFlags(&S->s[i], hipStreamNonBlocking));
-    HIP_TRYS(hipEventCreateWithFlags(&S->ev_join[i], hipEventDisableTiming));
-  }
-  HIP_TRYS(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
-  const size_t mi = S->max_items;
-  // [matrix records | items] are staged and uploaded together
-  const size_t mats_b = align256(mi * sizeof(BatchMatrix)) + align256(mi * sizeof(BatchItem));
-  const size_t parts_b = align256(mi * sizeof(Partial));
-  const size_t ctr_b = align256(mi * sizeof(uint32_t));
-  for (int k = 0; k < c.inflight; ++k) {
-    Iteration& I = S->it[k];
-    HIP_TRYS(hipMalloc(&I.dmem, mats_b + parts_b + ctr_b));
-    HIP_TRYS(hipHostMalloc(&I.hmem, mats_b + parts_b, hipHostMallocDefault));
-    char* d = (char*)I.dmem;
-    char* h = (char*)I.hmem;
-    I.d_mats = (BatchMatrix*)d;
-    I.d_parts = (Partial*)(d + mats_b);
-    I.d_counters = (uint32_t*)(d + mats_b + parts_b);
-    I.h_mats = (BatchMatrix*)h;
-    I.h_parts = (Partial*)(h + mats_b);
-    HIP_TRYS(hipEventCreate(&I.done));
-    HIP_TRYS(hipEventCreate(&I.start));
+  const perm_server_config& c = S->cfg;
+  S->arena_bytes = (uint64_t)(c.arena_mb * 1048576.0) / 256 * 256;
+  HIP_TRYS(hipMalloc((void**)&S->d_arena, S->arena_bytes));
+  HIP_TRYS(hipHostMalloc((void**)&S->h_arena, S->arena_bytes, hipHostMallocDefault));
+  HIP_TRYS(hipHostMalloc((void**)&S->ctl, sizeof(ServeControl), hipHostMallocCoherent));
+  std::memset(S->ctl, 0, sizeof(ServeControl));
+  HIP_TRYS(hipHostMalloc((void**)&S->done, perm::kServeRing * sizeof(ServeDone),
+                         hipHostMallocCoherent));
+  std::memset(S->done, 0, perm::kServeRing * sizeof(ServeDone));
+  HIP_TRYS(hipMalloc((void**)&S->d_state, sizeof(ServeState)));
+  HIP_TRYS(hipMemset(S->d_state, 0, sizeof(ServeState)));
+  HIP_TRYS(hipStreamCreateWithFlags(&S->s_copy, hipStreamNonBlocking));
+  HIP_TRYS(hipDeviceSynchronize());
+  // the two persistent kernels, each captured once and replayed per launch;
+  // the double-double X one on 1/16 of the main one's workgroups (each of its
+  // workgroups holds a whole CU's register files)
+  const uint64_t idle_ticks = (uint64_t)std::max(1.0, c.idle_us * 100.0);  // 100 MHz counter
+  for (int k = 0; k < 2; ++k) {
+    KernelSlot& K = S->K[k];
+    HIP_TRYS(hipStreamCreateWithFlags(&K.s, hipStreamNonBlocking));
+    HIP_TRYS(hipEventCreate(&K.e0));
+    HIP_TRYS(hipEventCreate(&K.e1));
+    const uint32_t grid = k == 0 ? (uint32_t)c.grid : (uint32_t)std::max(16, c.grid / 16);
+    HIP_TRYS(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
+    const hipError_t le =
+        perm::launch_serve(k == 1, S->ctl, S->d_state, S->done, idle_ticks, grid, K.s);
+    const hipError_t ce = hipStreamEndCapture(K.s, &K.graph);
+    HIP_TRYS(le);
+    HIP_TRYS(ce);
+    HIP_TRYS(hipGraphInstantiate(&K.exec, K.graph, nullptr, nullptr, 0));
   }
   return PERM_OK;
 }
@@ -819,15 +1052,14 @@ extern "C" {
 
 void perm_server_default_config(perm_server_config* c) {
   if (!c) return;
-  c->item_log2 = 18;
-  c->streams = 4;
-  c->inflight = 2;
-  c->max_items = 32768;
-  c->iteration_us = 8000.0;
-  c->max_resident = 2048;
+  c->item_log2 = 16;
+  c->grid = 1024;
+  c->max_items = (int)perm::kServeMaxItems;
+  c->idle_us = 2000.0;
+  c->max_resident = 4096;
   c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)
   c->prep_threads = 4;
-  c->max_wait_ms = 50.0;
+  c->arena_mb = 256.0;
 }
 
 int perm_server_create(const perm_server_config* cfg, perm_server** out) {
@@ -836,24 +1068,24 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
   perm_server_default_config(&c);
   if (cfg) {
     if (cfg->item_log2 > 0) c.item_log2 = cfg->item_log2;
-    if (cfg->streams > 0) c.streams = cfg->streams;
-    if (cfg->inflight > 0) c.inflight = cfg->inflight;
+    if (cfg->grid > 0) c.grid = cfg->grid;
     if (cfg->max_items > 0) c.max_items = cfg->max_items;
-    if (cfg->iteration_us > 0) c.iteration_us = cfg->iteration_us;
+    if (cfg->idle_us > 0) c.idle_us = cfg->idle_us;
     if (cfg->max_resident > 0) c.max_resident = cfg->max_resident;
     if (cfg->max_queued_work > 0) c.max_queued_work = cfg->max_queued_work;
     if (cfg->prep_threads > 0) c.prep_threads = std::min(cfg->prep_threads, 64);
-    if (cfg->max_wait_ms > 0) c.max_wait_ms = cfg->max_wait_ms;
+    if (cfg->arena_mb > 0) c.arena_mb = cfg->arena_mb;
   }
   if (c.item_log2 < 6 || c.item_log2 > 30) return perm::fail(PERM_ERR_ARG, "item_log2 in [6, 30]");
-  if (c.streams > kMaxStreams) c.streams = kMaxStreams;
-  if (c.inflight > kMaxInflight) c.inflight = kMaxInflight;
+  if (c.max_items > (int)perm::kServeMaxItems) c.max_items = (int)perm::kServeMaxItems;
+  if (c.grid > 65536) c.grid = 65536;
+  // passes in flight never exceed the completion ring (one per resident request)
+  if (c.max_resident > (int)perm::kServeRing / 2) c.max_resident = (int)perm::kServeRing / 2;
+  if (c.arena_mb < 16.0) c.arena_mb = 16.0;
   if (int s = perm::select_device()) return s;
   auto* S = new perm_server();
   S->cfg = c;
   S->device = perm::current_device();
-  S->max_items = (size_t)c.max_items;
-  S->iter_cost = c.iteration_us * 1e-6 * kLaneInstrPerSec;
   if (int st = setup(S)) {
     const std::string msg = perm::last_error();
     release(S);
@@ -955,11 +1187,15 @@ int perm_server_stats_get(perm_server* S, perm_server_stats* out) {
   *out = S->stats;
   out->pending = (long long)S->pending.size();
   // scheduler-owned counters, read without its lock (statistics only)
+  out->launches = S->n_launches.load(std::memory_order_relaxed);
+  out->items = S->n_items.load(std::memory_order_relaxed);
   out->uploads = S->n_uploads.load(std::memory_order_relaxed);
+  out->stops = S->n_stops.load(std::memory_order_relaxed);
+  out->laps = (long long)(S->vhead_seen.load(std::memory_order_relaxed) / std::max<uint64_t>(1, S->arena_bytes));
   out->admit_ms = S->t_admit.load(std::memory_order_relaxed);
-  out->issue_ms = S->t_issue.load(std::memory_order_relaxed);
+  out->publish_ms = S->t_publish.load(std::memory_order_relaxed);
   out->retire_ms = S->t_retire.load(std::memory_order_relaxed);
-  out->gpu_wait_ms = S->t_wait.load(std::memory_order_relaxed);
+  out->stop_ms = S->t_stop.load(std::memory_order_relaxed);
   out->gpu_ms = S->t_gpu.load(std::memory_order_relaxed);
   out->dd_passes = S->n_dd.load(std::memory_order_relaxed);
   return perm::ok();
@@ -977,8 +1213,126 @@ void perm_server_destroy(perm_server* S) {
   S->cv_submit.notify_all();
   if (S->worker.joinable()) S->worker.join();  // drains admitted and pending work
   (void)hipSetDevice(S->device);
-  if (S->s0) (void)hipStreamSynchronize(S->s0);
+  for (KernelSlot& K : S->K)
+    if (K.s) (void)hipStreamSynchronize(K.s);
+  if (S->s_copy) (void)hipStreamSynchronize(S->s_copy);
   release(S);
+}
+
+int perm_server_drive(perm_server* S, int count, const int* dtypes, const void* const* mats,
+                      const int* const* cptrs, const int* const* rows,
+                      const void* const* cvals, const int* novs, const double* arrival_s,
+                      perm_result* out, double* latency_ms, double* wall_s) {
+  if (!S || count < 0 || (count > 0 && (!dtypes || !mats || !novs || !arrival_s || !out)))
+    return perm::fail(PERM_ERR_ARG, "null pointer");
+  std::vector<unsigned long long> ids(count);
+  std::atomic<int> submitted{0};
+  std::atomic<int> sub_status{PERM_OK};
+  std::string sub_error;
+  const Clock::time_point t0 = Clock::now();
+  std::thread submitter([&] {
+    for (int i = 0; i < count; ++i) {
+      const Clock::time_point due =
+          t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+      for (;;) {
+        const Clock::time_point now = Clock::now();
+        if (now >= due) break;
+        if (due - now > std::chrono::microseconds(200))
+          std::this_thread::sleep_for(due - now - std::chrono::microseconds(100));
+      }
+      const bool sp = cptrs && cptrs[i];
+      Request* r = nullptr;
+      int st = make_request(dtypes[i], mats[i], sp ? cptrs[i] : nullptr, sp ? rows[i] : nullptr,
+                            sp ? cvals[i] : nullptr, novs[i], r);
+      if (st == PERM_OK) st = enqueue(S, &r, 1, &ids[i]);
+      if (st != PERM_OK) {
+        sub_error = perm::last_error();
+        sub_status.store(st);
+        submitted.store(-1);
+        return;
+      }
+      submitted.store(i + 1, std::memory_order_release);
+    }
+  });
+  int first_error = PERM_OK;
+  for (int i = 0; i < count; ++i) {
+    int k;
+    while ((k = submitted.load(std::memory_order_acquire)) >= 0 && k <= i)
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    if (k < 0) break;
+    const int st = perm_server_wait(S, ids[i], out + i, latency_ms ? latency_ms + i : nullptr);
+    if (st != PERM_OK && first_error == PERM_OK) first_error = st;
+  }
+  submitter.join();
+  if (wall_s) *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+  if (sub_status.load() != PERM_OK) return perm::fail(sub_status.load(), sub_error);
+  return first_error == PERM_OK ? perm::ok() : first_error;
+}
+
+// Closed-loop throughput of one configuration on the sample (requests per
+// second), or a negative status.
+static double serve_trial(const perm_server_config& c, int count, const int* dtypes,
+                          const void* const* mats, const int* const* cptrs,
+                          const int* const* rows, const void* const* cvals, const int* novs,
+                          int& st) {
+  perm_server* S = nullptr;
+  if ((st = perm_server_create(&c, &S)) != PERM_OK) return -1.0;
+  std::vector<unsigned long long> ids(count);
+  std::vector<perm_result> res(count);
+  const Clock::time_point t0 = Clock::now();
+  st = perm_server_submit_many(S, count, dtypes, mats, cptrs, rows, cvals, novs, ids.data());
+  if (st == PERM_OK) st = perm_server_wait_many(S, ids.data(), count, res.data(), nullptr);
+  const double s = std::chrono::duration<double>(Clock::now() - t0).count();
+  perm_server_destroy(S);
+  return st == PERM_OK ? count / s : -1.0;
+}
+
+int perm_server_autotune(const perm_server_config* base, int count, const int* dtypes,
+                         const void* const* mats, const int* const* cptrs, const int* const* rows,
+                         const void* const* cvals, const int* novs, perm_server_config* best,
+                         double* best_req_per_s) {
+  if (!best || count <= 0 || !dtypes || !mats || !novs)
+    return perm::fail(PERM_ERR_ARG, "null pointer or empty sample");
+  perm_server_config c;
+  perm_server_default_config(&c);
+  if (base) c = *base;
+  int st = PERM_OK;
+  // warm the kernel and the host paths once
+  (void)serve_trial(c, std::min(count, 256), dtypes, mats, cptrs, rows, cvals, novs, st);
+  if (st != PERM_OK) return st;
+  double best_rate = serve_trial(c, count, dtypes, mats, cptrs, rows, cvals, novs, st);
+  if (st != PERM_OK) return st;
+  // coordinate descent: item size, then the kernel's grid, then item size again
+  const int item_opts[] = {12, 14, 16, 18, 20};
+  const int grid_opts[] = {256, 512, 1024, 2048};
+  for (int round = 0; round < 2; ++round) {
+    for (int il : item_opts) {
+      if (il == c.item_log2) continue;
+      perm_server_config t = c;
+      t.item_log2 = il;
+      const double rate = serve_trial(t, count, dtypes, mats, cptrs, rows, cvals, novs, st);
+      if (st != PERM_OK) return st;
+      if (rate > best_rate) {
+        best_rate = rate;
+        c = t;
+      }
+    }
+    if (round == 1) break;
+    for (int g : grid_opts) {
+      if (g == c.grid) continue;
+      perm_server_config t = c;
+      t.grid = g;
+      const double rate = serve_trial(t, count, dtypes, mats, cptrs, rows, cvals, novs, st);
+      if (st != PERM_OK) return st;
+      if (rate > best_rate) {
+        best_rate = rate;
+        c = t;
+      }
+    }
+  }
+  *best = c;
+  if (best_req_per_s) *best_req_per_s = best_rate;
+  return perm::ok();
 }
 
 }  // extern "C"

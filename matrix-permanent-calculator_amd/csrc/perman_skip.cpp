@@ -101,7 +101,7 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
   const uint64_t blocks = (lanes + 255) / 256;
   const size_t cs_b = align256(sp.cs.size() * sizeof(double));
   const size_t x_b = align256((size_t)n * sizeof(double));
-  const size_t lev_b = align256((size_t)n * sizeof(int));
+  const size_t lev_b = align256((size_t)(n + 3) / 4 * sizeof(uint32_t));
   const size_t part_b = align256(blocks * sizeof(perm::Partial));
   const size_t walk_b = align256(blocks * 4 * sizeof(uint32_t));
   const size_t res_b = align256(sizeof(perm::Partial));
@@ -126,7 +126,7 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
   perm::SkipLaunch L{};
   L.colpm = (const double*)d;
   L.x0 = (const double*)(d + cs_b);
-  L.row_level = (const int*)(d + cs_b + x_b);
+  L.row_level = (const uint32_t*)(d + cs_b + x_b);
   L.partials = (perm::Partial*)(d + cs_b + x_b + lev_b);
   L.walked = (uint32_t*)(d + cs_b + x_b + lev_b + part_b);
   L.result = (perm::Partial*)(d + cs_b + x_b + lev_b + part_b + walk_b);
@@ -144,7 +144,9 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
                           hipMemcpyHostToDevice, st));
   HIP_TRYK(hipMemcpyAsync((void*)L.x0, sp.x0.data(), n * sizeof(double), hipMemcpyHostToDevice,
                           st));
-  HIP_TRYK(hipMemcpyAsync((void*)L.row_level, sp.lev.data(), n * sizeof(int),
+  std::vector<uint32_t> packed((n + 3) / 4, 0u);  // levels <= walk_bits < 32: one byte each
+  for (int r = 0; r < n; ++r) packed[r / 4] |= (uint32_t)sp.lev[r] << (8 * (r % 4));
+  HIP_TRYK(hipMemcpyAsync((void*)L.row_level, packed.data(), packed.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, st));
   HIP_TRYK(hipEventRecord(e0, st));
   HIP_TRYK(perm::launch_skip(L, st));

@@ -340,25 +340,6 @@ __device__ __forceinline__ void walk_groups(double (&x)[N], double (&xl)[N], Col
   q = __builtin_fma(d, d, q);
 }
 
-// Wave-wide min and max of v (result in every lane, scalar).
-__device__ __forceinline__ void wave_min_max(int v, int& mn, int& mx) {
-  int lo = v, hi = v;
-#define PERM_MM_STEP(M)                                                      \
-  {                                                                          \
-    constexpr int pat = (M << 10) | 0x1f;                                    \
-    lo = min(lo, __builtin_amdgcn_ds_swizzle(lo, pat));                      \
-    hi = max(hi, __builtin_amdgcn_ds_swizzle(hi, pat));                      \
-  }
-  PERM_MM_STEP(1)
-  PERM_MM_STEP(2)
-  PERM_MM_STEP(4)
-  PERM_MM_STEP(8)
-  PERM_MM_STEP(16)
-#undef PERM_MM_STEP
-  mn = min(__builtin_amdgcn_readlane(lo, 0), __builtin_amdgcn_readlane(lo, 32));
-  mx = max(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(hi, 32));
-}
-
 // The walk of one lane: closed-form start from its high columns, then 2^w
 // subsets of its low columns.  Returns the lane's signed double-double sum
 // (sign of the high pattern applied) and its moments: sum of |terms|, sums

@@ -43,8 +43,8 @@ struct DenseLaunch {
 struct SkipLaunch {
   const double* colpm;        // device: column store in walk order
   const double* x0;           // device: start vector
-  const int* row_level;       // device: n ints, first walk column with an entry in row r
-                              // (walk_bits if none); rows ordered by it, ascending
+  const uint32_t* row_level;  // device: byte r of word r/4 = first walk column with an
+                              // entry in row r (walk_bits if none); rows ordered by it
   Partial* partials;          // device: `blocks` records
   uint32_t* walked;           // device: 4 * blocks counters (groups walked per wave)
   Partial* result;            // device: 1 record

@@ -13,12 +13,15 @@
 //    column lev[r] >= m keeps its x_r throughout the block; if x_r is zero at
 //    the block's first subset, all 2^m terms of the block are zero.
 //  * The host orders the rows by lev, ascending.  Every 2^S steps each lane
-//    finds its highest zero row j (so lev[j] is the largest level it could
-//    skip), the wave takes the minimum j over its lanes, and skips the block
-//    of level m = min(lev[j], alignment of t) if m >= S: two column
-//    additions replace 2^m steps (in Gray order the subset before the next
-//    block differs from the block's first subset only in column m-1; the
-//    next block's first step flips column m + ctz(t/2^m + 1)).
+//    finds the level of its highest zero row (the largest level it could
+//    skip; rows are in level order, so it is the largest level among its
+//    zero rows), the wave takes the minimum over its lanes (six ballots of a
+//    binary search: no LDS traffic, no memory load on the path), and skips
+//    the block of level m = min(that level, alignment of t) if m >= S: two
+//    column additions replace 2^m steps (in Gray order the subset before the
+//    next block differs from the block's first subset only in column m-1;
+//    the next block's first step flips column m + ctz(t/2^m + 1)).  The row
+//    levels are packed four to a 32-bit word and held in SGPRs.
 //  * Otherwise the wave walks 2^S steps with the plain kernel's unrolled
 //    groups (ryser_dense_impl.hpp) and checks again.
 //
@@ -31,7 +34,20 @@
 
 namespace perm {
 
-typedef const int __attribute__((address_space(4)))* LevPtr;
+typedef const uint32_t __attribute__((address_space(4)))* LevPtr;
+
+// Minimum over the wave of v in [0, 64) (every lane active): the largest t
+// with v >= t in every lane, by binary search on ballots.
+__device__ __forceinline__ uint32_t wave_min_u6(uint32_t v) {
+  const uint64_t all = __builtin_amdgcn_ballot_w64(true);
+  uint32_t lo = 0;
+#pragma unroll
+  for (int b = 5; b >= 0; --b) {
+    const uint32_t t = lo | (1u << b);
+    if (__builtin_amdgcn_ballot_w64(v >= t) == all) lo = t;
+  }
+  return lo;
+}
 
 template <int N>
 __device__ __forceinline__ void load_col_n(double (&cv)[N], ColPtr base, int blk) {
@@ -48,12 +64,18 @@ __device__ __forceinline__ int step_block(uint32_t t) {
 template <int N, int R, int MODE>
 __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0,
-                                          const int* __restrict__ row_level,
+                                          const uint32_t* __restrict__ row_level,
                                           uint64_t pattern, bool valid, int n_walk_bits,
                                           int n_high_bits, int check_bits, DD& s, double& a,
                                           double& q, uint32_t& walked_groups) {
   const ColPtr cols = (ColPtr)colpm;
-  const LevPtr lev = (LevPtr)row_level;
+  // row levels, byte j of word j/4 = level of row j (scalar loads, SGPRs)
+  uint32_t lev[(N + 3) / 4];
+  {
+    const LevPtr lp = (LevPtr)row_level;
+#pragma unroll
+    for (int i = 0; i < (N + 3) / 4; ++i) lev[i] = lp[i];
+  }
   const int zero_blk = 2 * (N - 1);
   double x[N], xl[N];
 #pragma unroll
@@ -88,15 +110,17 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
   while (t < steps) {
     // x = x(gray(t-1)), cv = the change of step t: apply it, x = x(gray(t)).
     add_column<N, MODE, N>(x, xl, cv);
-    // highest zero row of this lane (rows are in ascending lev order)
-    int jz = -1;
+    // 1 + level of this lane's highest zero row (rows are in ascending
+    // level order), 0 without a zero row; lanes past the walk do not limit
+    uint32_t ml = 0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) jz = x[j] == 0.0 ? j : jz;
-    if (!valid) jz = N - 1;
-    int jmin, jmax;
-    wave_min_max(jz, jmin, jmax);
+    for (int j = 0; j < N; ++j) {
+      const uint32_t lj = ((lev[j >> 2] >> (8 * (j & 3))) & 0xffu) + 1u;
+      ml = x[j] == 0.0 ? lj : ml;
+    }
+    if (!valid) ml = 63;
     const int align = t == 0 ? n_walk_bits : __builtin_ctz(t);
-    int m = jmin < 0 ? -1 : lev[jmin];
+    int m = (int)wave_min_u6(ml) - 1;
     m = m < align ? m : align;
     if (m >= check_bits) {
       // skip [t, t + 2^m): x <- x(gray(t + 2^m - 1)), cv <- step t + 2^m
@@ -146,7 +170,7 @@ template <int N, int R, int MODE>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
-                       const int* __restrict__ row_level, Partial* __restrict__ partials,
+                       const uint32_t* __restrict__ row_level, Partial* __restrict__ partials,
                        uint32_t* __restrict__ walked, uint64_t lanes, int n_walk_bits,
                        int n_high_bits, int check_bits, int lane_reverse) {
   __shared__ Partial scratch[256 / kWave];

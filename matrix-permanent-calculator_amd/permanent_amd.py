@@ -90,19 +90,18 @@ class BatchTuningC(ctypes.Structure):
 
 
 class ServerConfigC(ctypes.Structure):
-    _fields_ = [("item_log2", ctypes.c_int), ("streams", ctypes.c_int),
-                ("inflight", ctypes.c_int), ("max_items", ctypes.c_int),
-                ("iteration_us", ctypes.c_double), ("max_resident", ctypes.c_int),
-                ("max_queued_work", ctypes.c_double), ("prep_threads", ctypes.c_int),
-                ("max_wait_ms", ctypes.c_double)]
+    _fields_ = [("item_log2", ctypes.c_int), ("grid", ctypes.c_int),
+                ("max_items", ctypes.c_int), ("idle_us", ctypes.c_double),
+                ("max_resident", ctypes.c_int), ("max_queued_work", ctypes.c_double),
+                ("prep_threads", ctypes.c_int), ("arena_mb", ctypes.c_double)]
 
 
 class ServerStatsC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in
-                ("submitted", "admitted", "completed", "iterations", "items", "pending",
-                 "uploads")] + \
+                ("submitted", "admitted", "completed", "launches", "items", "pending",
+                 "uploads", "stops", "laps")] + \
                [(k, ctypes.c_double) for k in
-                ("admit_ms", "issue_ms", "retire_ms", "gpu_wait_ms", "gpu_ms")] + \
+                ("admit_ms", "publish_ms", "retire_ms", "stop_ms", "gpu_ms")] + \
                [("dd_passes", ctypes.c_longlong)]
 
 
@@ -192,6 +191,12 @@ def lib():
         L.perm_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
         L.perm_server_destroy.argtypes = [vp]
         L.perm_server_destroy.restype = None
+        _bind(L, "perm_server_drive", [vp, ctypes.c_int, ip, ctypes.POINTER(vp), ipp, ipp,
+                                       ctypes.POINTER(vp), ip, dp, ctypes.POINTER(PermResultC),
+                                       dp, dp])
+        _bind(L, "perm_server_autotune", [ctypes.POINTER(ServerConfigC), ctypes.c_int, ip,
+                                          ctypes.POINTER(vp), ipp, ipp, ctypes.POINTER(vp), ip,
+                                          ctypes.POINTER(ServerConfigC), dp])
         L.perm_connect.restype = None
         L.perm_chunked.argtypes = [ctypes.c_int, vp, ip, ip, vp, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.POINTER(PermResultC),
@@ -355,6 +360,20 @@ class Plan:
             pass
 
 
+class BatchInputs:
+    """The C arrays of perm_batch_add_dense_many for a list of dense
+    matrices, built once (the converted matrices are kept alive here)."""
+
+    def __init__(self, mats):
+        arrs = [_mat(m) for m in mats]
+        k = len(arrs)
+        self.mats = [a for a, _ in arrs]
+        self.count = k
+        self.dts = np.array([dt for _, dt in arrs], dtype=np.int32)
+        self.ns = np.array([a.shape[0] for a, _ in arrs], dtype=np.int32)
+        self.ptrs = (ctypes.c_void_p * k)(*[a.ctypes.data for a, _ in arrs])
+
+
 class Batch:
     """Many matrices (n <= 40) resident on the device; run() replays the
     captured graph of persistent variant kernels and returns one result per
@@ -390,6 +409,14 @@ class Batch:
         ptrs = (ctypes.c_void_p * k)(*[a.ctypes.data for a, _ in arrs])
         first = ctypes.c_int()
         _check(lib().perm_batch_add_dense_many(self._h, k, _ip(dts), ptrs, _ip(ns), threads,
+                                               ctypes.byref(first)))
+        return first.value
+
+    def add_prepared(self, inputs: BatchInputs, threads: int = 8) -> int:
+        """add_many for prebuilt BatchInputs."""
+        first = ctypes.c_int()
+        _check(lib().perm_batch_add_dense_many(self._h, inputs.count, _ip(inputs.dts),
+                                               inputs.ptrs, _ip(inputs.ns), threads,
                                                ctypes.byref(first)))
         return first.value
 
@@ -540,6 +567,40 @@ def solve(req: Request) -> PermResult:
     return PermResult.from_c(r)
 
 
+class RequestBatch:
+    """The C arrays of perm_server_submit_many for a list of Requests, built
+    once (the Requests keep the matrices alive)."""
+
+    def __init__(self, reqs):
+        k = len(reqs)
+        vp = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.reqs = list(reqs)
+        self.count = k
+        dts = (ctypes.c_int * k)(*[q.dt for q in reqs])
+        ns = (ctypes.c_int * k)(*[q.n for q in reqs])
+        mats = (vp * k)(*[q.a.ctypes.data for q in reqs])
+        cps = (ip * k)(*[_ip(q.cptrs) if q.sparse else None for q in reqs])
+        rws = (ip * k)(*[_ip(q.rows) if q.sparse else None for q in reqs])
+        cvs = (vp * k)(*[q.cvals.ctypes.data if q.sparse else None for q in reqs])
+        self.args = (k, dts, mats, cps, rws, cvs, ns)
+
+
+def autotune_server(reqs, **base) -> tuple:
+    """perm_server_autotune on a sample of Requests: (best config as a dict,
+    its closed-loop requests per second)."""
+    c = ServerConfigC()
+    lib().perm_server_default_config(ctypes.byref(c))
+    for k, v in base.items():
+        setattr(c, k, v)
+    b = RequestBatch(reqs)
+    best = ServerConfigC()
+    rate = ctypes.c_double()
+    _check(lib().perm_server_autotune(ctypes.byref(c), *b.args, ctypes.byref(best),
+                                      ctypes.byref(rate)))
+    return {k: getattr(best, k) for k, _ in ServerConfigC._fields_}, rate.value
+
+
 class Server:
     """Continuous-batching permanent server on one GPU (perm_server_*).
     submit() returns a ticket at once; wait() blocks for its result and
@@ -571,18 +632,37 @@ class Server:
     def submit_many(self, reqs) -> list:
         """Submit prepared Requests in one call (perm_server_submit_many);
         returns their tickets."""
-        k = len(reqs)
-        vp = ctypes.c_void_p
-        dts = (ctypes.c_int * k)(*[q.dt for q in reqs])
-        ns = (ctypes.c_int * k)(*[q.n for q in reqs])
-        mats = (vp * k)(*[q.a.ctypes.data for q in reqs])
-        ip = ctypes.POINTER(ctypes.c_int)
-        cps = (ip * k)(*[_ip(q.cptrs) if q.sparse else None for q in reqs])
-        rws = (ip * k)(*[_ip(q.rows) if q.sparse else None for q in reqs])
-        cvs = (vp * k)(*[q.cvals.ctypes.data if q.sparse else None for q in reqs])
-        ids = (ctypes.c_ulonglong * k)()
-        _check(lib().perm_server_submit_many(self._h, k, dts, mats, cps, rws, cvs, ns, ids))
-        return list(ids)
+        return list(self.submit_batch(RequestBatch(reqs)))
+
+    def submit_batch(self, batch: "RequestBatch"):
+        """Submit a RequestBatch (its C arrays were built beforehand) in one
+        call; returns the ctypes array of tickets."""
+        ids = (ctypes.c_ulonglong * batch.count)()
+        _check(lib().perm_server_submit_many(self._h, *batch.args, ids))
+        return ids
+
+    def drive(self, batch: "RequestBatch", arrival_s):
+        """Open loop (perm_server_drive): request i submitted arrival_s[i]
+        seconds after the start; returns the ctypes arrays of results and
+        latencies and the wall time in seconds."""
+        k = batch.count
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        res = (PermResultC * k)()
+        lat = (ctypes.c_double * k)()
+        wall = ctypes.c_double()
+        _check(lib().perm_server_drive(self._h, *batch.args,
+                                       arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       res, lat, ctypes.byref(wall)))
+        return res, lat, wall.value
+
+    def wait_many_raw(self, ids):
+        """Wait for a ctypes array of tickets; returns the ctypes arrays of
+        results and latencies (no Python objects per result)."""
+        k = len(ids)
+        res = (PermResultC * k)()
+        lat = (ctypes.c_double * k)()
+        _check(lib().perm_server_wait_many(self._h, ids, k, res, lat))
+        return res, lat
 
     def submit_request(self, req: Request) -> int:
         tid = ctypes.c_ulonglong()

@@ -96,6 +96,32 @@ def test_cli_every_gpu_id(tmp_path):
         assert v == ex, (args, r.stdout)
 
 
+def test_cli_orders_on_the_gpu(tmp_path):
+    """`-r1` / `-r2` (the reference's sortOrder / skipOrder preprocessing,
+    util.h:553-684, main.cu:502-507) before the GPU paths -p4, -s -p4 and
+    -s -p7 (the skipper): the permanent is unchanged, exact for an integer
+    matrix with zeros, ties in the column counts and rows of several
+    degrees; FP64 within 1e-12 for a real matrix."""
+    rng = np.random.default_rng(2024)
+    a = ((rng.random((18, 18)) < 0.3) * rng.integers(1, 4, (18, 18))).astype(np.int32)
+    a[np.arange(18), (np.arange(18) * 5) % 18] = 1  # every row and column nonzero
+    f = tmp_path / "m18"
+    write_matrix(f, a, "int")
+    ex = float(O.exact_permanent(a))
+    b = P.synthetic_matrix(16, 0.4, seed=161, full_support=True)
+    g = tmp_path / "m16"
+    write_matrix(g, b)
+    exb = float(O.exact_permanent(b))
+    for r_opt in ("1", "2"):
+        for args in (("-p", "4"), ("-s", "-p", "4"), ("-s", "-p", "7")):
+            r = run_cli("-f", str(f), "-g", "-r", r_opt, *args)
+            assert r.returncode == 0, (r_opt, args, r.stdout, r.stderr)
+            assert cli_result(r.stdout)[1] == ex, (r_opt, args, r.stdout)
+            r = run_cli("-f", str(g), "-g", "-r", r_opt, *args)
+            assert r.returncode == 0, (r_opt, args, r.stdout, r.stderr)
+            assert cli_result(r.stdout)[1] == pytest.approx(exb, rel=1e-12), (r_opt, args)
+
+
 def test_compiled_caller_invokes_every_gpu_template(tmp_path):
     """A main.cu-style C++ caller (RunAlgo's calls, main.cu:31-148) compiled
     with g++ against include/perman.hpp calls every GPU template -- dense

@@ -1,10 +1,14 @@
-"""Served form: continuous batching of requests of mixed n (perm_server_*).
+"""Served form: continuous batching of requests of mixed n on one persistent
+kernel (perm_server_*).
 
 * results against the exact oracle (n <= 20) or the single-matrix path
-  (within the stated errors), integer inputs exact;
+  (within the stated errors), integer inputs exact, n < 8 embedded;
 * a request gives the same bits alone and in a concurrent mix, whatever
-  iterations its items were packed into;
-* poll/wait/stats semantics, rejection of bad requests, shutdown drains."""
+  waves and launches ran its items;
+* the persistent kernel ends when idle and is launched again; a small arena
+  forces the ring to wrap (launches stopped for the reuse of its space);
+* poll/wait/stats semantics, rejection of bad requests, shutdown drains;
+* the autotuner returns a configuration from its candidate set."""
 import ctypes
 import threading
 
@@ -51,11 +55,11 @@ def test_server_results_vs_oracle():
 
 def test_server_same_bits_alone_and_in_mix():
     target = P.synthetic_matrix(26, 0.5, seed=4242)
-    with P.Server(item_log2=12, iteration_us=300.0) as srv:
+    with P.Server(item_log2=12, idle_us=300.0) as srv:
         alone = srv.wait(srv.submit(target))[0]
     others = P.served_workload(120, seed=9, n_lo=12, n_hi=28)
     got = {}
-    with P.Server(item_log2=12, iteration_us=300.0, streams=3) as srv:
+    with P.Server(item_log2=12, idle_us=300.0, grid=256) as srv:
         def client(part):
             for m, sp in part:
                 srv.submit(m, sparse_path=sp)
@@ -69,7 +73,7 @@ def test_server_same_bits_alone_and_in_mix():
         got[1] = srv.wait(t1)[0]
         got[2] = srv.wait(t2)[0]
         stats = srv.stats()
-    assert stats["iterations"] > 1
+    assert stats["launches"] >= 1 and stats["items"] > 120
     for r in got.values():
         assert (r.value, r.sum_hi, r.sum_lo, r.abs_sum) == (
             alone.value, alone.sum_hi, alone.sum_lo, alone.abs_sum)
@@ -232,3 +236,39 @@ def test_server_sampled_error_measurement():
         assert r.certified, r
         assert abs(r.value - one.value) <= 1e-10 * abs(one.value), (m.shape, r, one)
     print("dd passes:", st["dd_passes"], [(r.mode, r.err_est / abs(r.value)) for r, _ in res])
+
+
+def test_server_relaunch_after_idle_and_arena_wrap():
+    """The persistent kernel leaves after idle_us without work and the next
+    submission launches it again; with a 16 MiB arena the block ring wraps
+    (the live launch is stopped before its space is reused) and every result
+    still matches the single-matrix path."""
+    import time
+
+    mats = [P.synthetic_matrix(22 + i % 7, 0.5, seed=5100 + i) for i in range(12)]
+    with P.Server(idle_us=200.0, arena_mb=16.0, item_log2=12) as srv:
+        first = srv.wait(srv.submit(mats[0]))[0]
+        time.sleep(0.05)  # the launch has ended by now
+        again = srv.wait(srv.submit(mats[0]))[0]
+        assert (first.sum_hi, first.sum_lo) == (again.sum_hi, again.sum_lo)
+        assert srv.stats()["launches"] >= 2
+        # ~30 MiB of blocks submitted at once: while the kernel works, space
+        # freed during its launch is reused only after stopping it
+        res = srv.wait_many([srv.submit(m) for _ in range(12) for m in mats])
+        st = srv.stats()
+    print(st)
+    assert st["laps"] >= 1
+    assert st["stops"] >= 1
+    for k, (r, _) in enumerate(res):
+        one = P.dense(mats[k % len(mats)])
+        assert abs(r.value - one.value) <= r.stated + one.stated
+        assert abs(r.value - one.value) <= 1e-10 * abs(one.value)
+
+
+def test_server_autotune_returns_a_candidate():
+    sample = [P.Request(m, sp) for m, sp in P.served_workload(200, seed=17, n_lo=12, n_hi=24)]
+    cfg, rate = P.autotune_server(sample)
+    print(cfg, rate)
+    assert rate > 0
+    assert cfg["item_log2"] in (12, 14, 16, 18, 20)
+    assert cfg["grid"] in (256, 512, 1024, 2048)
