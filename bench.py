@@ -23,6 +23,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "matrix-permanent-calculator_amd"))
 # OpenMP threads of the CPU legs stay on their cores (read when libgomp loads)
 os.environ.setdefault("OMP_PROC_BIND", "close")
+# The process's CPU set before libgomp loads: under OMP_PROC_BIND, loading
+# it pins the loading thread to one core at once, and every thread started
+# after that (the server's scheduler and prep threads, the batch's host
+# threads) would inherit the one-core mask.
+FULL_CPU_SET = os.sched_getaffinity(0)
 
 # HBM bytes per launch of the n=40 kernel on this workload: rocprofv3 FETCH_SIZE
 # + WRITE_SIZE, each scaled by its reading for a copy kernel of known size
@@ -36,15 +41,22 @@ SEED = 2
 
 @contextlib.contextmanager
 def affinity_kept():
-    """OpenMP under OMP_PROC_BIND pins the calling thread to one core at its
-    first parallel region, and threads started later (the server's scheduler
-    thread) inherit that mask; the CPU legs run inside this so the GPU legs
-    that follow them keep the process's full CPU set."""
-    mask = os.sched_getaffinity(0)
+    """OpenMP under OMP_PROC_BIND pins the calling thread to one core (at
+    load, and at its first parallel region), and threads started later (the
+    server's scheduler and prep threads) inherit that mask; the CPU legs run
+    inside this so the GPU legs that follow them get the process's full CPU
+    set back."""
     try:
         yield
     finally:
-        os.sched_setaffinity(0, mask)
+        os.sched_setaffinity(0, FULL_CPU_SET)
+
+
+def load_library(P):
+    """Load the library (and libgomp with it) and give this thread its full
+    CPU set back."""
+    P.lib()
+    os.sched_setaffinity(0, FULL_CPU_SET)
 
 
 def cpu_share() -> int:
@@ -54,7 +66,7 @@ def cpu_share() -> int:
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
-    return len(os.sched_getaffinity(0))
+    return len(FULL_CPU_SET)
 
 
 def cpu_model() -> str:
@@ -84,7 +96,7 @@ def cpu_baseline(mat, log2_sample: int, threads: int):
         "kind": "port",
         "cpu": cpu_model(),
         "nproc": os.cpu_count(),
-        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "affinity_cpus": len(FULL_CPU_SET),
         "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "sample": (f"parallel_perman64 float-X restatement (algo.h:662-746), OpenMP "
                    f"{threads} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), "
@@ -391,6 +403,7 @@ def main():
         sys.exit(2)
 
     import permanent_amd as P
+    load_library(P)
 
     progress("dense n=40 steps")
     mat = P.synthetic_matrix(N, DENSITY, SEED)

@@ -372,13 +372,15 @@ double perm_matlab_calculate_return(const int* mat, int algorithm, int nt, int x
  * sparse) are submitted from any thread and return a ticket at once: the
  * submission only copies the matrix; a pool of host threads prepares it
  * (start vector, column store, walk shape, padded to a served order that is
- * a multiple of 4) and a scheduler thread places it in a device arena and
- * publishes it to ONE persistent kernel, launched as a captured hipGraph,
- * that polls a ring of pass descriptors in pinned host memory.  The
- * kernel's waves claim 64-lane items in publication order; the wave that
- * finishes a request's last item sums its item records on the device in
- * item order and writes the sum to host memory: a result is bitwise the
- * same alone or in any mix.  Argument errors found while preparing (a
+ * a multiple of 4) and a scheduler thread writes it into a pinned mirror of
+ * a device arena and publishes it to ONE persistent kernel, launched as a
+ * captured hipGraph, that polls a ring of pass descriptors in pinned host
+ * memory and copies each new pass's range of the mirror into the arena
+ * itself (no host-issued copies).  The kernel's waves claim 64-lane items
+ * in publication order; the wave that finishes a request's last item sums
+ * its item records on the device in item order and appends the sum to a
+ * completion queue in host memory: a result is bitwise the same alone or in
+ * any mix.  Argument errors found while preparing (a
  * malformed CCS) are reported by the wait/poll of that request.
  * Admission is bounded by the resident requests, the arena and the queued
  * work.  perm_server_wait blocks until the request is done and frees its
@@ -403,15 +405,17 @@ typedef struct perm_server_stats {
   long long launches;     /* launches (graph replays) of the persistent kernel */
   long long items;        /* items published */
   long long pending;      /* prepared requests not yet admitted */
-  long long uploads;      /* host-to-device copies of arena blocks */
+  long long uploads;      /* publications (tail advances) of staged passes */
   long long stops;        /* launches stopped so that arena space could be reused */
   long long laps;         /* times the arena's block ring wrapped to its start */
-  double admit_ms;        /* scheduler time admitting, staging and copying requests */
+  double admit_ms;        /* scheduler time admitting and staging requests */
   double publish_ms;      /* scheduler time publishing passes and launching */
   double retire_ms;       /* scheduler time turning completed passes into results */
   double stop_ms;         /* scheduler time waiting for stopped launches */
   double gpu_ms;          /* summed duration of the kernel's launches */
   long long dd_passes;    /* passes in a double-double mode: escalations and sampled error measurements */
+  double prep_ms;         /* preparation time summed over the prep threads */
+  double submit_ms;       /* time in the submission calls (copying matrices, queueing) */
 } perm_server_stats;
 void perm_server_default_config(perm_server_config* cfg);
 int perm_server_create(const perm_server_config* cfg, perm_server** server);

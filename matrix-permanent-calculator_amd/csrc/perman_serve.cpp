@@ -12,23 +12,26 @@
 //    per pass, several 64-lane groups per item above that).
 //  * Admission (scheduler thread): a prepared request gets one block of the
 //    device arena for its whole life -- a header slot for each of its (at
-//    most three) passes, room for sampled item ids, its column store and
-//    start vector, one record per item (reused by every pass) -- while
+//    most three) passes, its column store and start vector, room for
+//    sampled item ids, one record per item (reused by every pass) -- while
 //    resident requests stay under max_resident and queued work under
 //    max_queued_work.  Blocks are taken from a ring in order and freed in
 //    order; since a later pass never needs new space, the oldest request
 //    always finishes and frees the ring's front.  Blocks are written into a
-//    pinned mirror of the arena and go up by one asynchronous copy per run of
-//    new blocks (a later pass uploads its header, and ids, alone).
-//  * Publication: once a round's copy has completed, each pass's header
-//    address goes into the ring of the pinned control block and `tail`
-//    advances.  The kernel's waves claim items in publication order, so no
-//    request waits behind later ones.  If no launch is live, the scheduler
-//    replays the graph of the persistent kernel.
-//  * Completion: the wave that finishes a pass's last item writes the
+//    pinned mirror of the arena at the same offsets.
+//  * Publication: each pass's header address and the mirror range it needs
+//    copied (its header; on the first pass also the matrix; on a
+//    verification pass also the sampled ids) go into the ring of the pinned
+//    control block and `tail` advances.  The kernel's polling wave copies
+//    the ranges into the arena and publishes the passes to the other waves:
+//    the host issues no copy and waits on no event.  The waves claim items
+//    in publication order, so no request waits behind later ones.  If no
+//    launch of the pass's kind is live, the scheduler replays its graph.
+//  * Completion: the wave that finishes a pass's last item appends the
 //    pass's sum (its records summed in item order, so a result depends only
-//    on its matrix: bitwise the same alone or in any mix) into a completion
-//    record in pinned memory.  The scheduler turns it into a result with the
+//    on its matrix: bitwise the same alone or in any mix) to a completion
+//    queue in pinned memory, which the scheduler reads in order (no scan of
+//    the passes in flight).  The scheduler turns it into a result with the
 //    stated error; a result that misses the accuracy target gets another
 //    pass over the same matrix (double-double products for an integer
 //    matrix; for other matrices first a sampled measurement of the FP64
@@ -36,9 +39,11 @@
 //    needed, the double-double X pass), else the request is published to its
 //    waiters.
 //  * Cache rule of the persistent kernel: a launch reads headers and
-//    matrices through caches that do not see device-memory writes made by
-//    copies after the launch began, and its waves may scan any pass from the
-//    oldest one in flight when it began.  So arena space is reused only if it
+//    matrices with plain loads, through caches that may still hold what an
+//    earlier tenant of the same arena space left there if this launch read
+//    it, and its waves may scan any pass from the oldest one in flight when
+//    it began.  Each range is copied once per use (a header slot, the
+//    matrix, the ids are never rewritten while their request lives).  So arena space is reused only if it
 //    was freed before the live launch began; otherwise the live launch is
 //    stopped first (about once per lap of the arena ring) and the next one
 //    starts with clean caches.
@@ -80,14 +85,27 @@ using Clock = std::chrono::steady_clock;
 // Pass kinds: 0 FP64, 1 double-double products, 2 double-double X;
 // kVerifyPass: sampled items of the FP64 pass walked in mode 2.
 constexpr int kVerifyPass = 3;
-// Request block: kPassSlots header slots, the sampled ids, the column store
-// and start vector, the item records.
+// Request block: header slot 0, the column store and start vector, header
+// slot 1, the sampled ids, header slot 2, the item records -- so each pass's
+// copy is one contiguous range: slot 0 with the matrix, slot 1 with the
+// ids (the verification pass is always the second), slot 2 alone.
 constexpr size_t kHeaderBytes = 256;
 constexpr int kPassSlots = 3;  // FP64; then double-double products, or verification then X
 constexpr size_t kIdsBytes = (perm::kVerifyUnits * sizeof(uint32_t) + 255) / 256 * 256;
-constexpr size_t kDataOff = kPassSlots * kHeaderBytes + kIdsBytes;
+static_assert(sizeof(perm::ServeHeader) <= kHeaderBytes, "header slot");
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Offsets in a request's block, from its column store + start vector size.
+struct BlockLayout {
+  size_t data_bytes;
+  size_t slot(int i) const {
+    return i == 0 ? 0 : i == 1 ? kHeaderBytes + data_bytes : 2 * kHeaderBytes + data_bytes + kIdsBytes;
+  }
+  size_t data() const { return kHeaderBytes; }
+  size_t ids() const { return slot(1) + kHeaderBytes; }
+  size_t records() const { return slot(2) + kHeaderBytes; }
+};
 
 // A submitted matrix as the caller gave it (copied), prepared by the pool.
 struct RawMatrix {
@@ -105,11 +123,9 @@ struct Pass {
   int kind = 0;          // 0, 1, 2, kVerifyPass
   uint32_t items = 0;
   uint64_t seq = 0;      // publication sequence
-  uint64_t tag = 0;      // the completion record carries tag + 1
-  uint32_t done_slot = 0;
-  bool first = false;    // the request's first pass (uploads its whole block)
+  bool first = false;    // the request's first pass (its copy carries the matrix)
   int slot = 0;          // header slot in the request's block
-  uint64_t up_v = 0;     // virtual arena position and size of the upload
+  size_t up_off = 0;     // arena offset and size of the range the kernel copies
   size_t up_bytes = 0;
 };
 
@@ -126,11 +142,12 @@ struct Request {
   bool integral = false;
   double product_bits = 1e9;
   double cost = 0.0;     // FP64 lane-instructions of the FP64 pass (admission)
-  std::vector<double> cs, x0;  // padded column store and start vector (until uploaded)
+  std::vector<double> cs, x0;  // padded column store and start vector (until staged)
   uint64_t vblock = 0;   // virtual arena position of the request's block
   size_t block_bytes = 0;
   size_t block = 0;      // block index in allocation order
   size_t cs_doubles = 0;
+  BlockLayout lay{0};
   int passes = 0;        // passes staged so far
   bool resident = false;
   bool sampled = false;
@@ -154,6 +171,7 @@ struct KernelSlot {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool launched = false;  // a launch was made and not yet accounted
+  uint64_t epoch = 0;     // launch number (the kernel reports the one that ends)
   uint64_t launch_vfront = 0;
 };
 
@@ -164,12 +182,6 @@ struct Block {
   bool live = true;
 };
 
-// One upload round: passes staged together, and the event after their copies.
-struct Round {
-  std::vector<Pass*> passes;
-  hipEvent_t ev = nullptr;
-};
-
 }  // namespace
 
 struct perm_server {
@@ -177,39 +189,48 @@ struct perm_server {
   int device = 0;
   std::thread worker;
   std::mutex mu;
-  std::condition_variable cv_submit, cv_done, cv_prep;
-  std::deque<Request*> to_prep;                  // submitted, not prepared (mu)
-  std::deque<Request*> pending;                  // prepared, not admitted (mu)
+  std::condition_variable cv_done;               // (mu)
+  // the queues between submitters, prep threads and the scheduler have a
+  // lock of their own, so that the scheduler and the prep threads do not
+  // contend with waiters on mu
+  std::mutex mu_q;
+  std::condition_variable cv_submit, cv_prep;    // (mu_q)
+  std::deque<Request*> to_prep;                  // submitted, not prepared (mu_q)
+  std::deque<Request*> pending;                  // prepared, not admitted (mu_q)
+  std::atomic<int> pending_n{0};                 // pending.size(), read without mu_q
+  std::atomic<bool> stopping{false};             // set under mu and mu_q
   std::vector<std::thread> preppers;
-  int prepping = 0;                              // requests taken by a prep thread (mu)
+  size_t preppers_n = 0;
+  int prepping = 0;                              // requests taken by a prep thread (mu_q)
   std::unordered_map<uint64_t, Request*> table;  // every uncollected request (mu)
   uint64_t next_id = 1;
-  bool stop = false;
   int fatal = PERM_OK;  // scheduler error (mu): every later call fails with it
   std::string fatal_msg;
   perm_server_stats stats{};  // (mu)
   // ---- scheduler-owned ----
   char* d_arena = nullptr;
-  char* h_arena = nullptr;     // pinned mirror (upload staging at the same offsets)
+  char* h_arena = nullptr;     // pinned mirror (the kernel copies from it, same offsets)
+  char* h_arena_dev = nullptr; // the mirror's device address
   uint64_t arena_bytes = 0;
   std::deque<Block> blocks;    // blocks in allocation order, from index block_base
   size_t block_base = 0;
   uint64_t vhead = 0;          // virtual position of the next block
   ServeControl* ctl = nullptr;  // pinned, coherent
-  ServeDone* done = nullptr;    // pinned, coherent, kServeRing records
+  ServeDone* done = nullptr;    // pinned, coherent: the completion queue (kServeRing)
   ServeState* d_state = nullptr;
-  uint64_t next_seq = 0, next_tag = 1, next_done = 0;
+  uint64_t next_seq = 0;        // publication sequence of the next pass
+  uint64_t lo_seq = 0;          // every pass before it has completed
+  uint64_t consumed = 0;        // completion records read
   int resident = 0;
   double queued_cost = 0.0;
-  std::vector<Request*> active;   // admitted, not finished
-  std::vector<Pass*> to_stage;    // written to the mirror, not copied yet
-  std::deque<Round> rounds;       // copies in flight, in order
-  std::vector<hipEvent_t> free_events;
-  std::vector<Pass*> inflight;    // published, not completed
-  hipStream_t s_copy = nullptr;
+  std::deque<Pass*> to_publish;  // written to the mirror, not published yet
+  std::vector<Pass*> by_seq;     // passes in flight at seq mod kServeRing
+  int n_inflight[2] = {0, 0};    // passes in flight per kernel kind
   KernelSlot K[2];  // [0] the main kernel, [1] the double-double X kernel
   std::atomic<double> t_admit{0.0}, t_publish{0.0}, t_retire{0.0}, t_stop{0.0}, t_gpu{0.0};
-  std::atomic<long long> n_uploads{0}, n_dd{0}, n_launches{0}, n_items{0}, n_stops{0};
+  std::atomic<double> t_prep{0.0}, t_submit{0.0};  // (added to under mu)
+  std::atomic<long long> n_publish{0}, n_dd{0}, n_launches{0}, n_items{0}, n_stops{0};
+  std::atomic<long long> n_admitted{0};
   std::atomic<uint64_t> vhead_seen{0};  // vhead, for the statistics
 };
 
@@ -240,7 +261,9 @@ bool trace_on() {
 
 void add_ms(std::atomic<double>& acc, Clock::time_point t0) {
   const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-  acc.store(acc.load(std::memory_order_relaxed) + ms, std::memory_order_relaxed);
+  double cur = acc.load(std::memory_order_relaxed);
+  while (!acc.compare_exchange_weak(cur, cur + ms, std::memory_order_relaxed)) {
+  }
 }
 void bump(std::atomic<long long>& c, long long v = 1) {
   c.store(c.load(std::memory_order_relaxed) + v, std::memory_order_relaxed);
@@ -264,11 +287,18 @@ double pass_cost(const Request* r, int kind, uint32_t items) {
 
 bool is_dd(int kind) { return kind == 2 || kind == kVerifyPass; }
 
+// The last launch of kind k may still run (exact: its end event).
 bool launch_live(perm_server* S, int k) {
   return S->K[k].launched && hipEventQuery(S->K[k].e1) == hipErrorNotReady;
 }
 
-// Account an ended launch's GPU time.
+// The last launch of kind k has decided to end (its waves leave between
+// items): a new launch may be queued behind it.  A read of host memory.
+bool launch_ending(perm_server* S, int k) {
+  return !S->K[k].launched || __atomic_load_n(&S->ctl->ended[k], __ATOMIC_ACQUIRE) == S->K[k].epoch;
+}
+
+// Account an ended launch's GPU time (dropped if it has not ended yet).
 void account_launch(perm_server* S, int k) {
   KernelSlot& K = S->K[k];
   if (!K.launched) return;
@@ -288,20 +318,33 @@ int launch(perm_server* S, int k) {
   KernelSlot& K = S->K[k];
   account_launch(S, k);
   uint64_t head = S->next_seq;
-  for (const Pass* p : S->inflight)
-    if (is_dd(p->kind) == (k == 1)) head = std::min(head, p->seq);
-  __atomic_store_n(k == 1 ? &S->ctl->head_dd : &S->ctl->head, head, __ATOMIC_RELEASE);
-  K.launch_vfront = vfront(S);
+  for (uint64_t q = S->lo_seq; q < S->next_seq; ++q) {
+    const Pass* p = S->by_seq[q & (perm::kServeRing - 1)];
+    if (p && is_dd(p->kind) == (k == 1)) {
+      head = q;
+      break;
+    }
+  }
+  __atomic_store_n(&S->ctl->head[k], head, __ATOMIC_RELEASE);
+  K.epoch += 1;
+  __atomic_store_n(&S->ctl->epoch[k], K.epoch, __ATOMIC_RELEASE);
+  // the arena positions this launch may have read begin at the live front;
+  // a previous launch still finishing its items keeps its older front (the
+  // new launch starts after it on the stream, so its events cover both)
+  if (!launch_live(S, k)) K.launch_vfront = vfront(S);
   HIP_TRYS(hipEventRecord(K.e0, K.s));
-  // the device copy of the stop flag starts clear (ordered before the
-  // kernel on its stream; outside the captured graph).  No launch is stopping
-  // here: stop_kernels waits for the launches it stops.
-  HIP_TRYS(hipMemsetAsync(&S->d_state->stop, 0, sizeof(uint64_t), K.s));
+  // the device copy of the stop flag starts clear (ordered after the
+  // previous launch of the kind, which may still be finishing its items,
+  // and before this one; outside the captured graph)
+  HIP_TRYS(hipMemsetAsync(&S->d_state->stop[k], 0, sizeof(uint64_t), K.s));
   HIP_TRYS(hipGraphLaunch(K.exec, K.s));
   HIP_TRYS(hipEventRecord(K.e1, K.s));
   K.launched = true;
   bump(S->n_launches);
-  SERVE_TRACE("launch kernel %d head %llu", k, (unsigned long long)head);
+  SERVE_TRACE("launch kernel %d head %llu (previous ended: %llu, why %llu, activity %llu)", k,
+              (unsigned long long)head, (unsigned long long)S->ctl->ended[k],
+              (unsigned long long)(S->ctl->why[k] & 0xff),
+              (unsigned long long)(S->ctl->why[k] >> 8));
   return PERM_OK;
 }
 
@@ -329,14 +372,13 @@ int stop_kernels(perm_server* S) {
 // every register file it is given): a live main launch is stopped first and
 // launched again after it.
 int ensure_kernels(perm_server* S) {
-  bool main_work = false, dd_work = false;
-  for (const Pass* p : S->inflight) (is_dd(p->kind) ? dd_work : main_work) = true;
-  if (dd_work && !launch_live(S, 1)) {
+  const bool main_work = S->n_inflight[0] > 0, dd_work = S->n_inflight[1] > 0;
+  if (dd_work && launch_ending(S, 1)) {
     if (launch_live(S, 0))
       if (int s = stop_kernels(S)) return s;
     if (int s = launch(S, 1)) return s;
   }
-  if (main_work && !launch_live(S, 0))
+  if (main_work && launch_ending(S, 0))
     if (int s = launch(S, 0)) return s;
   return PERM_OK;
 }
@@ -349,10 +391,13 @@ int ensure_kernels(perm_server* S) {
 int arena_alloc(perm_server* S, uint64_t size, uint64_t& vpos) {
   vpos = UINT64_MAX;
   const uint64_t A = S->arena_bytes;
-  if (size > A) return PERM_OK;
+  if (size > A / 2) return PERM_OK;
   uint64_t v = S->vhead;
   if (v % A + size > A) v = (v / A + 1) * A;  // blocks never straddle the end
-  if (v + size - vfront(S) > A) return PERM_OK;  // would overwrite live data
+  // live blocks span at most half the arena: a launch stopped so that space
+  // it may have read can be reused (the cache rule below) is then followed by
+  // half a lap of allocations before the next stop, not one per allocation
+  if (!S->blocks.empty() && v + size - vfront(S) > A / 2) return PERM_OK;
   // the block reuses space last used at positions [v - A, v + size - A): a
   // live launch that began before that space was freed may have read it
   if (v + size > A)
@@ -391,24 +436,25 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
   const perm::NestShape vshape =
       vmode == 2 ? perm::NestShape{r->n_pad, r->n_pad, r->n_pad} : r->shape;
   const int variant = perm::serve_variant(r->n_pad, vshape, vmode);
-  if (variant < 0 || r->passes >= kPassSlots || items > r->walk_items ||
-      (kind == kVerifyPass && items > perm::kVerifyUnits)) {  // not expected
+  const int slot = r->passes;
+  if (variant < 0 || slot >= kPassSlots || items > r->walk_items ||
+      (kind == kVerifyPass && (items > perm::kVerifyUnits || slot != 1))) {  // not expected
     st = perm::fail(PERM_ERR_ARG, "no served kernel variant or pass slot for this request");
     return nullptr;
   }
   if (first) {
-    const size_t data_bytes = align256((r->cs.size() + r->x0.size()) * sizeof(double));
-    const size_t bytes = kDataOff + data_bytes + align256((size_t)r->walk_items * sizeof(Partial));
+    r->cs_doubles = r->cs.size();
+    r->lay.data_bytes = align256((r->cs.size() + r->x0.size()) * sizeof(double));
+    const size_t bytes = r->lay.records() + align256((size_t)r->walk_items * sizeof(Partial));
     uint64_t v = 0;
     if ((st = arena_alloc(S, bytes, v)) != PERM_OK || v == UINT64_MAX) return nullptr;
     r->vblock = v;
     r->block_bytes = bytes;
     r->block = S->block_base + S->blocks.size() - 1;
-    r->cs_doubles = r->cs.size();
     r->resident = true;
     char* h = S->h_arena + (size_t)(v % S->arena_bytes);
-    std::memcpy(h + kDataOff, r->cs.data(), r->cs.size() * sizeof(double));
-    std::memcpy(h + kDataOff + r->cs.size() * sizeof(double), r->x0.data(),
+    std::memcpy(h + r->lay.data(), r->cs.data(), r->cs.size() * sizeof(double));
+    std::memcpy(h + r->lay.data() + r->cs.size() * sizeof(double), r->x0.data(),
                 r->x0.size() * sizeof(double));
     std::vector<double>().swap(r->cs);
     std::vector<double>().swap(r->x0);
@@ -416,102 +462,68 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
   const size_t off = (size_t)(r->vblock % S->arena_bytes);
   char* h = S->h_arena + off;
   char* d = S->d_arena + off;
-  const size_t data_bytes = align256(r->cs_doubles * sizeof(double) + 2 * r->n_pad * sizeof(double));
-  if (kind == kVerifyPass)
-    std::memcpy(h + kPassSlots * kHeaderBytes, r->vids.data(), items * sizeof(uint32_t));
-  const int slot = r->passes++;
+  const BlockLayout& L = r->lay;
+  size_t ids_bytes = 0;
+  if (kind == kVerifyPass) {
+    std::memcpy(h + L.ids(), r->vids.data(), items * sizeof(uint32_t));
+    ids_bytes = align256(items * sizeof(uint32_t));
+  }
+  r->passes += 1;
   auto* p = new Pass();
   p->r = r;
   p->kind = kind;
   p->items = items;
   p->first = first;
   p->slot = slot;
-  p->done_slot = (uint32_t)(S->next_done++ & (perm::kServeRing - 1));
-  p->tag = S->next_tag++;
-  S->done[p->done_slot].tag = 0;
-  // upload: the whole block (first pass: the records need no upload but a
-  // run of new blocks goes up as one copy), else this header slot through
-  // the ids area
-  p->up_v = first ? r->vblock : r->vblock + slot * kHeaderBytes;
-  p->up_bytes = first ? r->block_bytes : kDataOff - slot * kHeaderBytes;
+  p->up_off = off + L.slot(slot);
+  p->up_bytes = first ? kHeaderBytes + L.data_bytes : kHeaderBytes + ids_bytes;
   ServeHeader hd;
   std::memset(&hd, 0, sizeof(hd));
-  hd.colpm = (const double*)(d + kDataOff);
+  hd.colpm = (const double*)(d + L.data());
   hd.x0 = hd.colpm + r->cs_doubles;
-  hd.parts = (Partial*)(d + kDataOff + data_bytes);
-  hd.ids = kind == kVerifyPass ? (const uint32_t*)(d + kPassSlots * kHeaderBytes) : nullptr;
+  hd.parts = (Partial*)(d + L.records());
+  // the host reads the records of an FP64 pass over a non-integer matrix
+  // (sampled error measurement) and of the verification pass
+  if (kind == kVerifyPass || (kind == 0 && !r->integral))
+    hd.host_parts = (Partial*)(S->h_arena_dev + off + L.records());
+  hd.ids = kind == kVerifyPass ? (const uint32_t*)(d + L.ids()) : nullptr;
   hd.lanes = 1ull << r->high_bits;
   hd.walk_bits = r->walk_bits;
   hd.high_bits = r->high_bits;
   hd.variant = variant | (vmode == 2 ? (int32_t)perm::kServeDDFlag : 0);
   hd.items = items;
   hd.groups = r->groups;
-  hd.done_slot = p->done_slot;
-  hd.tag = p->tag;
-  std::memcpy(h + slot * kHeaderBytes, &hd, sizeof(hd));
+  std::memcpy(h + L.slot(slot), &hd, sizeof(hd));  // seq: at publication
   S->queued_cost += pass_cost(r, kind, items);
-  S->to_stage.push_back(p);
+  S->to_publish.push_back(p);
   return p;
 }
 
-// Copy the staged passes to the device -- first passes as runs of adjacent
-// new blocks, one copy per run; a later pass's header (and ids) alone -- and
-// queue the round behind its event.
-int upload_round(perm_server* S) {
-  if (S->to_stage.empty()) return PERM_OK;
-  Round R;
-  R.passes.swap(S->to_stage);
-  const uint64_t A = S->arena_bytes;
-  for (size_t i = 0; i < R.passes.size();) {
-    size_t j = i;
-    while (R.passes[i]->first && j + 1 < R.passes.size() && R.passes[j + 1]->first &&
-           R.passes[j + 1]->up_v == R.passes[j]->up_v + R.passes[j]->up_bytes &&
-           R.passes[j + 1]->up_v % A != 0)
-      ++j;
-    const size_t first = (size_t)(R.passes[i]->up_v % A);
-    const size_t bytes =
-        (size_t)(R.passes[j]->up_v - R.passes[i]->up_v) + R.passes[j]->up_bytes;
-    HIP_TRYS(hipMemcpyAsync(S->d_arena + first, S->h_arena + first, bytes,
-                            hipMemcpyHostToDevice, S->s_copy));
-    bump(S->n_uploads);
-    i = j + 1;
-  }
-  if (S->free_events.empty()) {
-    hipEvent_t e;
-    HIP_TRYS(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    S->free_events.push_back(e);
-  }
-  R.ev = S->free_events.back();
-  S->free_events.pop_back();
-  HIP_TRYS(hipEventRecord(R.ev, S->s_copy));
-  SERVE_TRACE("upload round of %zu passes", R.passes.size());
-  S->rounds.push_back(std::move(R));
-  return PERM_OK;
-}
-
-// Publish every round whose copy has completed (in order); launch the
-// kernel when work is in flight and no launch is live.
+// Publish the staged passes in order while the ring has room: sequence
+// number, header address and copy range into the control block's ring, then
+// `tail`; launch the kernels the passes in flight need.
 int publish(perm_server* S, bool& progressed) {
-  while (!S->rounds.empty()) {
-    Round& R = S->rounds.front();
-    const hipError_t q = hipEventQuery(R.ev);
-    if (q == hipErrorNotReady) break;
-    if (q != hipSuccess)
-      return perm::fail(PERM_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
-    for (Pass* p : R.passes) {
-      p->seq = S->next_seq++;
-      const size_t hoff = (size_t)(p->r->vblock % S->arena_bytes) + (size_t)p->slot * kHeaderBytes;
-      S->ctl->ring[p->seq & (perm::kServeRing - 1)] = (uint64_t)(uintptr_t)(S->d_arena + hoff);
-      S->inflight.push_back(p);
-      bump(S->n_items, p->items);
-    }
-    // the ring entries are visible before the tail that covers them
+  size_t count = 0;
+  while (!S->to_publish.empty() && S->next_seq - S->lo_seq < perm::kServeRing) {
+    Pass* p = S->to_publish.front();
+    S->to_publish.pop_front();
+    const uint64_t seq = S->next_seq++;
+    p->seq = seq;
+    std::memcpy(S->h_arena + p->up_off + offsetof(ServeHeader, seq), &seq, sizeof(seq));
+    const uint64_t q = seq & (perm::kServeRing - 1);
+    S->ctl->ring[q] = (uint64_t)(uintptr_t)(S->d_arena + p->up_off);
+    S->ctl->up[q] = ((uint64_t)(p->up_off / 256) << 32) | (uint64_t)(p->up_bytes / 256);
+    S->by_seq[q] = p;
+    S->n_inflight[is_dd(p->kind) ? 1 : 0] += 1;
+    bump(S->n_items, p->items);
+    ++count;
+  }
+  if (count > 0) {
+    // the mirror and the ring entries are visible before the tail that covers them
     __atomic_store_n(&S->ctl->tail, S->next_seq, __ATOMIC_RELEASE);
-    SERVE_TRACE("published %zu passes, tail %llu", R.passes.size(),
-                (unsigned long long)S->next_seq);
-    S->free_events.push_back(R.ev);
-    S->rounds.pop_front();
+    bump(S->n_publish);
     progressed = true;
+    SERVE_TRACE("published %zu passes, tail %llu", count, (unsigned long long)S->next_seq);
   }
   return ensure_kernels(S);
 }
@@ -532,16 +544,17 @@ void publish_finished(perm_server* S, const std::vector<Request*>& rs) {
   S->cv_done.notify_all();
 }
 
-// A pass's item records (sampled error measurement).
+// A pass's item records (sampled error measurement), from the host mirror:
+// the kernel stored them there before the pass's completion record.
 int read_parts(perm_server* S, const Pass* p, std::vector<Partial>& out) {
   const Request* r = p->r;
-  const size_t data_bytes = align256(r->cs_doubles * sizeof(double) + 2 * r->n_pad * sizeof(double));
   const Partial* parts =
-      (const Partial*)(S->d_arena + (size_t)(r->vblock % S->arena_bytes) + kDataOff + data_bytes);
+      (const Partial*)(S->h_arena + (size_t)(r->vblock % S->arena_bytes) + r->lay.records());
   out.resize(p->items);
-  HIP_TRYS(hipMemcpyAsync(out.data(), parts, p->items * sizeof(Partial),
-                          hipMemcpyDeviceToHost, S->s_copy));
-  HIP_TRYS(hipStreamSynchronize(S->s_copy));
+  for (uint32_t i = 0; i < p->items; ++i) {
+    const volatile Partial* q = parts + i;
+    out[i] = Partial{q->hi, q->lo, q->abs_sum, q->sq_sum, q->sq_run, 0.0};
+  }
   return PERM_OK;
 }
 
@@ -584,31 +597,38 @@ int finalize(perm_server* S, Request* r, const Pass* p, const Partial& rec, int&
   return next;
 }
 
-// Completed passes: results, further passes, publication to waiters.
+// Completed passes, in completion order: results, further passes,
+// publication to waiters.
 int retire(perm_server* S, bool& progressed) {
   std::vector<Request*> finished;
-  size_t keep = 0;
   int st = PERM_OK;
-  for (size_t i = 0; i < S->inflight.size(); ++i) {
-    Pass* p = S->inflight[i];
-    ServeDone& d = S->done[p->done_slot];
-    if (st != PERM_OK || __atomic_load_n(&d.tag, __ATOMIC_ACQUIRE) != p->tag + 1) {
-      S->inflight[keep++] = p;
-      continue;
-    }
+  for (;;) {
+    ServeDone& d = S->done[S->consumed & (perm::kServeRing - 1)];
+    if (__atomic_load_n(&d.tag, __ATOMIC_ACQUIRE) != S->consumed + 1) break;
     Partial rec = d.sum;
     rec.pad = 0.0;
+    const uint64_t seq = d.seq;
+    S->consumed += 1;
+    Pass*& slot = S->by_seq[seq & (perm::kServeRing - 1)];
+    Pass* p = slot;
+    if (!p || p->seq != seq) {
+      st = perm::fail(PERM_ERR_HIP, "served kernel completed an unknown pass");
+      break;
+    }
+    slot = nullptr;
+    S->n_inflight[is_dd(p->kind) ? 1 : 0] -= 1;
     Request* r = p->r;
     SERVE_TRACE("pass %llu (request %llu, kind %d) complete", (unsigned long long)p->seq,
                 (unsigned long long)r->id, p->kind);
     S->queued_cost -= pass_cost(r, p->kind, p->items);
     const int next = finalize(S, r, p, rec, st);
     delete p;
-    if (st != PERM_OK) continue;  // reported below, after the list is consistent
+    if (st != PERM_OK) break;
     if (next >= 0) {
       // the next pass lives in the request's block: it never waits for space
       bump(S->n_dd);
       make_pass(S, r, next, st);
+      if (st != PERM_OK) break;
     } else {
       r->result.n = r->n;
       r->result.subsets = std::ldexp(1.0, r->n - 1);
@@ -621,91 +641,78 @@ int retire(perm_server* S, bool& progressed) {
     }
     progressed = true;
   }
-  S->inflight.resize(keep);
+  while (S->lo_seq < S->next_seq && !S->by_seq[S->lo_seq & (perm::kServeRing - 1)]) ++S->lo_seq;
   publish_finished(S, finished);
   return st;
 }
 
 // Admit pending requests (arrival order) while resident requests, queued
-// work and arena space allow.
+// work and arena space allow.  Only the scheduler takes from `pending`: the
+// front requests are staged first, then popped together.
 int admit(perm_server* S, bool& progressed) {
-  std::vector<Request*> take;
-  {
-    std::lock_guard<std::mutex> lk(S->mu);
-    double queued = S->queued_cost;
-    int resident = S->resident;
-    while (!S->pending.empty() && resident < S->cfg.max_resident) {
-      Request* c = S->pending.front();
-      if (resident > 0 && queued + c->cost > S->cfg.max_queued_work) break;
-      S->pending.pop_front();
-      take.push_back(c);
-      queued += c->cost;
-      ++resident;
-    }
-  }
   size_t k = 0;
-  for (; k < take.size(); ++k) {
+  while (S->pending_n.load(std::memory_order_acquire) > (int)k &&
+         S->resident < S->cfg.max_resident) {
+    Request* c;
+    {
+      std::lock_guard<std::mutex> lk(S->mu_q);
+      if (k >= S->pending.size()) break;
+      c = S->pending[k];
+    }
+    if (S->resident > 0 && S->queued_cost + c->cost > S->cfg.max_queued_work) break;
     int st = PERM_OK;
-    Pass* p = make_pass(S, take[k], 0, st);
+    Pass* p = make_pass(S, c, 0, st);
     if (st != PERM_OK) return st;
-    if (!p) break;
+    if (!p) break;  // no arena room now
     S->resident += 1;
-    S->active.push_back(take[k]);
-  }
-  if (k < take.size()) {  // no arena room: back to the front, in order
-    std::lock_guard<std::mutex> lk(S->mu);
-    for (size_t i = take.size(); i-- > k;) S->pending.push_front(take[i]);
+    ++k;
   }
   if (k > 0) {
+    {
+      std::lock_guard<std::mutex> lk(S->mu_q);
+      for (size_t i = 0; i < k; ++i) S->pending.pop_front();
+      S->pending_n.store((int)S->pending.size(), std::memory_order_relaxed);
+    }
     progressed = true;
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->stats.admitted += (long long)k;
+    bump(S->n_admitted, (long long)k);
   }
   return PERM_OK;
 }
 
+// One round of the scheduler: completions first (they free arena space and
+// may stage further passes, and a launch that ended after its last pass is
+// not launched again for it), then admission, then publication (which
+// launches the kernels the passes in flight need).
 int scheduler_step(perm_server* S, bool& progressed) {
   Clock::time_point t0 = Clock::now();
+  if (int st = retire(S, progressed)) return st;
+  add_ms(S->t_retire, t0);
+  t0 = Clock::now();
   if (int st = admit(S, progressed)) return st;
-  if (int st = upload_round(S)) return st;
   add_ms(S->t_admit, t0);
   t0 = Clock::now();
   if (int st = publish(S, progressed)) return st;
   add_ms(S->t_publish, t0);
-  t0 = Clock::now();
-  if (int st = retire(S, progressed)) return st;
-  add_ms(S->t_retire, t0);
   return PERM_OK;
 }
 
-// Diagnostics (PERMAN_SERVE_TRACE): every 50 ms while passes are in flight,
-// the device-side state and the oldest pass's counters.
-void trace_device(perm_server* S) {
+// Diagnostics (PERMAN_SERVE_TRACE): every 5 ms while passes are in flight,
+// the host-side queue positions.
+void trace_state(perm_server* S) {
   static Clock::time_point last;
   const Clock::time_point now = Clock::now();
-  if (now - last < std::chrono::milliseconds(50)) return;
+  if (now - last < std::chrono::milliseconds(5)) return;
   last = now;
-  uint64_t st[8] = {};
-  uint32_t ctr[2] = {};
-  const Pass* p = S->inflight.front();
-  const char* hdr = S->d_arena + (size_t)(p->r->vblock % S->arena_bytes) +
-                    (size_t)p->slot * kHeaderBytes + offsetof(ServeHeader, next_item);
-  if (hipMemcpyAsync(st, S->d_state, sizeof(st), hipMemcpyDeviceToHost, S->s_copy) != hipSuccess ||
-      hipMemcpyAsync(ctr, hdr, sizeof(ctr), hipMemcpyDeviceToHost, S->s_copy) != hipSuccess ||
-      hipStreamSynchronize(S->s_copy) != hipSuccess)
-    return;
-  uint64_t rt = 0;
-  SERVE_TRACE("device: head %llu head_dd %llu pub %llu poll %llu activity %llu stop %llu | "
-              "pass %llu claimed %u done %u of %u | host tail %llu live %d%d",
-              (unsigned long long)st[0], (unsigned long long)st[1], (unsigned long long)st[2],
-              (unsigned long long)st[3], (unsigned long long)st[4], (unsigned long long)st[5],
-              (unsigned long long)p->seq, ctr[0], ctr[1], p->items,
-              (unsigned long long)S->ctl->tail, (int)launch_live(S, 0), (int)launch_live(S, 1));
-  (void)rt;
+  SERVE_TRACE("host: tail %llu lo %llu consumed %llu in flight %d/%d | live %d%d | "
+              "pending %d resident %d admitted %lld",
+              (unsigned long long)S->next_seq, (unsigned long long)S->lo_seq,
+              (unsigned long long)S->consumed, S->n_inflight[0], S->n_inflight[1],
+              (int)!launch_ending(S, 0), (int)!launch_ending(S, 1), S->pending_n.load(),
+              S->resident, S->n_admitted.load());
 }
 
 bool drained(perm_server* S) {
-  return S->inflight.empty() && S->rounds.empty() && S->to_stage.empty();
+  return S->n_inflight[0] == 0 && S->n_inflight[1] == 0 && S->to_publish.empty();
 }
 
 void fail_all(perm_server* S, int st) {
@@ -731,17 +738,17 @@ void scheduler_loop(perm_server* S) {
       return;
     }
     if (progressed) continue;
-    if (trace_on() && !S->inflight.empty()) trace_device(S);
-    std::unique_lock<std::mutex> lk(S->mu);
+    if (trace_on() && !drained(S)) trace_state(S);
+    std::unique_lock<std::mutex> lk(S->mu_q);
     if (!drained(S)) {
-      // work on the GPU or copies in flight: look again shortly (or at once
-      // on a new request)
+      // work on the GPU: look again shortly (or at once on a new request)
       S->cv_submit.wait_for(lk, std::chrono::microseconds(20),
-                            [&] { return S->stop || !S->pending.empty(); });
+                            [&] { return S->stopping.load() || !S->pending.empty(); });
       continue;
     }
-    if (S->stop && S->to_prep.empty() && S->prepping == 0 && S->pending.empty()) return;
-    S->cv_submit.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+    if (S->stopping.load() && S->to_prep.empty() && S->prepping == 0 && S->pending.empty())
+      return;
+    S->cv_submit.wait(lk, [&] { return S->stopping.load() || !S->pending.empty(); });
   }
 }
 
@@ -857,37 +864,58 @@ int prepare_request(perm_server* S, Request* r) {
   return PERM_OK;
 }
 
-// Prep thread: take submitted requests, prepare them, hand them to the
-// scheduler (or finish them: exactly 0, or a preparation error).
+// Prep thread: take submitted requests (a share of the queue at a time),
+// prepare them, hand them to the scheduler (or finish them: exactly 0, or a
+// preparation error).
 void prep_loop(perm_server* S) {
   perm::set_device(S->device);
+  const size_t threads = std::max<size_t>(1, S->preppers_n);
+  std::vector<Request*> got, ready, done_now;
   for (;;) {
-    Request* r;
+    got.clear();
     {
-      std::unique_lock<std::mutex> lk(S->mu);
-      S->cv_prep.wait(lk, [&] { return S->stop || !S->to_prep.empty(); });
+      std::unique_lock<std::mutex> lk(S->mu_q);
+      S->cv_prep.wait(lk, [&] { return S->stopping.load() || !S->to_prep.empty(); });
       if (S->to_prep.empty()) return;  // stopping and drained
-      r = S->to_prep.front();
-      S->to_prep.pop_front();
-      S->prepping += 1;
+      const size_t take = std::min<size_t>(16, std::max<size_t>(1, S->to_prep.size() / threads));
+      for (size_t i = 0; i < take; ++i) {
+        got.push_back(S->to_prep.front());
+        S->to_prep.pop_front();
+      }
+      S->prepping += (int)got.size();
     }
-    const int st = prepare_request(S, r);
-    bool done_now = false;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      S->prepping -= 1;
+    ready.clear();
+    done_now.clear();
+    const Clock::time_point t0 = Clock::now();
+    for (Request* r : got) {
+      const int st = prepare_request(S, r);
       if (st != PERM_OK || r->n_pad == 0) {  // finished at once (error, or exactly 0)
         if (st != PERM_OK) r->status = st;
-        r->t_done = Clock::now();
-        r->finished = true;
-        S->stats.completed += 1;
-        done_now = true;
+        done_now.push_back(r);
       } else {
-        S->pending.push_back(r);
+        ready.push_back(r);
       }
     }
-    if (done_now) S->cv_done.notify_all();
-    else S->cv_submit.notify_one();
+    add_ms(S->t_prep, t0);
+    if (!done_now.empty()) {
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        const Clock::time_point now = Clock::now();
+        for (Request* r : done_now) {
+          r->t_done = now;
+          r->finished = true;
+        }
+        S->stats.completed += (long long)done_now.size();
+      }
+      S->cv_done.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu_q);
+      for (Request* r : ready) S->pending.push_back(r);
+      S->prepping -= (int)got.size();
+      S->pending_n.store((int)S->pending.size(), std::memory_order_release);
+    }
+    S->cv_submit.notify_one();
   }
 }
 
@@ -921,11 +949,13 @@ int make_request(int dtype, const void* mat, const int* cptrs, const int* rows,
   return PERM_OK;
 }
 
-// Register requests (one lock) and wake the prep pool.
-int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* ids) {
+// Register requests (one lock) and wake the prep pool.  t0: when the
+// submission call began (statistics).
+int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* ids,
+            Clock::time_point t0) {
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    const char* why = S->fatal != PERM_OK ? "server failed: " : S->stop ? "server is shutting down" : nullptr;
+    const char* why = S->fatal != PERM_OK ? "server failed: " : S->stopping.load() ? "server is shutting down" : nullptr;
     if (why) {
       for (int i = 0; i < count; ++i) delete rs[i];
       return perm::fail(S->fatal != PERM_OK ? S->fatal : PERM_ERR_ARG,
@@ -937,11 +967,15 @@ int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* i
       r->t_submit = now;
       r->id = S->next_id++;
       S->table[r->id] = r;
-      S->to_prep.push_back(r);
       ids[i] = r->id;
     }
     S->stats.submitted += count;
   }
+  {
+    std::lock_guard<std::mutex> lk(S->mu_q);
+    for (int i = 0; i < count; ++i) S->to_prep.push_back(rs[i]);
+  }
+  add_ms(S->t_submit, t0);
   if (count == 1) S->cv_prep.notify_one();
   else S->cv_prep.notify_all();
   return perm::ok();
@@ -987,13 +1021,8 @@ int wait_collect(perm_server* S, std::unique_lock<std::mutex>& lk, uint64_t id, 
 }
 
 void release(perm_server* S) {
-  for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
-  for (Round& R : S->rounds) {
-    if (R.ev) (void)hipEventDestroy(R.ev);
-    for (Pass* p : R.passes) delete p;
-  }
-  for (Pass* p : S->to_stage) delete p;
-  for (Pass* p : S->inflight) delete p;
+  for (Pass* p : S->to_publish) delete p;
+  for (Pass* p : S->by_seq) delete p;
   for (KernelSlot& K : S->K) {
     if (K.exec) (void)hipGraphExecDestroy(K.exec);
     if (K.graph) (void)hipGraphDestroy(K.graph);
@@ -1001,7 +1030,6 @@ void release(perm_server* S) {
     if (K.e1) (void)hipEventDestroy(K.e1);
     if (K.s) (void)hipStreamDestroy(K.s);
   }
-  if (S->s_copy) (void)hipStreamDestroy(S->s_copy);
   if (S->d_state) (void)hipFree(S->d_state);
   if (S->d_arena) (void)hipFree(S->d_arena);
   if (S->h_arena) (void)hipHostFree(S->h_arena);
@@ -1015,7 +1043,9 @@ int setup(perm_server* S) {
   const perm_server_config& c = S->cfg;
   S->arena_bytes = (uint64_t)(c.arena_mb * 1048576.0) / 256 * 256;
   HIP_TRYS(hipMalloc((void**)&S->d_arena, S->arena_bytes));
-  HIP_TRYS(hipHostMalloc((void**)&S->h_arena, S->arena_bytes, hipHostMallocDefault));
+  HIP_TRYS(hipHostMalloc((void**)&S->h_arena, S->arena_bytes, hipHostMallocCoherent));
+  HIP_TRYS(hipHostGetDevicePointer((void**)&S->h_arena_dev, S->h_arena, 0));
+  S->by_seq.assign(perm::kServeRing, nullptr);
   HIP_TRYS(hipHostMalloc((void**)&S->ctl, sizeof(ServeControl), hipHostMallocCoherent));
   std::memset(S->ctl, 0, sizeof(ServeControl));
   HIP_TRYS(hipHostMalloc((void**)&S->done, perm::kServeRing * sizeof(ServeDone),
@@ -1023,7 +1053,6 @@ int setup(perm_server* S) {
   std::memset(S->done, 0, perm::kServeRing * sizeof(ServeDone));
   HIP_TRYS(hipMalloc((void**)&S->d_state, sizeof(ServeState)));
   HIP_TRYS(hipMemset(S->d_state, 0, sizeof(ServeState)));
-  HIP_TRYS(hipStreamCreateWithFlags(&S->s_copy, hipStreamNonBlocking));
   HIP_TRYS(hipDeviceSynchronize());
   // the two persistent kernels, each captured once and replayed per launch;
   // the double-double X one on 1/16 of the main one's workgroups (each of its
@@ -1037,7 +1066,8 @@ int setup(perm_server* S) {
     const uint32_t grid = k == 0 ? (uint32_t)c.grid : (uint32_t)std::max(16, c.grid / 16);
     HIP_TRYS(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
     const hipError_t le =
-        perm::launch_serve(k == 1, S->ctl, S->d_state, S->done, idle_ticks, grid, K.s);
+        perm::launch_serve(k == 1, S->ctl, S->d_state, S->done, S->h_arena_dev, S->d_arena,
+                           idle_ticks, grid, K.s);
     const hipError_t ce = hipStreamEndCapture(K.s, &K.graph);
     HIP_TRYS(le);
     HIP_TRYS(ce);
@@ -1092,6 +1122,7 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
     return perm::fail(st, msg);
   }
   S->worker = std::thread(scheduler_main, S);
+  S->preppers_n = (size_t)c.prep_threads;
   for (int t = 0; t < c.prep_threads; ++t) S->preppers.emplace_back(prep_loop, S);
   *out = S;
   return perm::ok();
@@ -1100,9 +1131,10 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
 int perm_server_submit_dense(perm_server* S, int dtype, const void* mat, int nov,
                              unsigned long long* id) {
   if (!S || !id) return perm::fail(PERM_ERR_ARG, "null pointer");
+  const Clock::time_point t0 = Clock::now();
   Request* r = nullptr;
   if (int s = make_request(dtype, mat, nullptr, nullptr, nullptr, nov, r)) return s;
-  return enqueue(S, &r, 1, id);
+  return enqueue(S, &r, 1, id, t0);
 }
 
 int perm_server_submit_many(perm_server* S, int count, const int* dtypes,
@@ -1112,6 +1144,7 @@ int perm_server_submit_many(perm_server* S, int count, const int* dtypes,
   if (!S || count < 0 || (count > 0 && (!dtypes || !mats || !novs || !ids)))
     return perm::fail(PERM_ERR_ARG, "null pointer");
   if (cptrs && (!rows || !cvals)) return perm::fail(PERM_ERR_ARG, "null rows/cvals arrays");
+  const Clock::time_point t0 = Clock::now();
   std::vector<Request*> rs;
   rs.reserve(count);
   int st = PERM_OK;
@@ -1124,7 +1157,7 @@ int perm_server_submit_many(perm_server* S, int count, const int* dtypes,
   }
   const std::string err = st != PERM_OK ? perm::last_error() : "";
   if (!rs.empty())
-    if (int s = enqueue(S, rs.data(), (int)rs.size(), ids)) return s;
+    if (int s = enqueue(S, rs.data(), (int)rs.size(), ids, t0)) return s;
   if (st != PERM_OK)
     return perm::fail(st, "matrix " + std::to_string(rs.size()) + ": " + err);
   return perm::ok();
@@ -1134,9 +1167,10 @@ int perm_server_submit_sparse(perm_server* S, int dtype, const void* mat, const 
                               const int* rows, const void* cvals, int nov,
                               unsigned long long* id) {
   if (!S || !id || !cptrs) return perm::fail(PERM_ERR_ARG, "null pointer");
+  const Clock::time_point t0 = Clock::now();
   Request* r = nullptr;
   if (int s = make_request(dtype, mat, cptrs, rows, cvals, nov, r)) return s;
-  return enqueue(S, &r, 1, id);
+  return enqueue(S, &r, 1, id, t0);
 }
 
 int perm_server_wait(perm_server* S, unsigned long long id, perm_result* out,
@@ -1185,11 +1219,12 @@ int perm_server_stats_get(perm_server* S, perm_server_stats* out) {
   if (!S || !out) return perm::fail(PERM_ERR_ARG, "null pointer");
   std::lock_guard<std::mutex> lk(S->mu);
   *out = S->stats;
-  out->pending = (long long)S->pending.size();
+  out->admitted = S->n_admitted.load(std::memory_order_relaxed);
+  out->pending = S->pending_n.load(std::memory_order_relaxed);
   // scheduler-owned counters, read without its lock (statistics only)
   out->launches = S->n_launches.load(std::memory_order_relaxed);
   out->items = S->n_items.load(std::memory_order_relaxed);
-  out->uploads = S->n_uploads.load(std::memory_order_relaxed);
+  out->uploads = S->n_publish.load(std::memory_order_relaxed);
   out->stops = S->n_stops.load(std::memory_order_relaxed);
   out->laps = (long long)(S->vhead_seen.load(std::memory_order_relaxed) / std::max<uint64_t>(1, S->arena_bytes));
   out->admit_ms = S->t_admit.load(std::memory_order_relaxed);
@@ -1198,6 +1233,8 @@ int perm_server_stats_get(perm_server* S, perm_server_stats* out) {
   out->stop_ms = S->t_stop.load(std::memory_order_relaxed);
   out->gpu_ms = S->t_gpu.load(std::memory_order_relaxed);
   out->dd_passes = S->n_dd.load(std::memory_order_relaxed);
+  out->prep_ms = S->t_prep.load(std::memory_order_relaxed);
+  out->submit_ms = S->t_submit.load(std::memory_order_relaxed);
   return perm::ok();
 }
 
@@ -1205,7 +1242,8 @@ void perm_server_destroy(perm_server* S) {
   if (!S) return;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    S->stop = true;
+    std::lock_guard<std::mutex> lq(S->mu_q);
+    S->stopping.store(true);
   }
   S->cv_prep.notify_all();
   for (auto& t : S->preppers)
@@ -1215,7 +1253,6 @@ void perm_server_destroy(perm_server* S) {
   (void)hipSetDevice(S->device);
   for (KernelSlot& K : S->K)
     if (K.s) (void)hipStreamSynchronize(K.s);
-  if (S->s_copy) (void)hipStreamSynchronize(S->s_copy);
   release(S);
 }
 
@@ -1241,10 +1278,11 @@ int perm_server_drive(perm_server* S, int count, const int* dtypes, const void* 
           std::this_thread::sleep_for(due - now - std::chrono::microseconds(100));
       }
       const bool sp = cptrs && cptrs[i];
+      const Clock::time_point ts = Clock::now();
       Request* r = nullptr;
       int st = make_request(dtypes[i], mats[i], sp ? cptrs[i] : nullptr, sp ? rows[i] : nullptr,
                             sp ? cvals[i] : nullptr, novs[i], r);
-      if (st == PERM_OK) st = enqueue(S, &r, 1, &ids[i]);
+      if (st == PERM_OK) st = enqueue(S, &r, 1, &ids[i], ts);
       if (st != PERM_OK) {
         sub_error = perm::last_error();
         sub_status.store(st);

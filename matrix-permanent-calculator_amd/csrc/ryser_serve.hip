@@ -13,15 +13,18 @@
 //      called through one non-inlined function per variant, so each body is
 //      register-allocated on its own (under its kernel's waves per SIMD: two
 //      instances, FP64 / double-double products at 4, double-double X at 1);
-//   3. store the item's record; the wave whose increment of the pass's done
-//      counter is the last sums the pass's records in item order (lane i
-//      takes items i, i+64, ... in order, then the fixed butterfly) and
-//      writes the completion record to host memory: the sum depends only on
-//      the pass, never on which waves ran which items;
-//   4. with nothing to claim: one wave at a time (at most once per 0.5 us)
-//      reads the host's stop flag and tail and copies new ring entries into
-//      device memory; all waves leave together (a device stop flag) on the
-//      host's stop, or when the launch's activity count (claims, new passes)
+//   3. store the item's record (also in the host mirror when the header asks
+//      for it); the wave whose increment of the pass's done counter is the
+//      last sums the pass's records in item order (lane i takes items i,
+//      i+64, ... in order, then the fixed butterfly) and appends the
+//      completion record to the queue in host memory: the sum depends only
+//      on the pass, never on which waves ran which items;
+//   4. with nothing to claim: one wave at a time (a lock; at most once per
+//      0.5 us) reads the host's tail, copies up to 64 new passes' ranges
+//      from the host mirror into the arena, copies their ring entries and
+//      publishes them; all waves of a launch leave together (a device stop
+//      flag of their kind) on the host's stop (read by each wave at most
+//      once per ms), or when the kind's activity count (claims, new passes)
 //      has not moved for the idle time on the deciding wave's own clock.
 // Every wait is bounded by the idle time measured on the 100 MHz real-time
 // counter, so the kernel always ends once work stops arriving.
@@ -71,6 +74,16 @@ __device__ __forceinline__ void store_dev_d(double* p, double v) {
 // issues; the compiler moves no memory access across it.
 __device__ __forceinline__ void wait_mem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// A 64-bit word of a pass header, wave-uniform.  Header fields are read
+// with device-scope loads: a launch's waves scan past completed passes'
+// headers, whose arena space may be reused (and the new header copied in)
+// while the launch lives, so a plain load could return a stale line of an
+// XCD's L2.
+__device__ __forceinline__ uint64_t hword(const ServeHeader* h, size_t off) {
+  return uni64(load_dev((const uint64_t*)((const char*)h + off)));
+}
+#define HDR(h, field) hword((h), offsetof(ServeHeader, field))
+
 struct ItemSum {
   DD s;
   Moments m;
@@ -84,12 +97,13 @@ __device__ __noinline__ ItemSum serve_item(const ServeHeader* h, uint32_t walk_i
   constexpr ServeVariant v = kServeVariants[V];
   constexpr int N = v.n;
   constexpr int R = unroll_bits_for<N>();
-  const double* colpm = (const double*)uni64((uint64_t)h->colpm);
-  const double* x0 = (const double*)uni64((uint64_t)h->x0);
-  const uint64_t lanes = uni64(h->lanes);
-  const int wb = (int)uni32((uint32_t)h->walk_bits);
-  const int hb = (int)uni32((uint32_t)h->high_bits);
-  const uint32_t groups = uni32(h->groups);
+  const double* colpm = (const double*)HDR(h, colpm);
+  const double* x0 = (const double*)HDR(h, x0);
+  const uint64_t lanes = HDR(h, lanes);
+  const uint64_t bits = HDR(h, walk_bits);  // walk_bits, high_bits
+  const int wb = (int)(uint32_t)bits;
+  const int hb = (int)(uint32_t)(bits >> 32);
+  const uint32_t groups = (uint32_t)HDR(h, groups);
   ItemSum r = {{0.0, 0.0}, {0.0, 0.0, 0.0}};
   const uint64_t base = (uint64_t)walk_item * groups * kWave + (threadIdx.x & (kWave - 1));
   for (uint32_t g = 0; g < groups; ++g) {
@@ -129,11 +143,12 @@ __device__ __forceinline__ ItemSum dispatch(int vid, const ServeHeader* h, uint3
   return r;
 }
 
-// The last item of pass H is done: sum its records in item order and write
-// the completion record (lane 0), tag last.
-__device__ void finish_pass(const ServeHeader* H, ServeDone* done, uint32_t lane) {
-  const uint32_t items = uni32(H->items);
-  const Partial* parts = (const Partial*)uni64((uint64_t)H->parts);
+// The last item of pass H is done: sum its records in item order and append
+// the completion record (lane 0) at the next queue position, tag last.
+__device__ void finish_pass(const ServeHeader* H, ServeState* st, ServeDone* done, uint32_t lane) {
+  const uint32_t items = (uint32_t)(HDR(H, variant) >> 32);  // variant, items
+  const Partial* parts = (const Partial*)HDR(H, parts);
+  const uint64_t seq = HDR(H, seq);
   DD s = {0.0, 0.0};
   Moments m = {0.0, 0.0, 0.0};
   for (uint32_t i = lane; i < items; i += kWave) {
@@ -145,21 +160,69 @@ __device__ void finish_pass(const ServeHeader* H, ServeDone* done, uint32_t lane
   }
   wave_reduce(s, m);
   if (lane == 0) {
-    ServeDone* d = done + H->done_slot;
+    const uint64_t slot = atomicAdd((unsigned long long*)&st->done_count, 1ull);
+    ServeDone* d = done + (slot & (kServeRing - 1));
     store_sys((uint64_t*)&d->sum.hi, __builtin_bit_cast(uint64_t, s.hi));
     store_sys((uint64_t*)&d->sum.lo, __builtin_bit_cast(uint64_t, s.lo));
     store_sys((uint64_t*)&d->sum.abs_sum, __builtin_bit_cast(uint64_t, m.a));
     store_sys((uint64_t*)&d->sum.sq_sum, __builtin_bit_cast(uint64_t, m.q));
     store_sys((uint64_t*)&d->sum.sq_run, __builtin_bit_cast(uint64_t, m.r));
-    wait_mem();  // the sum is in host memory before its tag
-    store_sys(&d->tag, H->tag + 1);
+    store_sys(&d->seq, seq);
+    wait_mem();  // the record is in host memory before its tag
+    store_sys(&d->tag, slot + 1);
+  }
+}
+
+// Copy the ranges of m <= 64 new passes (lane i < m holds pass i's range
+// descriptor `up`) from the host mirror into the arena, as one flat run of
+// 8-byte words spread over the lanes: kDepth loads in flight per lane, so
+// the copy of a batch of small passes takes a few PCIe round trips, not two
+// per pass.  pre/off: the workgroup's LDS (only the poll-lock holder uses it).
+constexpr uint32_t kCopyDepth = 12;
+__device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint32_t m,
+                           uint32_t lane, uint32_t* pre, uint64_t* off) {
+  const uint32_t words = lane < m ? (uint32_t)(up & 0xffffffffu) * 32u : 0u;  // 256 B = 32 words
+  uint32_t incl = words;  // inclusive prefix sum over the lanes
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, (unsigned)d, kWave);
+    if ((int)lane >= d) incl += y;
+  }
+  pre[lane] = incl - words;
+  off[lane] = (up >> 32) * 256;
+  if (lane == kWave - 1) pre[kWave] = incl;
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  const uint32_t total = pre[m];  // (m <= 64; pre[m] = total for every m)
+  uint32_t q = 0;                 // this lane's pass
+  for (uint32_t w0 = 0; w0 < total; w0 += kWave * kCopyDepth) {
+    uint64_t v[kCopyDepth], a[kCopyDepth];
+#pragma unroll
+    for (uint32_t j = 0; j < kCopyDepth; ++j) {
+      const uint32_t w = w0 + j * kWave + lane;
+      a[j] = 0;
+      v[j] = 0;
+      if (w < total) {
+        while (pre[q + 1] <= w) ++q;
+        a[j] = off[q] + (uint64_t)(w - pre[q]) * 8;
+        v[j] = load_sys((const uint64_t*)(h_arena + a[j]));
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kCopyDepth; ++j)
+      if (w0 + j * kWave + lane < total) store_dev((uint64_t*)(d_arena + a[j]), v[j]);
   }
 }
 
 // Passes one wave tries per claim round.
 constexpr uint64_t kScan = 64;
-// Least real-time ticks (10 ns) between two reads of the host's tail.
+// Least real-time ticks (10 ns) between two polls of the host's tail.
 constexpr uint64_t kPollTicks = 50;
+// Most passes one poll copies and publishes (one per lane of the poller).
+constexpr uint64_t kPollBatch = kWave;
+// Least real-time ticks between two reads of the host's stop flag by one wave
+// (1 ms: a few million reads per second over the whole launch).
+constexpr uint64_t kStopTicks = 100000;
 // Most real-time ticks between two increments of the activity count by a
 // wave that keeps claiming (and at most a quarter of the idle time).
 constexpr uint64_t kActivityTicks = 20000;
@@ -167,25 +230,43 @@ constexpr uint64_t kActivityTicks = 20000;
 // DD = false: the passes of FP64 and double-double-product variants, 4 waves
 // per SIMD (128 VGPRs).  DD = true: the double-double X passes (x and its
 // low parts take 4N VGPRs), 1 wave per SIMD.  Each kind skips the other's
-// passes and keeps its own head.
+// passes and keeps its own head, stop flag and activity count; either
+// kind's idle waves poll the host and copy new passes for both.
 template <bool DD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DD ? 1 : 4)))
-void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, uint64_t idle_ticks) {
+void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char* h_arena,
+                  char* d_arena, uint64_t idle_ticks) {
+  constexpr int k = DD ? 1 : 0;
   // The workgroup's current pass: its waves claim from it first, so the
   // waves of a CU tend to run one variant at a time and the instruction
   // cache holds few walk loops (several per CU thrash it).
   __shared__ unsigned long long wg_pass;
+  __shared__ uint32_t poll_pre[kWave + 1];  // the poll's copy plan (poll-lock holder only)
+  __shared__ uint64_t poll_off[kWave];
   if (threadIdx.x == 0) wg_pass = 0ull;
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  uint64_t* const head_p = DD ? &st->head_dd : &st->head;
-  uint64_t cursor = uni64(load_sys(DD ? &ctl->head_dd : &ctl->head));
-  // idle: the launch-wide activity count has not changed for idle_ticks of
-  // this wave's own clock
-  uint64_t idle_seen = ~0ull, idle_since = 0, last_activity = 0;
+  uint64_t cursor = uni64(load_sys(&ctl->head[k]));
+  const uint64_t epoch = uni64(load_sys(&ctl->epoch[k]));
+  // idle: the kind's activity count has not changed for idle_ticks of this
+  // wave's own clock
+  uint64_t idle_seen = ~0ull, idle_since = 0, last_activity = 0, last_stop = 0;
+  uint32_t idle_rounds = 0;  // rounds in a row without work (longer sleeps after a few)
   const uint64_t activity_ticks = idle_ticks / 4 < kActivityTicks ? idle_ticks / 4 : kActivityTicks;
   for (;;) {
-    if (uni64(load_dev(&st->stop)) != 0) break;  // the host's stop, or the launch went idle
+    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+    if (uni64(load_dev(&st->stop[k])) != 0) break;  // the host's stop, or the launch went idle
+    if (now - last_stop >= kStopTicks) {
+      last_stop = now;
+      if (uni64(load_sys(&ctl->stop)) != 0) {
+        if (lane == 0) {
+          atomicMax((unsigned long long*)&st->stop[k], 1ull);
+          store_sys(&ctl->why[k], 2ull);
+          store_sys(&ctl->ended[k], epoch);
+        }
+        break;
+      }
+    }
     ServeHeader* H = nullptr;
     uint32_t item = 0, items = 0;
     // first the workgroup's current pass
@@ -193,9 +274,7 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, uint64_t i
       ServeHeader* h = (ServeHeader*)uni64(__hip_atomic_load(&wg_pass, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP));
       if (h) {
-        const uint32_t n_items = uni32(__hip_atomic_load(&h->items, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT));
-        // one increment per wave (lane 0 adds 1, the others 0)
+        const uint32_t n_items = (uint32_t)(HDR(h, variant) >> 32);
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&h->next_item, 1u);
         t = uni32(t);
@@ -203,22 +282,28 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, uint64_t i
           H = h;
           item = t;
           items = n_items;
+        } else if (lane == 0) {
+          // every item is claimed: the workgroup's waves stop trying it (an
+          // atomic on a hot line per idle round slows every claimer)
+          unsigned long long cur = (unsigned long long)(uintptr_t)h;
+          __hip_atomic_compare_exchange_strong(&wg_pass, &cur, 0ull, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
     }
-    const uint64_t pub = uni64(load_dev(&st->pub));
     if (!H) {
-      const uint64_t head = uni64(load_dev(head_p));
+      const uint64_t pub = uni64(load_dev(&st->pub));
+      const uint64_t head = uni64(load_dev(&st->head[k]));
       if (cursor < head) cursor = head;
       const uint64_t scan_end = pub < cursor + kScan ? pub : cursor + kScan;
       for (uint64_t q = cursor; q < scan_end; ++q) {
         ServeHeader* h = (ServeHeader*)uni64(load_dev(&st->dring[q & (kServeRing - 1)]));
-        if (((uni32((uint32_t)h->variant) & kServeDDFlag) != 0) != DD) {
+        const uint64_t vi = HDR(h, variant);  // variant, items
+        if ((((uint32_t)vi & kServeDDFlag) != 0) != DD) {
           if (q == cursor) ++cursor;  // the other kernel's pass
           continue;
         }
-        const uint32_t n_items = uni32(__hip_atomic_load(&h->items, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT));
+        const uint32_t n_items = (uint32_t)(vi >> 32);
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&h->next_item, 1u);
         t = uni32(t);
@@ -230,34 +315,44 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, uint64_t i
         }
         if (q == cursor) ++cursor;  // every item of pass q is claimed
       }
-      if (cursor > head) atomicMax((unsigned long long*)head_p, (unsigned long long)cursor);
+      if (cursor > head) atomicMax((unsigned long long*)&st->head[k], (unsigned long long)cursor);
       if (H && lane == 0)
         __hip_atomic_store(&wg_pass, (unsigned long long)(uintptr_t)H, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    const uint64_t now = __builtin_amdgcn_s_memrealtime();
     if (H) {
       idle_seen = ~0ull;
-      // the launch's activity count moves at least every activity_ticks
-      // while this wave claims items (not on every claim: one counter for
-      // the whole launch)
+      idle_rounds = 0;
+      // the kind's activity count moves at least every activity_ticks while
+      // this wave claims items (not on every claim: one counter per kind)
       if (now - last_activity >= activity_ticks) {
         last_activity = now;
-        if (lane == 0) atomicAdd((unsigned long long*)&st->activity, 1ull);
+        if (lane == 0) atomicAdd((unsigned long long*)&st->activity[k], 1ull);
       }
-      const uint32_t* ids = (const uint32_t*)uni64((uint64_t)H->ids);
-      const uint32_t walk_item = ids ? uni32(ids[item]) : item;
-      const int vid = (int)(uni32((uint32_t)H->variant) & ~kServeDDFlag);
+      uint32_t* ids = (uint32_t*)HDR(H, ids);
+      const uint32_t walk_item =
+          ids ? uni32(__hip_atomic_load(ids + item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+              : item;
+      const int vid = (int)((uint32_t)HDR(H, variant) & ~kServeDDFlag);
       const ItemSum r = dispatch<DD>(vid, H, walk_item,
                                      std::make_integer_sequence<int, kNumServeVariants>{});
       if (lane == 0) {
-        Partial* p = (Partial*)uni64((uint64_t)H->parts) + item;
         const Partial v = make_partial(r.s, r.m);
+        Partial* p = (Partial*)HDR(H, parts) + item;
         store_dev_d(&p->hi, v.hi);
         store_dev_d(&p->lo, v.lo);
         store_dev_d(&p->abs_sum, v.abs_sum);
         store_dev_d(&p->sq_sum, v.sq_sum);
         store_dev_d(&p->sq_run, v.sq_run);
+        Partial* hp = (Partial*)HDR(H, host_parts);
+        if (hp) {  // the host reads this pass's records
+          hp += item;
+          store_sys((uint64_t*)&hp->hi, __builtin_bit_cast(uint64_t, v.hi));
+          store_sys((uint64_t*)&hp->lo, __builtin_bit_cast(uint64_t, v.lo));
+          store_sys((uint64_t*)&hp->abs_sum, __builtin_bit_cast(uint64_t, v.abs_sum));
+          store_sys((uint64_t*)&hp->sq_sum, __builtin_bit_cast(uint64_t, v.sq_sum));
+          store_sys((uint64_t*)&hp->sq_run, __builtin_bit_cast(uint64_t, v.sq_run));
+        }
       }
       wait_mem();  // the record is stored before the item counts as done
       uint32_t d = 0;
@@ -265,51 +360,75 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, uint64_t i
       d = uni32(d);
       if (d + 1 == items) {
         wait_mem();  // every record is read after the last count
-        finish_pass(H, done, lane);
+        finish_pass(H, st, done, lane);
       }
       continue;
     }
-    // nothing to claim: at most one wave at a time reads the host's control
-    // block (stop flag, tail) and copies what it finds into device memory
-    const uint64_t last = uni64(load_dev(&st->poll));
-    bool poll = false;
-    if (now - last >= kPollTicks) {
-      unsigned long long old = 0;
-      if (lane == 0)
-        old = atomicCAS((unsigned long long*)&st->poll, (unsigned long long)last,
-                        (unsigned long long)now);
-      poll = uni64(old) == last;
-    }
-    if (poll) {
-      if (uni64(load_sys(&ctl->stop)) != 0) {
-        if (lane == 0) atomicMax((unsigned long long*)&st->stop, 1ull);
-        break;
-      }
-      const uint64_t tail = uni64(load_sys(&ctl->tail));
-      wait_mem();  // the ring entries are read after the tail
-      if (tail > pub) {
-        for (uint64_t q = pub + lane; q < tail; q += kWave)
-          store_dev(&st->dring[q & (kServeRing - 1)], load_sys(&ctl->ring[q & (kServeRing - 1)]));
-        wait_mem();  // the entries are stored before they are published
-        if (lane == 0) {
-          atomicMax((unsigned long long*)&st->pub, (unsigned long long)tail);
-          atomicAdd((unsigned long long*)&st->activity, 1ull);
+    // nothing to claim: one wave at a time (the poll lock) reads the host's
+    // tail, copies the new passes' ranges from the host mirror into the
+    // arena, and publishes them
+    if (now - uni64(load_dev(&st->poll)) >= kPollTicks && uni64(load_dev(&st->poll_lock)) == 0) {
+      unsigned long long got = 1;
+      if (lane == 0) got = atomicCAS((unsigned long long*)&st->poll_lock, 0ull, 1ull);
+      if (uni64(got) == 0) {
+        if (lane == 0) store_dev(&st->poll, now);
+        const uint64_t tail = uni64(load_sys(&ctl->tail));
+        const uint64_t pub = uni64(load_dev(&st->pub));
+        wait_mem();  // the ring entries are read after the tail
+        const uint64_t end = tail < pub + kPollBatch ? tail : pub + kPollBatch;
+        const uint32_t m = end > pub ? (uint32_t)(end - pub) : 0u;  // <= kPollBatch = 64
+        if (m > 0) {
+          const uint64_t q = (pub + lane) & (kServeRing - 1);
+          const uint64_t up = lane < m ? load_sys(&ctl->up[q]) : 0ull;
+          const uint64_t entry = lane < m ? load_sys(&ctl->ring[q]) : 0ull;
+          copy_batch(h_arena, d_arena, up, m, lane, poll_pre, poll_off);
+          if (lane < m) store_dev(&st->dring[q], entry);
         }
-        continue;
+        wait_mem();  // the copies and entries are stored before they are published
+        if (lane == 0) {
+          if (end > pub) {
+            store_dev(&st->pub, end);
+            atomicAdd((unsigned long long*)&st->activity[k], 1ull);
+          }
+          store_dev(&st->poll, __builtin_amdgcn_s_memrealtime());
+        }
+        wait_mem();  // pub lands before the lock is free
+        if (lane == 0) store_dev(&st->poll_lock, 0ull);
+        if (end > pub) continue;
       }
     }
-    // the launch ends when the activity count stood still for idle_ticks:
-    // the wave that sees it stops every wave, so a launch never lingers with
-    // a few waves (the host launches again for anything published meanwhile)
-    const uint64_t act = uni64(load_dev(&st->activity));
+    // the launch ends when the kind's activity count stood still for
+    // idle_ticks: the wave that sees it stops every wave of the launch, so a
+    // launch never lingers with a few waves (the host launches again for
+    // anything published meanwhile)
+    const uint64_t act = uni64(load_dev(&st->activity[k]));
     if (act != idle_seen) {
       idle_seen = act;
       idle_since = now;
+    } else if (now - idle_since > idle_ticks &&
+               // never while a poll is copying passes in, or the host has
+               // published passes no poll has copied yet (a slow first
+               // touch of the host mirror must not end the launch)
+               (uni64(load_dev(&st->poll_lock)) != 0 ||
+                uni64(load_sys(&ctl->tail)) > uni64(load_dev(&st->pub)))) {
+      idle_since = now;
     } else if (now - idle_since > idle_ticks) {
-      if (lane == 0) atomicMax((unsigned long long*)&st->stop, 1ull);
+      if (lane == 0) {
+        atomicMax((unsigned long long*)&st->stop[k], 1ull);
+        store_sys(&ctl->why[k], 1ull | (act << 8));
+        store_sys(&ctl->ended[k], epoch);  // the host may launch again (after this one)
+      }
       break;
     }
-    __builtin_amdgcn_s_sleep(16);
+    // back off: every idle round reads a few hot device-scope lines, and
+    // thousands of idle waves polling them slow the claims of the busy ones
+    ++idle_rounds;
+    if (idle_rounds < 4) {
+      __builtin_amdgcn_s_sleep(16);  // about 0.4 us
+    } else {
+      const uint32_t naps = idle_rounds < 12 ? 1u : idle_rounds < 20 ? 2u : 8u;  // 3.4 .. 27 us
+      for (uint32_t i = 0; i < naps; ++i) __builtin_amdgcn_s_sleep(127);
+    }
   }
 }
 
@@ -324,10 +443,15 @@ int serve_variant(int n_pad, NestShape s, int mode) {
 }
 
 hipError_t launch_serve(bool dd, ServeControl* ctl, ServeState* st, ServeDone* done,
-                        uint64_t idle_ticks, uint32_t grid, hipStream_t stream) {
+                        const char* h_arena, char* d_arena, uint64_t idle_ticks, uint32_t grid,
+                        hipStream_t stream) {
   if (grid == 0) return hipErrorInvalidValue;
-  if (dd) serve_kernel<true><<<dim3(grid), dim3(256), 0, stream>>>(ctl, st, done, idle_ticks);
-  else serve_kernel<false><<<dim3(grid), dim3(256), 0, stream>>>(ctl, st, done, idle_ticks);
+  if (dd)
+    serve_kernel<true><<<dim3(grid), dim3(256), 0, stream>>>(ctl, st, done, h_arena, d_arena,
+                                                             idle_ticks);
+  else
+    serve_kernel<false><<<dim3(grid), dim3(256), 0, stream>>>(ctl, st, done, h_arena, d_arena,
+                                                              idle_ticks);
   return hipGetLastError();
 }
 

@@ -2,21 +2,25 @@
 // host-written control block it polls, and its variant table.
 //
 // The server (perman_serve.cpp) keeps every resident matrix and every
-// partial sum in one device arena.  Per pass of a request (its FP64 walk, or
-// a double-double pass) the host uploads a ServeHeader followed by the
-// matrix's column store and start vector, then publishes the header's
-// device address in a ring inside pinned, coherent host memory
-// (ServeControl) and advances `tail`.  One persistent kernel polls `tail`:
-// its waves claim the published passes' items through the headers' atomic
-// counters (publication order), walk them, store each item's record in the
-// pass's record array, and the wave that finishes a pass's last item sums
-// the records in item order and writes the pass's sum into a completion
-// record in pinned host memory (ServeDone).  Only one wave at a time reads
-// the host's control block (at most once per 0.5 us) and copies what it
-// finds into device memory.  A launch ends when the host sets `stop`, or
-// when no wave has claimed an item or seen a new pass for `idle` ticks: its
-// waves leave together, never one by one while work may still come.  The host launches it again (one captured graph) when it
-// publishes work and no launch is live.
+// partial sum in one device arena, mirrored at the same offsets in pinned
+// host memory.  Per pass of a request (its FP64 walk, or a double-double
+// pass) the host writes a ServeHeader (and, on the first pass, the matrix's
+// column store and start vector) into the mirror, then publishes it in a
+// ring inside pinned, coherent host memory (ServeControl): the header's
+// device address and the mirror range to copy.  One wave at a time of the
+// persistent kernel reads `tail`, copies the new ranges from the mirror into
+// the device arena, and publishes them to the other waves (`pub`): no copy
+// is issued by the host.  The waves claim the published passes' items
+// through the headers' atomic counters (publication order), walk them,
+// store each item's record in the pass's record array (and in the host
+// mirror when the host asks for a pass's records), and the wave that
+// finishes a pass's last item sums the records in item order and appends
+// the pass's sum to a completion queue in pinned host memory (ServeDone),
+// which the host reads in order.  A launch ends when the host sets `stop`,
+// or when no wave of its kind has claimed an item or seen a new pass for
+// `idle` ticks: its waves leave together, never one by one while work may
+// still come.  The host launches it again (one captured graph) when it
+// publishes work and no launch of that kind is live.
 //
 // Orders are served by kernels of padded orders: a request of order n runs
 // the kernel of N = serve_padded_n(n) >= n, its start vector padded with
@@ -42,56 +46,61 @@ constexpr uint32_t kServeRing = 1u << 16;
 // 64-lane groups gives each item several groups.
 constexpr uint32_t kServeMaxItems = 4096;
 
-// One pass of one request, in the device arena (uploaded; the kernel only
-// writes the two counters, which the upload zeroes).  128 bytes.
+// One pass of one request, in the device arena (copied from the host mirror
+// by the polling wave, which thereby zeroes the counters).  192 bytes.
 struct ServeHeader {
   const double* colpm;   // column store of the padded order N: 2(N-1)+1 blocks of N
   const double* x0;      // start vector (N doubles), then its low parts (N more)
   Partial* parts;        // one record per item of this pass
+  Partial* host_parts;   // nullptr, or the same records in the host mirror
   const uint32_t* ids;   // walk item index of each item (nullptr: item i is walk item i)
   uint64_t lanes;        // lanes of the walk, 2^high_bits
+  uint64_t seq;          // publication sequence (echoed in the completion record)
   int32_t walk_bits, high_bits;
   int32_t variant;       // served kernel variant (padded order, walk shape, mode)
   uint32_t items;        // items of this pass
   uint32_t groups;       // 64-lane groups per walk item
-  uint32_t done_slot;    // completion record index
-  // second 64-byte line: the counters the kernel updates (zero at upload)
+  // third 64-byte line: the counters the kernel updates (zero in the mirror)
   alignas(64) uint32_t next_item;  // items claimed
   uint32_t done_items;             // items finished
-  uint64_t tag;                    // echoed in the completion record
 };
-static_assert(sizeof(ServeHeader) == 128, "ServeHeader layout");
+static_assert(sizeof(ServeHeader) == 192, "ServeHeader layout");
 
-// Completion record of one pass (pinned host memory): the pass's sum and
-// moments; `tag` = header tag + 1 once written (stored after the sum has landed).
+// Completion record of one pass (pinned host memory, a queue in completion
+// order): the pass's sum and moments, its sequence, and `tag` = the record's
+// queue position + 1, stored after the rest has landed.
 struct ServeDone {
   Partial sum;
   uint64_t tag;
-  uint64_t pad;
+  uint64_t seq;
 };
 static_assert(sizeof(ServeDone) == 64, "ServeDone layout");
 
 // Host-written control block (pinned, coherent host memory), polled by the
 // kernel with system-scope loads.
 struct ServeControl {
-  uint64_t tail;     // passes published: ring[0, tail mod kServeRing) hold header addresses
+  uint64_t tail;     // passes published: ring/up[0, tail mod kServeRing) are written
   uint64_t stop;     // nonzero: waves leave after their current item
-  uint64_t head;     // oldest main-kernel pass in flight when its launch began
-  uint64_t head_dd;  // the same for the double-double X kernel
-  uint64_t pad[4];
+  uint64_t head[2];  // oldest pass of each kernel kind in flight when its launch began
+  uint64_t epoch[2];  // launch number of each kind (written before the launch)
+  uint64_t ended[2];  // written by the kernel: the launch number that decided to end
+  uint64_t why[2];    // and why (diagnostics): 1 idle, 2 the host's stop; activity << 8
+  uint64_t pad[6];
   uint64_t ring[kServeRing];  // device address of the header of pass q at q mod kServeRing
+  uint64_t up[kServeRing];    // range to copy for pass q: (offset / 256) << 32 | bytes / 256
 };
 
-// Device-side state, kept across launches.
+// Device-side state, kept across launches.  Index k: 0 the main kernel, 1
+// the double-double X kernel.
 struct ServeState {
-  uint64_t head;      // main-kernel passes before it have every item claimed
-  uint64_t head_dd;   // the same for the double-double X kernel
-  uint64_t pub;       // passes whose header addresses are in dring
-  uint64_t poll;      // real-time tick of the last read of the host's control block
-  uint64_t activity;  // count of claims and new passes (idle: it stands still)
-  uint64_t stop;      // all waves leave: the host's stop, or the launch went idle
-                      // (zeroed before each launch)
-  uint64_t pad[2];
+  uint64_t pub;          // passes copied and in dring (written by the poll-lock holder)
+  uint64_t poll;         // real-time tick of the last poll of the host's control block
+  uint64_t poll_lock;    // 1 while a wave polls and copies
+  uint64_t done_count;   // completion records written
+  uint64_t head[2];      // passes of kind k before head[k] have every item claimed
+  uint64_t stop[2];      // all waves of kind k leave (zeroed before each launch)
+  uint64_t activity[2];  // kind k's claims and publications (idle: it stands still)
+  uint64_t pad[6];
   uint64_t dring[kServeRing];  // device copy of ServeControl::ring
 };
 
@@ -140,8 +149,10 @@ constexpr uint32_t kServeDDFlag = 1u << 16;
 // Launch the persistent kernel of one kind (dd: the double-double X passes;
 // else all others): `grid` workgroups of 256 lanes; it returns after
 // `idle_ticks` (100 MHz ticks) without claimable work of its kind, or when
-// ctl->stop is set.
+// ctl->stop is set.  h_arena: the device address of the host mirror of
+// d_arena.
 hipError_t launch_serve(bool dd, ServeControl* ctl, ServeState* st, ServeDone* done,
-                        uint64_t idle_ticks, uint32_t grid, hipStream_t stream);
+                        const char* h_arena, char* d_arena, uint64_t idle_ticks, uint32_t grid,
+                        hipStream_t stream);
 
 }  // namespace perm

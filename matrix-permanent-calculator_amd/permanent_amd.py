@@ -102,7 +102,8 @@ class ServerStatsC(ctypes.Structure):
                  "uploads", "stops", "laps")] + \
                [(k, ctypes.c_double) for k in
                 ("admit_ms", "publish_ms", "retire_ms", "stop_ms", "gpu_ms")] + \
-               [("dd_passes", ctypes.c_longlong)]
+               [("dd_passes", ctypes.c_longlong)] + \
+               [(k, ctypes.c_double) for k in ("prep_ms", "submit_ms")]
 
 
 PERM_PENDING = 5

@@ -252,9 +252,16 @@ def test_server_relaunch_after_idle_and_arena_wrap():
         again = srv.wait(srv.submit(mats[0]))[0]
         assert (first.sum_hi, first.sum_lo) == (again.sum_hi, again.sum_lo)
         assert srv.stats()["launches"] >= 2
-        # ~30 MiB of blocks submitted at once: while the kernel works, space
-        # freed during its launch is reused only after stopping it
-        res = srv.wait_many([srv.submit(m) for _ in range(12) for m in mats])
+    # order 30 throughout: the GPU (not the host) sets the pace, requests
+    # finish about in arrival order (blocks are freed in order), and one
+    # launch lives through the wrap
+    mats = [P.synthetic_matrix(30, 0.5, seed=5200 + i) for i in range(12)]
+    with P.Server(arena_mb=16.0, item_log2=12, idle_us=20000.0) as srv:
+        # (a long idle time: the launch lives across the host's turnarounds)
+        # ~60 MiB of blocks submitted at once (live blocks span at most half
+        # the arena): while the kernel works, space freed during its launch
+        # is reused only after stopping it
+        res = srv.wait_many([srv.submit(m) for _ in range(24) for m in mats])
         st = srv.stats()
     print(st)
     assert st["laps"] >= 1
