@@ -393,7 +393,7 @@ typedef struct perm_server perm_server;
 typedef struct perm_server_config {
   int item_log2;          /* subsets per item, log2 (default 16) */
   int grid;               /* workgroups of the persistent kernel (default 1024: 4 waves per SIMD) */
-  int max_items;          /* most items per pass (default 4096: more lane groups per item above) */
+  int max_items;          /* most items per pass (default 4096, at most 16384: more lane groups per item above) */
   double idle_us;         /* a launch of the persistent kernel ends after this long without work (2000) */
   int max_resident;       /* requests resident on the device (default 4096) */
   double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */

@@ -83,14 +83,18 @@ using perm::ServeState;
 using Clock = std::chrono::steady_clock;
 
 // Pass kinds: 0 FP64, 1 double-double products, 2 double-double X;
-// kVerifyPass: sampled items of the FP64 pass walked in mode 2.
+// kVerifyPass: sampled items of the FP64 pass walked in mode 2;
+// kExportPass: the FP64 pass's records copied to the host mirror (no walk),
+// before the sampling that picks the verification's items.
 constexpr int kVerifyPass = 3;
+constexpr int kExportPass = 4;
 // Request block: header slot 0, the column store and start vector, header
-// slot 1, the sampled ids, header slot 2, the item records -- so each pass's
-// copy is one contiguous range: slot 0 with the matrix, slot 1 with the
-// ids (the verification pass is always the second), slot 2 alone.
+// slots 1 and 2, the sampled ids, header slot 3, the item records -- so each
+// pass's copy is one contiguous range: slot 0 with the matrix, slot 2 with
+// the ids (the verification pass follows the FP64 pass and its export),
+// slots 1 and 3 alone.
 constexpr size_t kHeaderBytes = 256;
-constexpr int kPassSlots = 3;  // FP64; then double-double products, or verification then X
+constexpr int kPassSlots = 4;  // FP64; then double-double products, or export, verification, X
 constexpr size_t kIdsBytes = (perm::kVerifyUnits * sizeof(uint32_t) + 255) / 256 * 256;
 static_assert(sizeof(perm::ServeHeader) <= kHeaderBytes, "header slot");
 
@@ -100,11 +104,14 @@ size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 struct BlockLayout {
   size_t data_bytes;
   size_t slot(int i) const {
-    return i == 0 ? 0 : i == 1 ? kHeaderBytes + data_bytes : 2 * kHeaderBytes + data_bytes + kIdsBytes;
+    return i == 0 ? 0
+         : i == 1 ? kHeaderBytes + data_bytes
+         : i == 2 ? 2 * kHeaderBytes + data_bytes
+                  : 3 * kHeaderBytes + data_bytes + kIdsBytes;
   }
   size_t data() const { return kHeaderBytes; }
-  size_t ids() const { return slot(1) + kHeaderBytes; }
-  size_t records() const { return slot(2) + kHeaderBytes; }
+  size_t ids() const { return slot(2) + kHeaderBytes; }
+  size_t records() const { return slot(3) + kHeaderBytes; }
 };
 
 // A submitted matrix as the caller gave it (copied), prepared by the pool.
@@ -120,7 +127,7 @@ struct Request;
 // One pass of a request: a header (and records) in one arena block.
 struct Pass {
   Request* r = nullptr;
-  int kind = 0;          // 0, 1, 2, kVerifyPass
+  int kind = 0;          // 0, 1, 2, kVerifyPass, kExportPass
   uint32_t items = 0;
   uint64_t seq = 0;      // publication sequence
   bool first = false;    // the request's first pass (its copy carries the matrix)
@@ -226,6 +233,8 @@ struct perm_server {
   std::deque<Pass*> to_publish;  // written to the mirror, not published yet
   std::vector<Pass*> by_seq;     // passes in flight at seq mod kServeRing
   int n_inflight[2] = {0, 0};    // passes in flight per kernel kind
+  bool dd_waiting = false;       // double-double X passes wait for a main-kernel stop
+  Clock::time_point dd_wait_since;
   KernelSlot K[2];  // [0] the main kernel, [1] the double-double X kernel
   std::atomic<double> t_admit{0.0}, t_publish{0.0}, t_retire{0.0}, t_stop{0.0}, t_gpu{0.0};
   std::atomic<double> t_prep{0.0}, t_submit{0.0};  // (added to under mu)
@@ -279,6 +288,7 @@ double subset_cost(int n_pad, perm::NestShape s, int kind) {
 }
 
 double pass_cost(const Request* r, int kind, uint32_t items) {
+  if (kind == kExportPass) return 0.0;
   return (double)items * r->groups * std::ldexp((double)perm::kWave, r->walk_bits) *
          subset_cost(r->n_pad, r->shape, kind);
 }
@@ -368,15 +378,31 @@ int stop_kernels(perm_server* S) {
 }
 
 // Launch the kernels the passes in flight need.  The double-double X kernel
-// (1 wave per SIMD) is launched before the main one (4 waves per SIMD takes
-// every register file it is given): a live main launch is stopped first and
-// launched again after it.
+// (1 wave per SIMD) can only become resident where the main one (4 waves
+// per SIMD take every register file) leaves room: when double-double X
+// passes have waited kDDWaitMs behind a live main launch, that launch is
+// stopped (its waves finish their items), the double-double X kernel
+// launched, and the main one launched again after it.  The wait lets one
+// stop serve the double-double passes that arrive close together.
+constexpr double kDDWaitMs = 5.0;
 int ensure_kernels(perm_server* S) {
   const bool main_work = S->n_inflight[0] > 0, dd_work = S->n_inflight[1] > 0;
   if (dd_work && launch_ending(S, 1)) {
-    if (launch_live(S, 0))
-      if (int s = stop_kernels(S)) return s;
-    if (int s = launch(S, 1)) return s;
+    bool go = true;
+    if (launch_live(S, 0)) {
+      const Clock::time_point now = Clock::now();
+      if (!S->dd_waiting) {
+        S->dd_waiting = true;
+        S->dd_wait_since = now;
+      }
+      go = std::chrono::duration<double, std::milli>(now - S->dd_wait_since).count() >= kDDWaitMs;
+      if (go)
+        if (int s = stop_kernels(S)) return s;
+    }
+    if (go) {
+      S->dd_waiting = false;
+      if (int s = launch(S, 1)) return s;
+    }
   }
   if (main_work && launch_ending(S, 0))
     if (int s = launch(S, 0)) return s;
@@ -431,14 +457,19 @@ void arena_free(perm_server* S, size_t block) {
 Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
   st = PERM_OK;
   const bool first = !r->resident;
-  const uint32_t items = kind == kVerifyPass ? (uint32_t)r->vids.size() : r->walk_items;
+  const uint32_t items =
+      kind == kVerifyPass ? (uint32_t)r->vids.size()
+      : kind == kExportPass
+          ? (r->walk_items + perm::kServeExportChunk - 1) / perm::kServeExportChunk
+          : r->walk_items;
   const int vmode = kind == kVerifyPass ? 2 : kind;
   const perm::NestShape vshape =
       vmode == 2 ? perm::NestShape{r->n_pad, r->n_pad, r->n_pad} : r->shape;
-  const int variant = perm::serve_variant(r->n_pad, vshape, vmode);
+  const int variant = kind == kExportPass ? 0 : perm::serve_variant(r->n_pad, vshape, vmode);
   const int slot = r->passes;
   if (variant < 0 || slot >= kPassSlots || items > r->walk_items ||
-      (kind == kVerifyPass && (items > perm::kVerifyUnits || slot != 1))) {  // not expected
+      (kind == kVerifyPass && (items > perm::kVerifyUnits || slot != 2)) ||
+      (kind == kExportPass && slot != 1)) {  // not expected
     st = perm::fail(PERM_ERR_ARG, "no served kernel variant or pass slot for this request");
     return nullptr;
   }
@@ -482,15 +513,16 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
   hd.colpm = (const double*)(d + L.data());
   hd.x0 = hd.colpm + r->cs_doubles;
   hd.parts = (Partial*)(d + L.records());
-  // the host reads the records of an FP64 pass over a non-integer matrix
-  // (sampled error measurement) and of the verification pass
-  if (kind == kVerifyPass || (kind == 0 && !r->integral))
+  // the host reads the records of the verification pass, and those of the
+  // FP64 pass through its export pass
+  if (kind == kVerifyPass || kind == kExportPass)
     hd.host_parts = (Partial*)(S->h_arena_dev + off + L.records());
   hd.ids = kind == kVerifyPass ? (const uint32_t*)(d + L.ids()) : nullptr;
-  hd.lanes = 1ull << r->high_bits;
+  hd.lanes = kind == kExportPass ? r->walk_items : 1ull << r->high_bits;
   hd.walk_bits = r->walk_bits;
   hd.high_bits = r->high_bits;
-  hd.variant = variant | (vmode == 2 ? (int32_t)perm::kServeDDFlag : 0);
+  hd.variant = kind == kExportPass ? (int32_t)perm::kServeExportFlag
+                                   : variant | (vmode == 2 ? (int32_t)perm::kServeDDFlag : 0);
   hd.items = items;
   hd.groups = r->groups;
   std::memcpy(h + L.slot(slot), &hd, sizeof(hd));  // seq: at publication
@@ -544,14 +576,14 @@ void publish_finished(perm_server* S, const std::vector<Request*>& rs) {
   S->cv_done.notify_all();
 }
 
-// A pass's item records (sampled error measurement), from the host mirror:
-// the kernel stored them there before the pass's completion record.
-int read_parts(perm_server* S, const Pass* p, std::vector<Partial>& out) {
-  const Request* r = p->r;
+// The first `count` item records of r (sampled error measurement), from the
+// host mirror: the kernel stored them there (an export or verification
+// pass) before that pass's completion record.
+int read_parts(perm_server* S, const Request* r, uint32_t count, std::vector<Partial>& out) {
   const Partial* parts =
       (const Partial*)(S->h_arena + (size_t)(r->vblock % S->arena_bytes) + r->lay.records());
-  out.resize(p->items);
-  for (uint32_t i = 0; i < p->items; ++i) {
+  out.resize(count);
+  for (uint32_t i = 0; i < count; ++i) {
     const volatile Partial* q = parts + i;
     out[i] = Partial{q->hi, q->lo, q->abs_sum, q->sq_sum, q->sq_run, 0.0};
   }
@@ -565,7 +597,7 @@ int finalize(perm_server* S, Request* r, const Pass* p, const Partial& rec, int&
   const int n = r->n_eff;
   if (p->kind == kVerifyPass) {
     std::vector<Partial> ver;
-    if ((st = read_parts(S, p, ver)) != PERM_OK) return -1;
+    if ((st = read_parts(S, r, p->items, ver)) != PERM_OK) return -1;
     std::vector<double> e(r->vids.size());
     for (size_t k = 0; k < r->vids.size(); ++k) {
       const Partial& f = r->fp_parts[r->vids[k]];
@@ -577,23 +609,26 @@ int finalize(perm_server* S, Request* r, const Pass* p, const Partial& rec, int&
     std::vector<Partial>().swap(r->fp_parts);
     return r->result.certified ? -1 : 2;
   }
-  perm::fill_item_result(n, r->walk_bits, r->integral, r->product_bits, r->shape_n, p->kind, rec,
-                         r->result);
-  const int next = perm::escalation_mode(r->result);
-  if (next == 2 && p->kind == 0 && !r->sampled && r->walk_items >= perm::kVerifyMinUnits) {
+  if (p->kind == kExportPass) {
+    // the FP64 pass's records are in the mirror: sample the items to walk
+    // again in double-double X (r->result still holds the FP64 result)
     std::vector<Partial> parts;
-    if ((st = read_parts(S, p, parts)) != PERM_OK) return -1;
+    if ((st = read_parts(S, r, r->walk_items, parts)) != PERM_OK) return -1;
     std::vector<double> z(parts.size());
     for (size_t i = 0; i < z.size(); ++i)
       z[i] = perm::unit_weight(parts[i].sq_sum, parts[i].sq_run, r->walk_bits, r->high_bits);
     perm::pps_sample(z, perm::kVerifyUnits, r->vids, r->vweight, r->vcertain);
-    if (!r->vids.empty()) {
-      r->sampled = true;
-      r->fp_result = r->result;
-      r->fp_parts.swap(parts);
-      return kVerifyPass;
-    }
+    r->sampled = true;
+    if (r->vids.empty()) return 2;
+    r->fp_result = r->result;
+    r->fp_parts.swap(parts);
+    return kVerifyPass;
   }
+  perm::fill_item_result(n, r->walk_bits, r->integral, r->product_bits, r->shape_n, p->kind, rec,
+                         r->result);
+  const int next = perm::escalation_mode(r->result);
+  if (next == 2 && p->kind == 0 && !r->sampled && r->walk_items >= perm::kVerifyMinUnits)
+    return kExportPass;
   return next;
 }
 
@@ -626,7 +661,7 @@ int retire(perm_server* S, bool& progressed) {
     if (st != PERM_OK) break;
     if (next >= 0) {
       // the next pass lives in the request's block: it never waits for space
-      bump(S->n_dd);
+      if (next != kExportPass) bump(S->n_dd);
       make_pass(S, r, next, st);
       if (st != PERM_OK) break;
     } else {
@@ -703,10 +738,17 @@ void trace_state(perm_server* S) {
   const Clock::time_point now = Clock::now();
   if (now - last < std::chrono::milliseconds(5)) return;
   last = now;
-  SERVE_TRACE("host: tail %llu lo %llu consumed %llu in flight %d/%d | live %d%d | "
-              "pending %d resident %d admitted %lld",
-              (unsigned long long)S->next_seq, (unsigned long long)S->lo_seq,
-              (unsigned long long)S->consumed, S->n_inflight[0], S->n_inflight[1],
+  // completion records written and not yet read
+  uint64_t ready = 0;
+  while (ready < perm::kServeRing &&
+         __atomic_load_n(&S->done[(S->consumed + ready) & (perm::kServeRing - 1)].tag,
+                         __ATOMIC_ACQUIRE) == S->consumed + ready + 1)
+    ++ready;
+  SERVE_TRACE("host: tail %llu device pub %llu lo %llu consumed %llu ready %llu in flight %d/%d"
+              " | live %d%d | pending %d resident %d admitted %lld",
+              (unsigned long long)S->next_seq, (unsigned long long)S->ctl->dev_pub,
+              (unsigned long long)S->lo_seq, (unsigned long long)S->consumed,
+              (unsigned long long)ready, S->n_inflight[0], S->n_inflight[1],
               (int)!launch_ending(S, 0), (int)!launch_ending(S, 1), S->pending_n.load(),
               S->resident, S->n_admitted.load());
 }
@@ -1057,7 +1099,9 @@ int setup(perm_server* S) {
   // the two persistent kernels, each captured once and replayed per launch;
   // the double-double X one on 1/16 of the main one's workgroups (each of its
   // workgroups holds a whole CU's register files)
-  const uint64_t idle_ticks = (uint64_t)std::max(1.0, c.idle_us * 100.0);  // 100 MHz counter
+  // (100 MHz counter; the double-double X kernel waits twice as long: it is
+  // launched only after stopping the main one)
+  const uint64_t idle_ticks = (uint64_t)std::max(1.0, c.idle_us * 100.0);
   for (int k = 0; k < 2; ++k) {
     KernelSlot& K = S->K[k];
     HIP_TRYS(hipStreamCreateWithFlags(&K.s, hipStreamNonBlocking));
@@ -1067,7 +1111,7 @@ int setup(perm_server* S) {
     HIP_TRYS(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
     const hipError_t le =
         perm::launch_serve(k == 1, S->ctl, S->d_state, S->done, S->h_arena_dev, S->d_arena,
-                           idle_ticks, grid, K.s);
+                           k == 1 ? 2 * idle_ticks : idle_ticks, grid, K.s);
     const hipError_t ce = hipStreamEndCapture(K.s, &K.graph);
     HIP_TRYS(le);
     HIP_TRYS(ce);
@@ -1084,7 +1128,7 @@ void perm_server_default_config(perm_server_config* c) {
   if (!c) return;
   c->item_log2 = 16;
   c->grid = 1024;
-  c->max_items = (int)perm::kServeMaxItems;
+  c->max_items = 4096;
   c->idle_us = 2000.0;
   c->max_resident = 4096;
   c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)

@@ -19,13 +19,16 @@
 //      i+64, ... in order, then the fixed butterfly) and appends the
 //      completion record to the queue in host memory: the sum depends only
 //      on the pass, never on which waves ran which items;
-//   4. with nothing to claim: one wave at a time (a lock; at most once per
-//      0.5 us) reads the host's tail, copies up to 64 new passes' ranges
-//      from the host mirror into the arena, copies their ring entries and
-//      publishes them; all waves of a launch leave together (a device stop
-//      flag of their kind) on the host's stop (read by each wave at most
-//      once per ms), or when the kind's activity count (claims, new passes)
-//      has not moved for the idle time on the deciding wave's own clock.
+//   4. with nothing to claim: one wave at a time (at most once per 0.5 us)
+//      reads the host's tail into device memory; idle waves take up to 16
+//      published passes at a time, copy their ranges from the host mirror
+//      into the arena (many waves in parallel: each GPU read of host memory
+//      is a PCIe round trip), mark them copied and advance `pub` over the
+//      passes copied so far, in publication order.  All waves of a launch
+//      leave together (a device stop flag of their kind) on the host's stop
+//      (read by each wave at most once per ms), or when the kind's activity
+//      count (claims, publications) has not moved for the idle time on the
+//      deciding wave's own clock and nothing published waits to be copied.
 // Every wait is bounded by the idle time measured on the 100 MHz real-time
 // counter, so the kernel always ends once work stops arriving.
 #include "ryser_dense_impl.hpp"
@@ -146,7 +149,9 @@ __device__ __forceinline__ ItemSum dispatch(int vid, const ServeHeader* h, uint3
 // The last item of pass H is done: sum its records in item order and append
 // the completion record (lane 0) at the next queue position, tag last.
 __device__ void finish_pass(const ServeHeader* H, ServeState* st, ServeDone* done, uint32_t lane) {
-  const uint32_t items = (uint32_t)(HDR(H, variant) >> 32);  // variant, items
+  const uint64_t vi = HDR(H, variant);  // variant, items
+  // (an export pass's sum is zero: its records are the walk's, not its own)
+  const uint32_t items = ((uint32_t)vi & kServeExportFlag) ? 0u : (uint32_t)(vi >> 32);
   const Partial* parts = (const Partial*)HDR(H, parts);
   const uint64_t seq = HDR(H, seq);
   DD s = {0.0, 0.0};
@@ -175,9 +180,9 @@ __device__ void finish_pass(const ServeHeader* H, ServeState* st, ServeDone* don
 
 // Copy the ranges of m <= 64 new passes (lane i < m holds pass i's range
 // descriptor `up`) from the host mirror into the arena, as one flat run of
-// 8-byte words spread over the lanes: kDepth loads in flight per lane, so
-// the copy of a batch of small passes takes a few PCIe round trips, not two
-// per pass.  pre/off: the workgroup's LDS (only the poll-lock holder uses it).
+// 8-byte words spread over the lanes: kCopyDepth loads in flight per lane,
+// so the copy of a batch of small passes takes a few PCIe round trips, not
+// two per pass.  pre/off: this wave's LDS scratch.
 constexpr uint32_t kCopyDepth = 12;
 __device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint32_t m,
                            uint32_t lane, uint32_t* pre, uint64_t* off) {
@@ -214,12 +219,59 @@ __device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint
   }
 }
 
+// Item `item` of an export pass: its chunk of the walk's records, device
+// arena -> host mirror (8 words in flight per lane).
+__device__ void export_item(const ServeHeader* H, uint32_t item, uint32_t lane) {
+  const uint64_t count = HDR(H, lanes);
+  const uint64_t r0 = (uint64_t)item * kServeExportChunk;
+  const uint64_t r1 = r0 + kServeExportChunk < count ? r0 + kServeExportChunk : count;
+  if (r1 <= r0) return;
+  constexpr uint32_t kWords = sizeof(Partial) / 8;
+  const uint64_t* src = (const uint64_t*)((const Partial*)HDR(H, parts) + r0);
+  uint64_t* dst = (uint64_t*)((Partial*)HDR(H, host_parts) + r0);
+  const uint32_t words = (uint32_t)(r1 - r0) * kWords;
+  for (uint32_t w0 = 0; w0 < words; w0 += kWave * 8) {
+    uint64_t v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+      const uint32_t w = w0 + j * kWave + lane;
+      v[j] = w < words ? load_dev(src + w) : 0ull;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+      const uint32_t w = w0 + j * kWave + lane;
+      if (w < words) store_sys(dst + w, v[j]);
+    }
+  }
+}
+
 // Passes one wave tries per claim round.
 constexpr uint64_t kScan = 64;
-// Least real-time ticks (10 ns) between two polls of the host's tail.
+// Least real-time ticks (10 ns) between two reads of the host's tail.
 constexpr uint64_t kPollTicks = 50;
-// Most passes one poll copies and publishes (one per lane of the poller).
-constexpr uint64_t kPollBatch = kWave;
+// Most passes one idle wave takes to copy in at a time (many waves copy in
+// parallel: a GPU read of host memory is a PCIe round trip).
+constexpr uint64_t kCopyBatch = 16;
+
+// Advance `pub` over the passes whose copies are complete (64 flags per
+// round, one per lane); a wave that advanced it bumps the kind's activity.
+__device__ void advance_pub(ServeControl* ctl, ServeState* st, int k, uint32_t lane) {
+  const uint64_t p0 = uni64(load_dev(&st->pub));
+  uint64_t p = p0;
+  for (;;) {
+    const uint64_t q = p + lane;
+    const bool ok = load_dev(&st->copied[q & (kServeRing - 1)]) == q + 1;
+    const uint64_t miss = ~__builtin_amdgcn_ballot_w64(ok);
+    const uint64_t run = miss ? (uint64_t)__builtin_ctzll(miss) : (uint64_t)kWave;
+    p += run;
+    if (run < (uint64_t)kWave) break;
+  }
+  if (p > p0 && lane == 0) {
+    atomicMax((unsigned long long*)&st->pub, (unsigned long long)p);
+    atomicAdd((unsigned long long*)&st->activity[k], 1ull);
+    store_sys(&ctl->dev_pub, p);
+  }
+}
 // Least real-time ticks between two reads of the host's stop flag by one wave
 // (1 ms: a few million reads per second over the whole launch).
 constexpr uint64_t kStopTicks = 100000;
@@ -241,11 +293,12 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
   // waves of a CU tend to run one variant at a time and the instruction
   // cache holds few walk loops (several per CU thrash it).
   __shared__ unsigned long long wg_pass;
-  __shared__ uint32_t poll_pre[kWave + 1];  // the poll's copy plan (poll-lock holder only)
-  __shared__ uint64_t poll_off[kWave];
+  __shared__ uint32_t copy_pre[256 / kWave][kWave + 1];  // each wave's copy plan
+  __shared__ uint64_t copy_off[256 / kWave][kWave];
   if (threadIdx.x == 0) wg_pass = 0ull;
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = threadIdx.x / kWave;
   uint64_t cursor = uni64(load_sys(&ctl->head[k]));
   const uint64_t epoch = uni64(load_sys(&ctl->epoch[k]));
   // idle: the kind's activity count has not changed for idle_ticks of this
@@ -329,11 +382,21 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
         last_activity = now;
         if (lane == 0) atomicAdd((unsigned long long*)&st->activity[k], 1ull);
       }
+      const uint32_t vflags = (uint32_t)HDR(H, variant);
+      if (!DD && (vflags & kServeExportFlag)) {
+        export_item(H, item, lane);
+        wait_mem();  // the chunk is in host memory before the item counts as done
+        uint32_t d = 0;
+        if (lane == 0) d = atomicAdd(&H->done_items, 1u);
+        d = uni32(d);
+        if (d + 1 == items) finish_pass(H, st, done, lane);
+        continue;
+      }
       uint32_t* ids = (uint32_t*)HDR(H, ids);
       const uint32_t walk_item =
           ids ? uni32(__hip_atomic_load(ids + item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
               : item;
-      const int vid = (int)((uint32_t)HDR(H, variant) & ~kServeDDFlag);
+      const int vid = (int)(vflags & ~kServeDDFlag);
       const ItemSum r = dispatch<DD>(vid, H, walk_item,
                                      std::make_integer_sequence<int, kNumServeVariants>{});
       if (lane == 0) {
@@ -364,37 +427,49 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
       }
       continue;
     }
-    // nothing to claim: one wave at a time (the poll lock) reads the host's
-    // tail, copies the new passes' ranges from the host mirror into the
-    // arena, and publishes them
-    if (now - uni64(load_dev(&st->poll)) >= kPollTicks && uni64(load_dev(&st->poll_lock)) == 0) {
-      unsigned long long got = 1;
-      if (lane == 0) got = atomicCAS((unsigned long long*)&st->poll_lock, 0ull, 1ull);
-      if (uni64(got) == 0) {
-        if (lane == 0) store_dev(&st->poll, now);
-        const uint64_t tail = uni64(load_sys(&ctl->tail));
-        const uint64_t pub = uni64(load_dev(&st->pub));
-        wait_mem();  // the ring entries are read after the tail
-        const uint64_t end = tail < pub + kPollBatch ? tail : pub + kPollBatch;
-        const uint32_t m = end > pub ? (uint32_t)(end - pub) : 0u;  // <= kPollBatch = 64
-        if (m > 0) {
-          const uint64_t q = (pub + lane) & (kServeRing - 1);
+    // nothing to claim: (a) one wave per kPollTicks reads the host's tail
+    // into device memory; (b) idle waves take up to kCopyBatch of the passes
+    // the host published, copy their ranges from the host mirror into the
+    // arena and their ring entries into dring, mark them copied, and
+    // advance `pub` over the passes copied so far (in order)
+    {
+      const uint64_t last = uni64(load_dev(&st->poll));
+      if (now - last >= kPollTicks) {
+        unsigned long long old = 0;
+        if (lane == 0)
+          old = atomicCAS((unsigned long long*)&st->poll, (unsigned long long)last,
+                          (unsigned long long)now);
+        if (uni64(old) == last) {
+          const uint64_t tail = uni64(load_sys(&ctl->tail));
+          if (lane == 0) atomicMax((unsigned long long*)&st->host_tail, (unsigned long long)tail);
+        }
+      }
+    }
+    {
+      const uint64_t ht = uni64(load_dev(&st->host_tail));
+      const uint64_t c = uni64(load_dev(&st->copy_claim));
+      if (c < ht) {
+        const uint64_t n = ht - c < kCopyBatch ? ht - c : kCopyBatch;
+        unsigned long long old = ~0ull;
+        if (lane == 0)
+          old = atomicCAS((unsigned long long*)&st->copy_claim, (unsigned long long)c,
+                          (unsigned long long)(c + n));
+        if (uni64(old) == c) {
+          // (the host wrote the ring entries before the tail this wave or
+          // another read: they are read after it)
+          const uint32_t m = (uint32_t)n;
+          const uint64_t q = (c + lane) & (kServeRing - 1);
           const uint64_t up = lane < m ? load_sys(&ctl->up[q]) : 0ull;
           const uint64_t entry = lane < m ? load_sys(&ctl->ring[q]) : 0ull;
-          copy_batch(h_arena, d_arena, up, m, lane, poll_pre, poll_off);
+          copy_batch(h_arena, d_arena, up, m, lane, copy_pre[wave], copy_off[wave]);
           if (lane < m) store_dev(&st->dring[q], entry);
+          wait_mem();  // the copies and entries land before they are marked
+          if (lane < m) store_dev(&st->copied[q], c + lane + 1);
+          wait_mem();
+          advance_pub(ctl, st, k, lane);
+          idle_rounds = 0;
+          continue;
         }
-        wait_mem();  // the copies and entries are stored before they are published
-        if (lane == 0) {
-          if (end > pub) {
-            store_dev(&st->pub, end);
-            atomicAdd((unsigned long long*)&st->activity[k], 1ull);
-          }
-          store_dev(&st->poll, __builtin_amdgcn_s_memrealtime());
-        }
-        wait_mem();  // pub lands before the lock is free
-        if (lane == 0) store_dev(&st->poll_lock, 0ull);
-        if (end > pub) continue;
       }
     }
     // the launch ends when the kind's activity count stood still for
@@ -406,11 +481,10 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
       idle_seen = act;
       idle_since = now;
     } else if (now - idle_since > idle_ticks &&
-               // never while a poll is copying passes in, or the host has
-               // published passes no poll has copied yet (a slow first
-               // touch of the host mirror must not end the launch)
-               (uni64(load_dev(&st->poll_lock)) != 0 ||
-                uni64(load_sys(&ctl->tail)) > uni64(load_dev(&st->pub)))) {
+               // never while the host has published passes not yet copied in
+               // (a slow first touch of the host mirror must not end the
+               // launch)
+               uni64(load_sys(&ctl->tail)) > uni64(load_dev(&st->pub))) {
       idle_since = now;
     } else if (now - idle_since > idle_ticks) {
       if (lane == 0) {

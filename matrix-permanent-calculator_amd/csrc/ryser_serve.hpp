@@ -44,7 +44,9 @@ constexpr int serve_padded_n(int n) { return n < kServeMinN ? kServeMinN : (n + 
 constexpr uint32_t kServeRing = 1u << 16;
 // Most items of one pass (the per-pass record array); a pass with more
 // 64-lane groups gives each item several groups.
-constexpr uint32_t kServeMaxItems = 4096;
+constexpr uint32_t kServeMaxItems = 16384;
+// Records one item of an export pass copies to the host mirror.
+constexpr uint32_t kServeExportChunk = 512;
 
 // One pass of one request, in the device arena (copied from the host mirror
 // by the polling wave, which thereby zeroes the counters).  192 bytes.
@@ -85,7 +87,8 @@ struct ServeControl {
   uint64_t epoch[2];  // launch number of each kind (written before the launch)
   uint64_t ended[2];  // written by the kernel: the launch number that decided to end
   uint64_t why[2];    // and why (diagnostics): 1 idle, 2 the host's stop; activity << 8
-  uint64_t pad[6];
+  uint64_t dev_pub;   // the device's publication count after each poll (diagnostics)
+  uint64_t pad[5];
   uint64_t ring[kServeRing];  // device address of the header of pass q at q mod kServeRing
   uint64_t up[kServeRing];    // range to copy for pass q: (offset / 256) << 32 | bytes / 256
 };
@@ -93,15 +96,17 @@ struct ServeControl {
 // Device-side state, kept across launches.  Index k: 0 the main kernel, 1
 // the double-double X kernel.
 struct ServeState {
-  uint64_t pub;          // passes copied and in dring (written by the poll-lock holder)
-  uint64_t poll;         // real-time tick of the last poll of the host's control block
-  uint64_t poll_lock;    // 1 while a wave polls and copies
+  uint64_t pub;          // passes copied in and in dring, in order (claimable below it)
+  uint64_t poll;         // real-time tick of the last read of the host's tail
+  uint64_t host_tail;    // the host's tail as last read
+  uint64_t copy_claim;   // passes below it are taken by a copying wave
   uint64_t done_count;   // completion records written
   uint64_t head[2];      // passes of kind k before head[k] have every item claimed
   uint64_t stop[2];      // all waves of kind k leave (zeroed before each launch)
   uint64_t activity[2];  // kind k's claims and publications (idle: it stands still)
-  uint64_t pad[6];
-  uint64_t dring[kServeRing];  // device copy of ServeControl::ring
+  uint64_t pad[5];
+  uint64_t dring[kServeRing];   // device copy of ServeControl::ring
+  uint64_t copied[kServeRing];  // q + 1 once pass q's range and ring entry are in
 };
 
 // Served kernel variants: padded order N in {8, 12, ..., 40}; FP64 and
@@ -145,6 +150,10 @@ constexpr std::array<ServeVariant, kNumServeVariants> kServeVariants = make_serv
 int serve_variant(int n_pad, NestShape s, int mode);
 // Flag in ServeHeader::variant: a double-double X pass (run by the DD kernel).
 constexpr uint32_t kServeDDFlag = 1u << 16;
+// Flag in ServeHeader::variant: an export pass (main kernel) -- no walk; item
+// i copies records [i * kServeExportChunk, ...) of `parts` (`lanes` records
+// in all) to `host_parts`, for the host's sampled error measurement.
+constexpr uint32_t kServeExportFlag = 1u << 17;
 
 // Launch the persistent kernel of one kind (dd: the double-double X passes;
 // else all others): `grid` workgroups of 256 lanes; it returns after
