@@ -315,8 +315,11 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
     reqs = [P.Request(m, sp) for m, sp in P.served_workload(count, seed, decay=decay)]
     work = sum(2.0 * q.a.shape[0] * 2.0 ** (q.a.shape[0] - 1) for q in reqs)
     batch = P.RequestBatch(reqs)
+    # the tuning sample: a few times the server's resident capacity for the
+    # small-request mix (a 1000-request trial lasts only milliseconds there)
+    sample_n = min(count, 1000 if decay > 2 else 8000)
     sample = [P.Request(m, sp)
-              for m, sp in P.served_workload(min(count, 1000), seed + 100, decay=decay)]
+              for m, sp in P.served_workload(sample_n, seed + 100, decay=decay)]
     t0 = time.perf_counter()
     cfg, tuned_rate = P.autotune_server(sample)
     t_tune = time.perf_counter() - t0
@@ -335,8 +338,19 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
     gaps = -np.log(1.0 - P.uniforms((seed << 20) + (1 << 18), count)) / rate
     arrivals = np.cumsum(gaps)
     with P.Server(**cfg) as srv:
-        _, lat_c, t_open = srv.drive(batch, arrivals)
+        open_c, lat_c, t_open = srv.drive(batch, arrivals)
     lat_ms = np.array(list(lat_c))
+    open_modes = np.array([r.mode for r in open_c])
+    open_n = np.array([q.a.shape[0] for q in reqs])
+    slowest = int(np.argmax(lat_ms))
+    tail = {}
+    for label, sel in (("fp64_pass_only", open_modes == 0),
+                       ("escalated", open_modes != 0)):
+        if sel.any():
+            tail[label] = {"requests": int(sel.sum()),
+                           "p50_ms": float(np.percentile(lat_ms[sel], 50)),
+                           "p99_ms": float(np.percentile(lat_ms[sel], 99)),
+                           "max_ms": float(lat_ms[sel].max())}
     progress(f"  open loop done in {t_open:.1f} s")
     plain_c = (P.PermResultC * count)()
     one = P.PermResultC()
@@ -367,6 +381,8 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
         "latency_p50_ms": float(np.percentile(lat_ms, 50)),
         "latency_p99_ms": float(np.percentile(lat_ms, 99)),
         "latency_max_ms": float(lat_ms.max()),
+        "latency_worst_request": {"n": int(open_n[slowest]), "mode": int(open_modes[slowest])},
+        "latency_by_passes": tail,
         "plain_req_per_s": count / t_plain,
         "capacity_vs_plain": t_plain / t_closed,
         "max_rel_diff_vs_plain": worst,

@@ -395,7 +395,7 @@ typedef struct perm_server_config {
   int grid;               /* workgroups of the persistent kernel (default 1024: 4 waves per SIMD) */
   int max_items;          /* most items per pass (default 4096, at most 16384: more lane groups per item above) */
   double idle_us;         /* a launch of the persistent kernel ends after this long without work (2000) */
-  int max_resident;       /* requests resident on the device (default 4096) */
+  int max_resident;       /* requests resident on the device (default 8192, at most 32768) */
   double max_queued_work; /* admitted work cap, FP64 lane-instructions (4e13) */
   int prep_threads;       /* host threads preparing submitted matrices (default 4) */
   double arena_mb;        /* device arena for matrices and item records, MiB (default 256) */

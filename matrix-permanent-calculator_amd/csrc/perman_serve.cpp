@@ -632,12 +632,17 @@ int finalize(perm_server* S, Request* r, const Pass* p, const Partial& rec, int&
   return next;
 }
 
+// Most completions retired, and most requests admitted, per scheduler round:
+// the GPU gets new work while the host still has completions to read
+// (retiring thousands before admitting any leaves the kernel idle).
+constexpr int kRoundBatch = 256;
+
 // Completed passes, in completion order: results, further passes,
 // publication to waiters.
 int retire(perm_server* S, bool& progressed) {
   std::vector<Request*> finished;
   int st = PERM_OK;
-  for (;;) {
+  for (int taken = 0; taken < kRoundBatch; ++taken) {
     ServeDone& d = S->done[S->consumed & (perm::kServeRing - 1)];
     if (__atomic_load_n(&d.tag, __ATOMIC_ACQUIRE) != S->consumed + 1) break;
     Partial rec = d.sum;
@@ -687,7 +692,7 @@ int retire(perm_server* S, bool& progressed) {
 int admit(perm_server* S, bool& progressed) {
   size_t k = 0;
   while (S->pending_n.load(std::memory_order_acquire) > (int)k &&
-         S->resident < S->cfg.max_resident) {
+         S->resident < S->cfg.max_resident && k < (size_t)kRoundBatch) {
     Request* c;
     {
       std::lock_guard<std::mutex> lk(S->mu_q);
@@ -1130,7 +1135,7 @@ void perm_server_default_config(perm_server_config* c) {
   c->grid = 1024;
   c->max_items = 4096;
   c->idle_us = 2000.0;
-  c->max_resident = 4096;
+  c->max_resident = 8192;
   c->max_queued_work = 4.0e13;  // lane-instructions (about one second of GPU work)
   c->prep_threads = 4;
   c->arena_mb = 256.0;
