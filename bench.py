@@ -31,8 +31,9 @@ FULL_CPU_SET = os.sched_getaffinity(0)
 
 # HBM bytes per launch of the n=40 kernel on this workload: rocprofv3 FETCH_SIZE
 # + WRITE_SIZE, each scaled by its reading for a copy kernel of known size
-# (profiles/ryser_dense_n40.md): 691.5 KiB read + 16097.3 KiB written.
-TRAFFIC_BYTES_N40 = (691.5 + 16097.3) * 1024
+# (profiles/ryser_dense_n40.md): 719.4 KiB read + 128778.2 KiB written (2^21
+# workgroup records of 48 B, each one 64-B line write).
+TRAFFIC_BYTES_N40 = (719.4 + 128778.2) * 1024
 FP64_PEAK_TFLOPS = 78.6
 N = 40
 DENSITY = 0.5
@@ -475,7 +476,7 @@ def main():
             "bound_detail": "FP64 vector (VALU) compute: the kernel issues FP64 VALU instructions "
                             "and no MFMA; the FP64 vector peak is 78.6 TFLOP/s (256 CUs x 128 "
                             "flop/clk x 2.4 GHz, FMA = 2); HBM does not bound it (traffic below, "
-                            "~26 MB/s)",
+                            "~0.2 GB/s)",
             "clock_ghz": r.clock_ghz,
             "peak_at_measured_clock": FP64_PEAK_TFLOPS * r.clock_ghz / 2.4 if r.clock_ghz else None,
             "frac_at_measured_clock": achieved / (FP64_PEAK_TFLOPS * r.clock_ghz / 2.4)

@@ -38,12 +38,7 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const Partial* __r
     c += in[i].pad;
   }
   clk[threadIdx.x] = c;
-  block_reduce_store(s, m, scratch, out);  // (its __syncthreads orders clk)
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < (int)blockDim.x; ++i) t += clk[i];
-    out->pad = t;
-  }
+  block_reduce_store(s, m, scratch, out, 0.0, clk);  // (its first __syncthreads orders clk)
 }
 
 int dense_unroll_bits(int n) { return n - 1 < 3 ? n - 1 : 3; }

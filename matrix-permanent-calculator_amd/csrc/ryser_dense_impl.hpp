@@ -446,13 +446,12 @@ void ryser_dense_kernel(const double* __restrict__ colpm, const double* __restri
     s = {0.0, 0.0};
     m = {0.0, 0.0, 0.0};
   }
-  block_reduce_store(s, m, scratch, partials + blockIdx.x);
-  if (threadIdx.x == 0) {
-    const uint64_t c1 = __builtin_amdgcn_s_memtime();
-    const uint64_t r1 = __builtin_amdgcn_s_memrealtime();
-    // cycles per 10 ns tick; the grid reduction sums it (pad) over workgroups
-    partials[blockIdx.x].pad = r1 > r0 ? (double)(c1 - c0) / (double)(r1 - r0) : 0.0;
-  }
+  // cycles per 10 ns tick over the walk (thread 0's); the grid reduction
+  // sums it (pad) over workgroups
+  const uint64_t c1 = __builtin_amdgcn_s_memtime();
+  const uint64_t r1 = __builtin_amdgcn_s_memrealtime();
+  const double ratio = r1 > r0 ? (double)(c1 - c0) / (double)(r1 - r0) : 0.0;
+  block_reduce_store(s, m, scratch, partials + blockIdx.x, ratio);
 }
 
 template <int N>

@@ -110,10 +110,13 @@ __device__ __forceinline__ Partial make_partial(DD s, const Moments& m) {
   return {s.hi, s.lo, m.a, m.q, m.r, 0.0};
 }
 
-// Workgroup reduction of (s, m) into one Partial written by thread 0.
+// Workgroup reduction of (s, m) into one Partial (thread 0 sums the waves).
 // `scratch` must hold blockDim.x/64 entries.
+// pad: the record's last word as thread 0 holds it, or, with pad_terms,
+// the sum of pad_terms[0, blockDim.x) (written by the block before the call).
 __device__ __forceinline__ void block_reduce_store(DD s, Moments m, Partial* scratch,
-                                                   Partial* out) {
+                                                   Partial* out, double pad = 0.0,
+                                                   const double* pad_terms = nullptr) {
   wave_reduce(s, m);
   const int wid = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
@@ -128,8 +131,21 @@ __device__ __forceinline__ void block_reduce_store(DD s, Moments m, Partial* scr
       tm.q += scratch[w].sq_sum;
       tm.r += scratch[w].sq_run;
     }
-    *out = make_partial(t, tm);
+    scratch[0] = make_partial(t, tm);
+    if (pad_terms) {
+      pad = 0.0;
+      for (int i = 0; i < (int)blockDim.x; ++i) pad += pad_terms[i];
+    }
+    scratch[0].pad = pad;
   }
+  __syncthreads();
+  // the record goes out as one store instruction of six lanes (one 48-byte
+  // request) instead of three 16-byte stores of one lane, each a separate
+  // partial-line write to HBM
+  constexpr int kWords = (int)(sizeof(Partial) / sizeof(double));
+  if (threadIdx.x < kWords)
+    reinterpret_cast<double*>(out)[threadIdx.x] =
+        reinterpret_cast<const double*>(&scratch[0])[threadIdx.x];
 }
 
 }  // namespace perm
