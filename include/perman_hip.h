@@ -233,6 +233,17 @@ int perm_batch_add_dense_many(perm_batch* batch, int count, const int* dtypes,
                               const void* const* mats, const int* novs, int threads,
                               int* first_index);
 int perm_batch_size(const perm_batch* batch);
+/* One batch end to end: adds `count` dense matrices to an EMPTY batch and
+ * runs it, pipelined -- the matrices are prepared on `threads` host threads
+ * in decreasing order n, and the kernels of the largest orders (about 40 %
+ * of the work) run while the rest is prepared and uploaded.  results: one
+ * per matrix in the caller's order, the same values as perm_batch_add_dense_many
+ * + perm_batch_run (a matrix's items depend only on its n); the batch is then
+ * built, and later perm_batch_run calls replay it (results still in the
+ * caller's order). */
+int perm_batch_compute_dense(perm_batch* batch, int count, const int* dtypes,
+                             const void* const* mats, const int* novs, int threads,
+                             perm_result* results, double* device_ms);
 /* results: one per matrix, in insertion order; device_ms: graph time plus
  * the double-double passes, if any. */
 int perm_batch_run(perm_batch* batch, perm_result* results, double* device_ms);

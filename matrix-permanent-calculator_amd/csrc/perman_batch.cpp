@@ -80,6 +80,9 @@ struct perm_batch {
   // sampled verification: copies of the chosen items and their records
   void* vmem = nullptr;
   size_t vbytes = 0;
+  // caller[m]: the caller's index of entry m, when entries are not in the
+  // caller's order (perm_batch_compute_dense); empty: the same order
+  std::vector<uint32_t> caller;
 };
 
 namespace {
@@ -99,11 +102,13 @@ void release_device(perm_batch* B) {
 }
 
 
-int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
+// Fill entry e from the prepared matrix and append its column store and
+// start vector to B->data.
+int fill_entry(perm_batch* B, Entry& e, int n, perm::Prepared& p) {
   if (n > perm::kMaxBatchN)
     return perm::fail(PERM_ERR_SIZE, "batch matrices are limited to n <= " +
                                          std::to_string(perm::kMaxBatchN));
-  Entry e;
+  e = Entry{};
   e.n = n;
   e.integral = p.integral;
   e.zero = p.zero;
@@ -119,6 +124,15 @@ int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
   e.x0_off = B->data.size();
   B->data.insert(B->data.end(), p.x0.begin(), p.x0.end());
   while (B->data.size() % 32) B->data.push_back(0.0);  // 256-B alignment of the next store
+  return PERM_OK;
+}
+
+int add_prepared(perm_batch* B, int n, perm::Prepared& p, int* index) {
+  Entry e;
+  if (int st = fill_entry(B, e, n, p)) return st;
+  // (after perm_batch_compute_dense the entries are in processing order;
+  // a matrix added later keeps the caller's insertion index)
+  if (!B->caller.empty()) B->caller.push_back((uint32_t)B->ents.size());
   B->ents.push_back(e);
   B->built = false;
   if (index) *index = (int)B->ents.size() - 1;
@@ -188,15 +202,12 @@ int capture(perm_batch* B) {
 }
 
 // Plan items and upload.
-int build(perm_batch* B) {
-  HIP_TRYB(hipSetDevice(B->device));
-  release_device(B);
-  const size_t nm = B->ents.size();
-  if (nm == 0) return perm::fail(PERM_ERR_ARG, "empty batch");
-  // items, grouped by variant, groups ordered by decreasing work
+// Append the items of entries `idx` to B->items, grouped by variant, the
+// groups in decreasing work, each group a contiguous item range.
+void plan_items(perm_batch* B, const std::vector<uint32_t>& idx) {
   std::vector<std::vector<uint32_t>> by_var;
   std::vector<int> var_ids;
-  for (size_t m = 0; m < nm; ++m) {
+  for (uint32_t m : idx) {
     Entry& e = B->ents[m];
     perm::batch_split(e.n, B->item_log2, e.walk_bits, e.high_bits);
     e.first_item = 0;
@@ -208,10 +219,8 @@ int build(perm_batch* B) {
       by_var.emplace_back();
       it = var_ids.end() - 1;
     }
-    by_var[it - var_ids.begin()].push_back((uint32_t)m);
+    by_var[it - var_ids.begin()].push_back(m);
   }
-  B->items.clear();
-  B->groups.clear();
   std::vector<size_t> order(var_ids.size());
   std::vector<double> work(var_ids.size(), 0.0);
   for (size_t v = 0; v < var_ids.size(); ++v) {
@@ -235,15 +244,19 @@ int build(perm_batch* B) {
     g.end = (uint32_t)B->items.size();
     B->groups.push_back(g);
   }
-  if (B->items.size() > 0xffffffffull) return perm::fail(PERM_ERR_SIZE, "too many items");
-  // device memory
-  const size_t data_b = align256(B->data.size() * sizeof(double));
+}
+
+// Device memory for `data` doubles, nm matrices, `items` items and `groups`
+// variant groups: data, matrices, items, item records, results, counters
+// (one per group, then one per matrix for double-double passes), and the
+// pinned result copy.
+int alloc_device(perm_batch* B, size_t data, size_t nm, size_t items, size_t groups) {
+  const size_t data_b = align256(data * sizeof(double));
   const size_t mats_b = align256(nm * sizeof(BatchMatrix));
-  const size_t items_b = align256(B->items.size() * sizeof(BatchItem));
-  const size_t part_b = align256(B->items.size() * sizeof(Partial));
+  const size_t items_b = align256(items * sizeof(BatchItem));
+  const size_t part_b = align256(items * sizeof(Partial));
   const size_t res_b = align256(nm * sizeof(Partial));
-  // one counter per variant group, then one per matrix for double-double passes
-  const size_t ctr_b = align256((B->groups.size() + nm) * sizeof(uint32_t));
+  const size_t ctr_b = align256((groups + nm) * sizeof(uint32_t));
   B->dbytes = data_b + mats_b + items_b + part_b + res_b + ctr_b;
   HIP_TRYB(hipMalloc(&B->dmem, B->dbytes));
   char* p = (char*)B->dmem;
@@ -254,12 +267,30 @@ int build(perm_batch* B) {
   B->d_results = (Partial*)(p + data_b + mats_b + items_b + part_b);
   B->d_counters = (uint32_t*)(p + data_b + mats_b + items_b + part_b + res_b);
   HIP_TRYB(hipHostMalloc((void**)&B->h_results, nm * sizeof(Partial), hipHostMallocDefault));
-  std::vector<BatchMatrix> mats(nm);
-  for (size_t m = 0; m < nm; ++m) {
-    const Entry& e = B->ents[m];
-    mats[m] = {B->d_data + e.col_off, B->d_data + e.x0_off, 1ull << e.high_bits,
-               e.walk_bits, e.high_bits, e.variant, e.first_item, e.item_count, 0u};
+  return PERM_OK;
+}
+
+BatchMatrix device_matrix(const perm_batch* B, const Entry& e) {
+  return {B->d_data + e.col_off, B->d_data + e.x0_off, 1ull << e.high_bits, e.walk_bits,
+          e.high_bits, e.variant, e.first_item, e.item_count, 0u};
+}
+
+int build(perm_batch* B) {
+  HIP_TRYB(hipSetDevice(B->device));
+  release_device(B);
+  const size_t nm = B->ents.size();
+  if (nm == 0) return perm::fail(PERM_ERR_ARG, "empty batch");
+  B->items.clear();
+  B->groups.clear();
+  {
+    std::vector<uint32_t> all(nm);
+    for (size_t m = 0; m < nm; ++m) all[m] = (uint32_t)m;
+    plan_items(B, all);
   }
+  if (B->items.size() > 0xffffffffull) return perm::fail(PERM_ERR_SIZE, "too many items");
+  if (int st = alloc_device(B, B->data.size(), nm, B->items.size(), B->groups.size())) return st;
+  std::vector<BatchMatrix> mats(nm);
+  for (size_t m = 0; m < nm; ++m) mats[m] = device_matrix(B, B->ents[m]);
   hipStream_t s0 = B->s[0];
   HIP_TRYB(hipMemcpyAsync(B->d_data, B->data.data(), B->data.size() * sizeof(double),
                           hipMemcpyHostToDevice, s0));
@@ -404,6 +435,67 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     o.certified = perm::is_certified(o) ? 1 : 0;
   }
   return PERM_OK;
+}
+
+// Results of the main run (ms: its device time) in entry order, the
+// escalation rounds, then the results in the caller's order.
+int finish_run(perm_batch* B, perm_result* out, float ms, double* device_ms) {
+  const size_t nm = B->ents.size();
+  std::vector<perm_result> res(nm);
+  perm_result* results = res.data();
+  for (size_t m = 0; m < nm; ++m) {
+    const Entry& e = B->ents[m];
+    if (e.zero) {
+      perm::fill_zero_result(e.n, e.integral, results[m]);
+      continue;
+    }
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 0,
+                           B->h_results[m], results[m]);
+    results[m].kernel_ms = ms;
+  }
+  double total_ms = ms;
+  // Results that miss the accuracy target (perm::escalation_mode) get a pass
+  // in the more accurate mode over the same items: double-double products
+  // for integer matrices (same walk shape), double-double X for the others
+  // (plain walk); one persistent kernel per matrix on the batch's streams,
+  // then the per-matrix reduction again.  At most three rounds: a sampled
+  // measurement, then passes whose mode only increases.
+  // A non-integer matrix of many items first has its FP64 error measured on
+  // a sample of its items (verify_batch); it gets the full pass only if that
+  // does not certify it.
+  std::vector<char> sampled(nm, 0);
+  for (int round = 0; round < 3; ++round) {
+    std::vector<std::pair<size_t, int>> esc;
+    std::vector<size_t> sample;
+    for (size_t m = 0; m < nm; ++m) {
+      const Entry& e = B->ents[m];
+      if (e.zero) continue;
+      const int md = perm::escalation_mode(results[m]);
+      if (md < 0) continue;
+      if (md == 2 && results[m].mode == 0 && !sampled[m] && e.item_count >= perm::kVerifyMinUnits) {
+        sample.push_back(m);
+        sampled[m] = 1;
+      } else {
+        esc.push_back({m, md});
+      }
+    }
+    if (esc.empty() && sample.empty()) break;
+    float pass_ms = 0.f;
+    if (!sample.empty()) {
+      if (int s = verify_batch(B, sample, results, pass_ms)) return s;
+      total_ms += pass_ms;
+    }
+    if (!esc.empty()) {
+      if (int s = escalate(B, esc, results, pass_ms)) return s;
+      total_ms += pass_ms;
+    }
+  }
+  if (device_ms) *device_ms = total_ms;
+  for (size_t m = 0; m < nm; ++m) {
+    results[m].kernel_ms = total_ms;
+    out[B->caller.empty() ? m : B->caller[m]] = results[m];
+  }
+  return perm::ok();
 }
 
 }  // namespace
@@ -557,56 +649,150 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
   HIP_TRYB(hipStreamSynchronize(B->s[0]));
   float ms = 0.f;
   HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
-  for (size_t m = 0; m < B->ents.size(); ++m) {
-    const Entry& e = B->ents[m];
-    if (e.zero) {
-      perm::fill_zero_result(e.n, e.integral, results[m]);
-      continue;
-    }
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, e.shape, 0,
-                           B->h_results[m], results[m]);
-    results[m].kernel_ms = ms;
+  return finish_run(B, results, ms, device_ms);
+}
+
+int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
+                             const void* const* mats, const int* novs, int threads,
+                             perm_result* results, double* device_ms) {
+  if (!B || count <= 0 || !dtypes || !mats || !novs || !results)
+    return perm::fail(PERM_ERR_ARG, "null pointer or empty input");
+  if (!B->ents.empty()) return perm::fail(PERM_ERR_ARG, "the batch must be empty");
+  for (int i = 0; i < count; ++i)
+    if (novs[i] < 1 || novs[i] > perm::kMaxBatchN)
+      return perm::fail(PERM_ERR_SIZE, "matrix " + std::to_string(i) +
+                                           ": batch matrices are limited to 1 <= n <= " +
+                                           std::to_string(perm::kMaxBatchN));
+  HIP_TRYB(hipSetDevice(B->device));
+  release_device(B);
+  threads = std::max(1, std::min(threads, count));
+  // processing order: decreasing n; phase 0 = the largest orders until 40 %
+  // of the work (whole orders), phase 1 the rest
+  std::vector<uint32_t> order((size_t)count);
+  for (int i = 0; i < count; ++i) order[(size_t)i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return novs[a] > novs[b]; });
+  auto work_of = [&](uint32_t i) { return std::ldexp((double)novs[i], novs[i] - 1); };
+  double total = 0.0;
+  for (int i = 0; i < count; ++i) total += work_of((uint32_t)i);
+  size_t split = 0;
+  for (double acc = 0.0; split < (size_t)count && acc < 0.4 * total; ++split)
+    acc += work_of(order[split]);
+  while (split < (size_t)count && novs[order[split]] == novs[order[split - 1]]) ++split;
+  // device memory from the orders alone (upper bounds: column store, start
+  // vector and padding per matrix; every matrix's items)
+  size_t data_ub = 0, items_ub = 0;
+  for (int i = 0; i < count; ++i) {
+    const size_t n = (size_t)novs[i];
+    data_ub += (2 * n - 1) * n + 2 * n + 32;
+    int w = 0, h = 0;
+    perm::batch_split(novs[i], B->item_log2, w, h);
+    items_ub += ((1ull << h) + perm::kItemLanes - 1) / perm::kItemLanes;
   }
-  double total_ms = ms;
-  // Results that miss the accuracy target (perm::escalation_mode) get a pass
-  // in the more accurate mode over the same items: double-double products
-  // for integer matrices (same walk shape), double-double X for the others
-  // (plain walk); one persistent kernel per matrix on the batch's streams,
-  // then the per-matrix reduction again.  At most three rounds: a sampled
-  // measurement, then passes whose mode only increases.
-  // A non-integer matrix of many items first has its FP64 error measured on
-  // a sample of its items (verify_batch); it gets the full pass only if that
-  // does not certify it.
-  std::vector<char> sampled(B->ents.size(), 0);
-  for (int round = 0; round < 3; ++round) {
-    std::vector<std::pair<size_t, int>> esc;
-    std::vector<size_t> sample;
-    for (size_t m = 0; m < B->ents.size(); ++m) {
-      const Entry& e = B->ents[m];
-      if (e.zero) continue;
-      const int md = perm::escalation_mode(results[m]);
-      if (md < 0) continue;
-      if (md == 2 && results[m].mode == 0 && !sampled[m] && e.item_count >= perm::kVerifyMinUnits) {
-        sample.push_back(m);
-        sampled[m] = 1;
-      } else {
-        esc.push_back({m, md});
-      }
+  if (items_ub > 0xffffffffull) return perm::fail(PERM_ERR_SIZE, "too many items");
+  B->ents.assign((size_t)count, Entry{});
+  B->caller.assign(order.begin(), order.end());
+  B->data.clear();
+  B->data.reserve(data_ub);  // no reallocation: the phase uploads read from it
+  B->items.clear();
+  B->items.reserve(items_ub);
+  B->groups.clear();
+  auto undo = [&](int st) {
+    (void)hipDeviceSynchronize();
+    release_device(B);
+    B->ents.clear();
+    B->caller.clear();
+    B->data.clear();
+    B->items.clear();
+    B->groups.clear();
+    return st;
+  };
+  if (int st = alloc_device(B, data_ub, (size_t)count, items_ub, 2 * (size_t)count))
+    return undo(st);
+  hipStream_t s0 = B->s[0];
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  // the variant kernels go to the streams other than s0 (s0 carries the
+  // uploads, so the second phase's copy does not wait behind the first
+  // phase's kernels), or to s0 alone with one stream
+  const int k0 = ns > 1 ? 1 : 0;
+  std::vector<double> stream_work((size_t)ns, 0.0);
+  HIP_TRYB(hipEventRecord(B->ev0, s0));
+  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)(2 * count), s0));
+  std::vector<perm::Prepared> prep((size_t)count);
+  std::vector<int> status((size_t)count, PERM_OK);
+  std::vector<std::string> msg((size_t)count);
+  std::vector<BatchMatrix> dm((size_t)count);
+  const size_t phases[3] = {0, split, (size_t)count};
+  for (int ph = 0; ph < 2; ++ph) {
+    const size_t p0 = phases[ph], p1 = phases[ph + 1];
+    if (p0 == p1) continue;
+    // prepare the phase's matrices on `threads` host threads (the first
+    // phase's kernels run meanwhile)
+    {
+      const int t_n = std::max(1, std::min<int>(threads, (int)(p1 - p0)));
+      std::vector<std::thread> pool;
+      for (int t = 0; t < t_n; ++t)
+        pool.emplace_back([&, t] {
+          for (size_t k = p0 + (size_t)t; k < p1; k += (size_t)t_n) {
+            const uint32_t i = order[k];
+            status[k] = perm::prepare_dense(dtypes[i], mats[i], novs[i], prep[k], perm::kMenuBatch);
+            if (status[k] != PERM_OK) msg[k] = perm::last_error();
+          }
+        });
+      for (auto& th : pool) th.join();
     }
-    if (esc.empty() && sample.empty()) break;
-    float pass_ms = 0.f;
-    if (!sample.empty()) {
-      if (int s = verify_batch(B, sample, results, pass_ms)) return s;
-      total_ms += pass_ms;
+    for (size_t k = p0; k < p1; ++k)
+      if (status[k] != PERM_OK)
+        return undo(perm::fail(status[k], "matrix " + std::to_string(order[k]) + ": " + msg[k]));
+    const size_t d0 = B->data.size(), i0 = B->items.size(), g0 = B->groups.size();
+    std::vector<uint32_t> idx;
+    for (size_t k = p0; k < p1; ++k) {
+      if (int st = fill_entry(B, B->ents[k], novs[order[k]], prep[k])) return undo(st);
+      prep[k] = perm::Prepared();
+      idx.push_back((uint32_t)k);
     }
-    if (!esc.empty()) {
-      if (int s = escalate(B, esc, results, pass_ms)) return s;
-      total_ms += pass_ms;
+    if (B->data.size() > data_ub) return undo(perm::fail(PERM_ERR_SIZE, "batch data bound"));
+    plan_items(B, idx);
+    for (size_t k = p0; k < p1; ++k) dm[k] = device_matrix(B, B->ents[k]);
+    // this phase's data, matrices and items, then its variant kernels
+    HIP_TRYB(hipMemcpyAsync(B->d_data + d0, B->data.data() + d0,
+                            (B->data.size() - d0) * sizeof(double), hipMemcpyHostToDevice, s0));
+    HIP_TRYB(hipMemcpyAsync(B->d_mats + p0, dm.data() + p0, (p1 - p0) * sizeof(BatchMatrix),
+                            hipMemcpyHostToDevice, s0));
+    if (B->items.size() > i0)
+      HIP_TRYB(hipMemcpyAsync(B->d_items + i0, B->items.data() + i0,
+                              (B->items.size() - i0) * sizeof(BatchItem), hipMemcpyHostToDevice,
+                              s0));
+    HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+    for (int i = k0; i < ns; ++i)
+      if (i != 0) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+    for (size_t gi = g0; gi < B->groups.size(); ++gi) {
+      const auto& G = B->groups[gi];
+      const uint32_t waves = G.end - G.begin;
+      const uint32_t grid = std::max(1u, std::min<uint32_t>((waves + 3) / 4, B->max_grid));
+      const int si = (int)(std::min_element(stream_work.begin() + k0, stream_work.end()) -
+                           stream_work.begin());
+      stream_work[(size_t)si] += G.work;
+      HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
+                                          B->d_counters + gi, B->d_partials, grid, B->s[si]));
     }
   }
-  if (device_ms) *device_ms = total_ms;
-  for (size_t m = 0; m < B->ents.size(); ++m) results[m].kernel_ms = total_ms;
-  return perm::ok();
+  for (int i = 1; i < ns; ++i) {
+    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+  }
+  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, (uint32_t)count, s0));
+  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, (size_t)count * sizeof(Partial),
+                          hipMemcpyDeviceToHost, s0));
+  HIP_TRYB(hipEventRecord(B->ev1, s0));
+  HIP_TRYB(hipStreamSynchronize(s0));
+  float ms = 0.f;
+  HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+  // the batch is built: later runs replay it (a captured graph from the
+  // second perm_batch_run on), results in the caller's order
+  B->built = true;
+  B->ran = true;
+  return finish_run(B, results, ms, device_ms);
 }
 
 // Autotuner of the batch's work split: item size (Gray-code chunk length per

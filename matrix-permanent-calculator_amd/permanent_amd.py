@@ -168,6 +168,10 @@ def lib():
         _bind(L, "perm_batch_add_dense_many", [vp, ctypes.c_int, ip,
                                                ctypes.POINTER(ctypes.c_void_p), ip,
                                                ctypes.c_int, ip])
+        _bind(L, "perm_batch_compute_dense", [vp, ctypes.c_int, ip,
+                                              ctypes.POINTER(ctypes.c_void_p), ip, ctypes.c_int,
+                                              ctypes.POINTER(PermResultC),
+                                              ctypes.POINTER(ctypes.c_double)])
         L.perm_batch_size.argtypes = [vp]
         L.perm_batch_run.argtypes = [vp, ctypes.POINTER(PermResultC),
                                      ctypes.POINTER(ctypes.c_double)]
@@ -420,6 +424,16 @@ class Batch:
                                                inputs.ptrs, _ip(inputs.ns), threads,
                                                ctypes.byref(first)))
         return first.value
+
+    def compute_prepared(self, inputs: BatchInputs, results, threads: int = 8) -> float:
+        """perm_batch_compute_dense on an empty batch: prepare, upload and
+        run pipelined; results (a ctypes array of PermResultC, caller's
+        order) are filled; returns the device time in ms."""
+        ms = ctypes.c_double()
+        _check(lib().perm_batch_compute_dense(self._h, inputs.count, _ip(inputs.dts),
+                                              inputs.ptrs, _ip(inputs.ns), threads, results,
+                                              ctypes.byref(ms)))
+        return ms.value
 
     def set(self, item_log2: int = 0, streams: int = 0, max_grid: int = 0):
         _check(lib().perm_batch_set(self._h, item_log2, streams, max_grid))

@@ -64,6 +64,39 @@ def test_same_alone_and_in_the_mix():
     assert [r.value for r in res1] == [r.value for r in res2]
 
 
+def test_pipelined_compute_matches_add_and_run():
+    """perm_batch_compute_dense (preparation of the smaller orders overlapped
+    with the kernels of the largest) gives bitwise the results of
+    add_dense_many + run, in the caller's order, zero-pattern matrices
+    included; later runs of the same batch (direct, then captured and
+    replayed) return them in the caller's order too."""
+    import ctypes
+
+    mats = P.mixed_batch(300, 8, 26, 0.2, 0.8, seed=21)
+    z = P.synthetic_matrix(15, 0.6, seed=5).copy()
+    z[:, 3] = 0.0  # a zero column: no perfect matching, exactly 0
+    mats.insert(17, z)
+    inputs = P.BatchInputs(mats)
+    ref = P.Batch()
+    ref.add_prepared(inputs, threads=4)
+    want, _ = ref.run()
+    ref.close()
+    b = P.Batch()
+    got_c = (P.PermResultC * len(mats))()
+    ms = b.compute_prepared(inputs, got_c, threads=4)
+    assert ms > 0
+    got = [P.PermResult.from_c(r) for r in got_c]
+    again1, _ = b.run()
+    again2, _ = b.run()  # graph capture, then a replay
+    again3, _ = b.run()
+    b.close()
+    assert got[17].value == 0.0
+    for res in (got, again1, again2, again3):
+        for r, w in zip(res, want):
+            assert (r.value, r.sum_hi, r.sum_lo, r.abs_sum, r.mode) == (
+                w.value, w.sum_hi, w.sum_lo, w.abs_sum, w.mode)
+
+
 def test_batch_sparse_entries_follow_reference_semantics():
     a = P.synthetic_matrix(14, 0.6, seed=9) - 0.25 * (np.arange(14)[:, None] % 4 == 0)
     b = P.Batch()
