@@ -357,10 +357,14 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
                  float& pass_ms) {
   pass_ms = 0.f;
   hipStream_t s0 = B->s[0];
-  const size_t n_items = B->items.size();
-  std::vector<Partial> fp(n_items);
-  HIP_TRYB(hipMemcpyAsync(fp.data(), B->d_partials, n_items * sizeof(Partial),
-                          hipMemcpyDeviceToHost, s0));
+  // the FP64 item records of the sampled matrices only (not the batch's)
+  std::vector<std::vector<Partial>> fp(ms.size());
+  for (size_t k = 0; k < ms.size(); ++k) {
+    const Entry& e = B->ents[ms[k]];
+    fp[k].resize(e.item_count);
+    HIP_TRYB(hipMemcpyAsync(fp[k].data(), B->d_partials + e.first_item,
+                            e.item_count * sizeof(Partial), hipMemcpyDeviceToHost, s0));
+  }
   HIP_TRYB(hipStreamSynchronize(s0));
   struct Pick {
     std::vector<uint32_t> ids;
@@ -373,7 +377,7 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     const Entry& e = B->ents[ms[k]];
     std::vector<double> z(e.item_count);
     for (uint32_t i = 0; i < e.item_count; ++i) {
-      const Partial& p = fp[e.first_item + i];
+      const Partial& p = fp[k][i];
       z[i] = perm::unit_weight(p.sq_sum, p.sq_run, e.walk_bits, e.high_bits);
     }
     perm::pps_sample(z, perm::kVerifyUnits, picks[k].ids, picks[k].weight, picks[k].certain);
@@ -426,7 +430,7 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     const Pick& pk = picks[k];
     std::vector<double> err(pk.ids.size());
     for (size_t j = 0; j < pk.ids.size(); ++j) {
-      const Partial& f = fp[e.first_item + pk.ids[j]];
+      const Partial& f = fp[k][pk.ids[j]];
       const Partial& r = ref[pk.begin + j];
       err[j] = (f.hi - r.hi) + (f.lo - r.lo);
     }
