@@ -132,8 +132,11 @@ struct Pass {
   uint64_t seq = 0;      // publication sequence
   bool first = false;    // the request's first pass (its copy carries the matrix)
   int slot = 0;          // header slot in the request's block
-  size_t up_off = 0;     // arena offset and size of the range the kernel copies
+  size_t up_off = 0;     // arena offset and size of the mirror range the kernel copies
   size_t up_bytes = 0;
+  uint64_t up2_src = 0;  // a second range (the first pass's matrix): source address,
+  size_t up2_off = 0;    // arena offset and size (0: none)
+  size_t up2_bytes = 0;
 };
 
 struct Request {
@@ -149,7 +152,11 @@ struct Request {
   bool integral = false;
   double product_bits = 1e9;
   double cost = 0.0;     // FP64 lane-instructions of the FP64 pass (admission)
-  std::vector<double> cs, x0;  // padded column store and start vector (until staged)
+  // padded column store then start vector (2N doubles), written by the prep
+  // thread into pinned staging memory that the kernel copies from
+  char* staged = nullptr;
+  uint64_t staged_dev = 0;  // its device address
+  size_t staged_bytes = 0;  // a multiple of 256
   uint64_t vblock = 0;   // virtual arena position of the request's block
   size_t block_bytes = 0;
   size_t block = 0;      // block index in allocation order
@@ -189,6 +196,63 @@ struct Block {
   bool live = true;
 };
 
+// Pinned, coherent host memory for prepared matrices, read by the kernel's
+// copying waves: blocks of one size per served order (free lists by size),
+// cut from 8 MiB slabs allocated on demand and kept until the server ends.
+struct StagingPool {
+  static constexpr size_t kSlab = 8u << 20;
+  struct Slab {
+    char* host;
+    uint64_t dev;
+  };
+  std::mutex mu;
+  std::vector<Slab> slabs;
+  std::unordered_map<size_t, std::vector<char*>> free;
+  size_t used = 0;  // bytes handed out of the current slab
+
+  // A block of `bytes` (<= kSlab): host pointer, device address; nullptr on a
+  // HIP failure.
+  char* get(size_t bytes, uint64_t& dev) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto& fl = free[bytes];
+    char* p = nullptr;
+    if (!fl.empty()) {
+      p = fl.back();
+      fl.pop_back();
+    } else {
+      if (slabs.empty() || used + bytes > kSlab) {
+        Slab sl{nullptr, 0};
+        if (hipHostMalloc((void**)&sl.host, kSlab, hipHostMallocCoherent) != hipSuccess)
+          return nullptr;
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, sl.host, 0) != hipSuccess) {
+          (void)hipHostFree(sl.host);
+          return nullptr;
+        }
+        sl.dev = (uint64_t)(uintptr_t)d;
+        slabs.push_back(sl);
+        used = 0;
+      }
+      p = slabs.back().host + used;
+      used += bytes;
+    }
+    dev = 0;
+    for (const Slab& sl : slabs)
+      if (p >= sl.host && p < sl.host + kSlab) dev = sl.dev + (uint64_t)(p - sl.host);
+    return p;
+  }
+  void put(char* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu);
+    free[bytes].push_back(p);
+  }
+  void release() {
+    for (const Slab& sl : slabs) (void)hipHostFree(sl.host);
+    slabs.clear();
+    free.clear();
+  }
+};
+
 }  // namespace
 
 struct perm_server {
@@ -206,6 +270,8 @@ struct perm_server {
   std::deque<Request*> pending;                  // prepared, not admitted (mu_q)
   std::atomic<int> pending_n{0};                 // pending.size(), read without mu_q
   std::atomic<bool> stopping{false};             // set under mu and mu_q
+  int prep_ahead = 0;  // prep threads wait while this many requests are prepared and pending
+  StagingPool staging;
   std::vector<std::thread> preppers;
   size_t preppers_n = 0;
   int prepping = 0;                              // requests taken by a prep thread (mu_q)
@@ -474,8 +540,7 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
     return nullptr;
   }
   if (first) {
-    r->cs_doubles = r->cs.size();
-    r->lay.data_bytes = align256((r->cs.size() + r->x0.size()) * sizeof(double));
+    r->lay.data_bytes = r->staged_bytes;
     const size_t bytes = r->lay.records() + align256((size_t)r->walk_items * sizeof(Partial));
     uint64_t v = 0;
     if ((st = arena_alloc(S, bytes, v)) != PERM_OK || v == UINT64_MAX) return nullptr;
@@ -483,12 +548,6 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
     r->block_bytes = bytes;
     r->block = S->block_base + S->blocks.size() - 1;
     r->resident = true;
-    char* h = S->h_arena + (size_t)(v % S->arena_bytes);
-    std::memcpy(h + r->lay.data(), r->cs.data(), r->cs.size() * sizeof(double));
-    std::memcpy(h + r->lay.data() + r->cs.size() * sizeof(double), r->x0.data(),
-                r->x0.size() * sizeof(double));
-    std::vector<double>().swap(r->cs);
-    std::vector<double>().swap(r->x0);
   }
   const size_t off = (size_t)(r->vblock % S->arena_bytes);
   char* h = S->h_arena + off;
@@ -507,7 +566,12 @@ Pass* make_pass(perm_server* S, Request* r, int kind, int& st) {
   p->first = first;
   p->slot = slot;
   p->up_off = off + L.slot(slot);
-  p->up_bytes = first ? kHeaderBytes + L.data_bytes : kHeaderBytes + ids_bytes;
+  p->up_bytes = kHeaderBytes + ids_bytes;  // the header slot (and the ids)
+  if (first) {  // and the matrix, from its staging block
+    p->up2_src = r->staged_dev;
+    p->up2_off = off + L.data();
+    p->up2_bytes = L.data_bytes;
+  }
   ServeHeader hd;
   std::memset(&hd, 0, sizeof(hd));
   hd.colpm = (const double*)(d + L.data());
@@ -545,6 +609,8 @@ int publish(perm_server* S, bool& progressed) {
     const uint64_t q = seq & (perm::kServeRing - 1);
     S->ctl->ring[q] = (uint64_t)(uintptr_t)(S->d_arena + p->up_off);
     S->ctl->up[q] = ((uint64_t)(p->up_off / 256) << 32) | (uint64_t)(p->up_bytes / 256);
+    S->ctl->up2_src[q] = p->up2_src;
+    S->ctl->up2[q] = ((uint64_t)(p->up2_off / 256) << 32) | (uint64_t)(p->up2_bytes / 256);
     S->by_seq[q] = p;
     S->n_inflight[is_dd(p->kind) ? 1 : 0] += 1;
     bump(S->n_items, p->items);
@@ -661,6 +727,10 @@ int retire(perm_server* S, bool& progressed) {
     SERVE_TRACE("pass %llu (request %llu, kind %d) complete", (unsigned long long)p->seq,
                 (unsigned long long)r->id, p->kind);
     S->queued_cost -= pass_cost(r, p->kind, p->items);
+    if (p->first) {  // the kernel has copied the matrix in: its staging block is free
+      S->staging.put(r->staged, r->staged_bytes);
+      r->staged = nullptr;
+    }
     const int next = finalize(S, r, p, rec, st);
     delete p;
     if (st != PERM_OK) break;
@@ -713,6 +783,7 @@ int admit(perm_server* S, bool& progressed) {
       for (size_t i = 0; i < k; ++i) S->pending.pop_front();
       S->pending_n.store((int)S->pending.size(), std::memory_order_relaxed);
     }
+    S->cv_prep.notify_all();  // prep threads may wait on the look-ahead bound
     progressed = true;
     bump(S->n_admitted, (long long)k);
   }
@@ -898,14 +969,24 @@ int prepare_request(perm_server* S, Request* r) {
   const uint64_t items0 = std::min<uint64_t>(groups64, max_items);
   r->groups = (uint32_t)((groups64 + items0 - 1) / items0);
   r->walk_items = (uint32_t)((groups64 + r->groups - 1) / r->groups);
-  // padded column store: block b of n values -> block b of N values
+  // padded column store (block b of n values -> block b of N values), then
+  // the start vector and its low parts, written straight into a pinned
+  // staging block the kernel copies from
   const size_t blocks_n = (size_t)(2 * (n - 1));
-  r->cs.assign((size_t)(2 * (N - 1) + 1) * N, 0.0);
+  r->cs_doubles = (size_t)(2 * (N - 1) + 1) * N;
+  r->staged_bytes = align256((r->cs_doubles + 2 * (size_t)N) * sizeof(double));
+  r->staged = S->staging.get(r->staged_bytes, r->staged_dev);
+  if (!r->staged) {
+    r->error = "hipHostMalloc of a staging slab failed";
+    return PERM_ERR_HIP;
+  }
+  double* cs = (double*)r->staged;
+  double* x0 = cs + r->cs_doubles;
+  std::memset(r->staged, 0, r->staged_bytes);
   for (size_t b = 0; b < blocks_n; ++b)
-    std::copy(p.cs.begin() + b * n, p.cs.begin() + (b + 1) * n, r->cs.begin() + b * N);
-  r->x0.assign(2 * (size_t)N, 0.0);
-  for (int j = 0; j < N; ++j) r->x0[j] = j < n ? p.x0[j] : 1.0;
-  for (int j = 0; j < n; ++j) r->x0[N + j] = p.x0[n + j];
+    std::memcpy(cs + b * N, p.cs.data() + b * n, (size_t)n * sizeof(double));
+  for (int j = 0; j < N; ++j) x0[j] = j < n ? p.x0[j] : 1.0;
+  for (int j = 0; j < n; ++j) x0[N + j] = p.x0[n + j];
   r->cost = pass_cost(r, 0, r->walk_items);
   r->raw = RawMatrix();
   return PERM_OK;
@@ -922,7 +1003,12 @@ void prep_loop(perm_server* S) {
     got.clear();
     {
       std::unique_lock<std::mutex> lk(S->mu_q);
-      S->cv_prep.wait(lk, [&] { return S->stopping.load() || !S->to_prep.empty(); });
+      // (at most prep_ahead prepared requests wait for admission: their
+      // staging blocks are pinned memory)
+      S->cv_prep.wait(lk, [&] {
+        return S->stopping.load() ||
+               (!S->to_prep.empty() && S->pending_n.load() < S->prep_ahead);
+      });
       if (S->to_prep.empty()) return;  // stopping and drained
       const size_t take = std::min<size_t>(16, std::max<size_t>(1, S->to_prep.size() / threads));
       for (size_t i = 0; i < take; ++i) {
@@ -1080,6 +1166,7 @@ void release(perm_server* S) {
   if (S->d_state) (void)hipFree(S->d_state);
   if (S->d_arena) (void)hipFree(S->d_arena);
   if (S->h_arena) (void)hipHostFree(S->h_arena);
+  S->staging.release();
   if (S->ctl) (void)hipHostFree(S->ctl);
   if (S->done) (void)hipHostFree(S->done);
   for (auto& kv : S->table) delete kv.second;
@@ -1172,6 +1259,7 @@ int perm_server_create(const perm_server_config* cfg, perm_server** out) {
   }
   S->worker = std::thread(scheduler_main, S);
   S->preppers_n = (size_t)c.prep_threads;
+  S->prep_ahead = std::max(1024, 2 * c.max_resident);
   for (int t = 0; t < c.prep_threads; ++t) S->preppers.emplace_back(prep_loop, S);
   *out = S;
   return perm::ok();

@@ -178,15 +178,14 @@ __device__ void finish_pass(const ServeHeader* H, ServeState* st, ServeDone* don
   }
 }
 
-// Copy the ranges of m <= 64 new passes (lane i < m holds pass i's range
-// descriptor `up`) from the host mirror into the arena, as one flat run of
-// 8-byte words spread over the lanes: kCopyDepth loads in flight per lane,
-// so the copy of a batch of small passes takes a few PCIe round trips, not
-// two per pass.  pre/off: this wave's LDS scratch.
+// Copy up to 64 ranges (lane i: `words` 8-byte words from `src` to `dst`;
+// 0 words for none) as one flat run of words spread over the lanes:
+// kCopyDepth loads in flight per lane, so the copy of a batch of small
+// passes takes a few PCIe round trips, not two per pass.  pre/from/to: this
+// wave's LDS scratch.
 constexpr uint32_t kCopyDepth = 12;
-__device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint32_t m,
-                           uint32_t lane, uint32_t* pre, uint64_t* off) {
-  const uint32_t words = lane < m ? (uint32_t)(up & 0xffffffffu) * 32u : 0u;  // 256 B = 32 words
+__device__ void copy_ranges(const char* src, char* dst, uint32_t words, uint32_t lane,
+                            uint32_t* pre, uint64_t* from, uint64_t* to) {
   uint32_t incl = words;  // inclusive prefix sum over the lanes
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
@@ -194,12 +193,13 @@ __device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint
     if ((int)lane >= d) incl += y;
   }
   pre[lane] = incl - words;
-  off[lane] = (up >> 32) * 256;
+  from[lane] = (uint64_t)(uintptr_t)src;
+  to[lane] = (uint64_t)(uintptr_t)dst;
   if (lane == kWave - 1) pre[kWave] = incl;
   __builtin_amdgcn_wave_barrier();
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  const uint32_t total = pre[m];  // (m <= 64; pre[m] = total for every m)
-  uint32_t q = 0;                 // this lane's pass
+  const uint32_t total = pre[kWave];
+  uint32_t q = 0;  // this lane's range
   for (uint32_t w0 = 0; w0 < total; w0 += kWave * kCopyDepth) {
     uint64_t v[kCopyDepth], a[kCopyDepth];
 #pragma unroll
@@ -209,13 +209,14 @@ __device__ void copy_batch(const char* h_arena, char* d_arena, uint64_t up, uint
       v[j] = 0;
       if (w < total) {
         while (pre[q + 1] <= w) ++q;
-        a[j] = off[q] + (uint64_t)(w - pre[q]) * 8;
-        v[j] = load_sys((const uint64_t*)(h_arena + a[j]));
+        const uint64_t k = (uint64_t)(w - pre[q]) * 8;
+        a[j] = to[q] + k;
+        v[j] = load_sys((const uint64_t*)(uintptr_t)(from[q] + k));
       }
     }
 #pragma unroll
     for (uint32_t j = 0; j < kCopyDepth; ++j)
-      if (w0 + j * kWave + lane < total) store_dev((uint64_t*)(d_arena + a[j]), v[j]);
+      if (w0 + j * kWave + lane < total) store_dev((uint64_t*)(uintptr_t)a[j], v[j]);
   }
 }
 
@@ -294,7 +295,8 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
   // cache holds few walk loops (several per CU thrash it).
   __shared__ unsigned long long wg_pass;
   __shared__ uint32_t copy_pre[256 / kWave][kWave + 1];  // each wave's copy plan
-  __shared__ uint64_t copy_off[256 / kWave][kWave];
+  __shared__ uint64_t copy_from[256 / kWave][kWave];
+  __shared__ uint64_t copy_to[256 / kWave][kWave];
   if (threadIdx.x == 0) wg_pass = 0ull;
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -457,11 +459,19 @@ void serve_kernel(ServeControl* ctl, ServeState* st, ServeDone* done, const char
         if (uni64(old) == c) {
           // (the host wrote the ring entries before the tail this wave or
           // another read: they are read after it)
-          const uint32_t m = (uint32_t)n;
-          const uint64_t q = (c + lane) & (kServeRing - 1);
-          const uint64_t up = lane < m ? load_sys(&ctl->up[q]) : 0ull;
+          // lane i < m: pass c+i's mirror range; lane m+i: its second range
+          // (the first pass's matrix, from its pinned staging block)
+          const uint32_t m = (uint32_t)n;  // <= kCopyBatch = 16
+          const uint32_t pi = lane < m ? lane : lane - m;
+          const bool mine = lane < 2 * m;
+          const uint64_t q = (c + pi) & (kServeRing - 1);
           const uint64_t entry = lane < m ? load_sys(&ctl->ring[q]) : 0ull;
-          copy_batch(h_arena, d_arena, up, m, lane, copy_pre[wave], copy_off[wave]);
+          const uint64_t up = mine ? load_sys(lane < m ? &ctl->up[q] : &ctl->up2[q]) : 0ull;
+          const uint64_t src2 = mine && lane >= m ? load_sys(&ctl->up2_src[q]) : 0ull;
+          const uint64_t off = (up >> 32) * 256;
+          const uint32_t words = (lane >= m && src2 == 0) ? 0u : (uint32_t)(up & 0xffffffffu) * 32u;
+          copy_ranges(lane < m ? h_arena + off : (const char*)(uintptr_t)src2, d_arena + off,
+                      mine ? words : 0u, lane, copy_pre[wave], copy_from[wave], copy_to[wave]);
           if (lane < m) store_dev(&st->dring[q], entry);
           wait_mem();  // the copies and entries land before they are marked
           if (lane < m) store_dev(&st->copied[q], c + lane + 1);

@@ -90,7 +90,12 @@ struct ServeControl {
   uint64_t dev_pub;   // the device's publication count after each poll (diagnostics)
   uint64_t pad[5];
   uint64_t ring[kServeRing];  // device address of the header of pass q at q mod kServeRing
-  uint64_t up[kServeRing];    // range to copy for pass q: (offset / 256) << 32 | bytes / 256
+  uint64_t up[kServeRing];    // mirror range of pass q: (offset / 256) << 32 | bytes / 256
+  // a second range of pass q (its matrix, on the first pass): the device
+  // address of its source in pinned memory (0: none), and the arena range
+  // as in `up`
+  uint64_t up2_src[kServeRing];
+  uint64_t up2[kServeRing];
 };
 
 // Device-side state, kept across launches.  Index k: 0 the main kernel, 1
