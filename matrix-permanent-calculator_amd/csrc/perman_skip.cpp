@@ -94,23 +94,57 @@ void prepare_order(int n, const std::vector<double>& x0, const std::vector<doubl
       n, [&](int k, int r) { return cols[(size_t)order[k] * n + rp[r]]; }, sp.cs);
 }
 
-// One run on the device (mode 0 FP64, 1 double-double products).
+// Sum of two records (double-double sum, plain moments), in this order.
+perm::Partial add_records(const perm::Partial& u, const perm::Partial& v) {
+  const double s = u.hi + v.hi;
+  const double bb = s - u.hi;
+  const double err = (u.hi - (s - bb)) + (v.hi - bb);
+  const double lo = err + u.lo + v.lo;
+  const double hi = s + lo;
+  return {hi, lo - (hi - s), u.abs_sum + v.abs_sum, u.sq_sum + v.sq_sum, u.sq_run + v.sq_run, 0.0};
+}
+
+// Walked groups a wave may spend before the rest of its range goes to
+// continuation waves (PERMAN_SKIP_BUDGET; 0: none), from order
+// kSkipContMinN on, and the log2 of the pieces a continuation range is cut
+// into, relative to the walk (PERMAN_SKIP_SPLIT_BITS).  Measured on grid
+// graphs (profiles/ryser_skip_grid_graphs.md): 8x12 (n = 48) 341 -> 233 ms;
+// below n = 44 the second launch and the host's planning between the two
+// cost more than the balance gains (8x10: 4.6 -> 9.3 ms), so the walk runs
+// in one launch there.
+constexpr int kSkipBudget = 64;
+constexpr int kSkipContMinN = 44;
+constexpr int kContSplitBits = 8;
+
+// One run on the device (mode 0 FP64, 1 double-double products).  The
+// skipping walk leaves most waves with little to walk and a few with most
+// of it (on the 8x10 grid graph the busiest wave walks ~30x the mean); so,
+// for large orders, every wave stops after a budget of walked groups and
+// the remaining ranges of the waves that stopped are cut into aligned
+// pieces (2^-8 of the walk at most), each walked by a wave of a second
+// launch (its x in closed form at the piece's first step).  Both launches'
+// records are summed in a fixed order: the result does not depend on
+// scheduling.
 int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Partial& r,
              double& walked_frac, float& ms) {
   const uint64_t lanes = 1ull << sp.high_bits;
   const uint64_t blocks = (lanes + 255) / 256;
+  const uint64_t waves = blocks * 4;
   const size_t cs_b = align256(sp.cs.size() * sizeof(double));
   const size_t x_b = align256((size_t)n * sizeof(double));
   const size_t lev_b = align256((size_t)(n + 3) / 4 * sizeof(uint32_t));
   const size_t part_b = align256(blocks * sizeof(perm::Partial));
-  const size_t walk_b = align256(blocks * 4 * sizeof(uint32_t));
-  const size_t res_b = align256(sizeof(perm::Partial));
+  const size_t walk_b = align256(waves * sizeof(uint32_t));
+  const size_t res_b = align256(2 * sizeof(perm::Partial));
+  const size_t cont_b = align256(waves * sizeof(uint32_t));
   char* d = nullptr;
+  char* d2 = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
     if (d) (void)hipFree(d);
+    if (d2) (void)hipFree(d2);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
@@ -122,7 +156,9 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
   HIP_TRYK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   HIP_TRYK(hipEventCreate(&e0));
   HIP_TRYK(hipEventCreate(&e1));
-  HIP_TRYK(hipMalloc((void**)&d, cs_b + x_b + lev_b + part_b + walk_b + res_b));
+  HIP_TRYK(hipMalloc((void**)&d, cs_b + x_b + lev_b + part_b + walk_b + res_b + cont_b));
+  const int budget_env = perm::env_int("PERMAN_SKIP_BUDGET", n >= kSkipContMinN ? kSkipBudget : 0);
+  const uint32_t budget = (uint32_t)std::max(0, budget_env);
   perm::SkipLaunch L{};
   L.colpm = (const double*)d;
   L.x0 = (const double*)(d + cs_b);
@@ -130,6 +166,7 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
   L.partials = (perm::Partial*)(d + cs_b + x_b + lev_b);
   L.walked = (uint32_t*)(d + cs_b + x_b + lev_b + part_b);
   L.result = (perm::Partial*)(d + cs_b + x_b + lev_b + part_b + walk_b);
+  uint32_t* d_cont = (uint32_t*)(d + cs_b + x_b + lev_b + part_b + walk_b + res_b);
   L.lanes = lanes;
   L.blocks = blocks;
   L.n = n;
@@ -140,6 +177,8 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
   // bit-reversed lane ids: the lanes of a wave differ in the last high
   // columns; measured 1.5-1.9x faster on grid graphs 8x9, 8x10, 6x12
   L.lane_reverse = perm::env_int("PERMAN_SKIP_LANE_REVERSE", 1);
+  L.budget = budget;
+  L.cont = budget ? d_cont : nullptr;
   HIP_TRYK(hipMemcpyAsync((void*)L.colpm, sp.cs.data(), sp.cs.size() * sizeof(double),
                           hipMemcpyHostToDevice, st));
   HIP_TRYK(hipMemcpyAsync((void*)L.x0, sp.x0.data(), n * sizeof(double), hipMemcpyHostToDevice,
@@ -150,20 +189,69 @@ int run_skip(int n, const SkipPrepared& sp, int mode, int check_bits, perm::Part
                           hipMemcpyHostToDevice, st));
   HIP_TRYK(hipEventRecord(e0, st));
   HIP_TRYK(perm::launch_skip(L, st));
-  HIP_TRYK(hipEventRecord(e1, st));
-  std::vector<uint32_t> walked(blocks * 4);
-  HIP_TRYK(hipMemcpyAsync(&r, L.result, sizeof(r), hipMemcpyDeviceToHost, st));
+  HIP_TRYK(hipEventRecord(e1, st));  // (recorded again after the continuations, if any)
+  std::vector<uint32_t> walked(waves);
+  std::vector<uint32_t> cont(budget ? waves : 0);
+  perm::Partial r1{};
+  HIP_TRYK(hipMemcpyAsync(&r1, L.result, sizeof(r1), hipMemcpyDeviceToHost, st));
   HIP_TRYK(hipMemcpyAsync(walked.data(), L.walked, walked.size() * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, st));
+  if (budget)
+    HIP_TRYK(hipMemcpyAsync(cont.data(), d_cont, cont.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
   HIP_TRYK(hipStreamSynchronize(st));
+  r = r1;
+  // the continuations: each stopped wave's remaining range [stop, 2^w) in
+  // aligned pieces of at most 2^(w - kContSplitBits) steps
+  std::vector<uint32_t> work;
+  const int split_bits = perm::env_int("PERMAN_SKIP_SPLIT_BITS", kContSplitBits);
+  const int piece_bits = std::max(check_bits, sp.walk_bits - std::max(0, split_bits));
+  const uint64_t steps = 1ull << sp.walk_bits;
+  for (uint64_t w = 0; w < cont.size(); ++w) {
+    uint64_t t = cont[w];
+    if (t == 0) continue;
+    while (t < steps) {
+      const int b = std::min(__builtin_ctzll(t), piece_bits);
+      work.insert(work.end(), {(uint32_t)w, (uint32_t)t, (uint32_t)b, 0u});
+      t += 1ull << b;
+    }
+  }
+  if (!work.empty()) {
+    const uint64_t n_work = work.size() / 4;
+    const uint64_t blocks2 = (n_work + 3) / 4;
+    const size_t work_b = align256(work.size() * sizeof(uint32_t));
+    const size_t part2_b = align256(blocks2 * sizeof(perm::Partial));
+    const size_t walk2_b = align256(blocks2 * 4 * sizeof(uint32_t));
+    HIP_TRYK(hipMalloc((void**)&d2, work_b + part2_b + walk2_b));
+    perm::SkipLaunch L2 = L;
+    L2.work = (const uint32_t*)d2;
+    L2.n_work = (uint32_t)n_work;
+    L2.blocks = blocks2;
+    L2.partials = (perm::Partial*)(d2 + work_b);
+    L2.walked = (uint32_t*)(d2 + work_b + part2_b);
+    L2.result = L.result + 1;
+    L2.budget = 0;
+    L2.cont = nullptr;
+    HIP_TRYK(hipMemcpyAsync((void*)L2.work, work.data(), work.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, st));
+    HIP_TRYK(perm::launch_skip(L2, st));
+    HIP_TRYK(hipEventRecord(e1, st));  // the time includes the host's planning between launches
+    std::vector<uint32_t> walked2(blocks2 * 4);
+    perm::Partial r2{};
+    HIP_TRYK(hipMemcpyAsync(&r2, L2.result, sizeof(r2), hipMemcpyDeviceToHost, st));
+    HIP_TRYK(hipMemcpyAsync(walked2.data(), L2.walked, walked2.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    HIP_TRYK(hipStreamSynchronize(st));
+    r = add_records(r1, r2);
+    walked.insert(walked.end(), walked2.begin(), walked2.end());
+  }
   HIP_TRYK(hipEventElapsedTime(&ms, e0, e1));
   // walked[] counts groups of 2^R steps per wave (64 lanes); lanes past
   // `lanes` in the last block walk with their wave
   const int R = perm::dense_unroll_bits(n);
   double groups = 0.0;
   for (uint32_t w : walked) groups += w;
-  const double waves = (double)blocks * 4.0;
-  walked_frac = groups * std::ldexp(1.0, R) / (waves * std::ldexp(1.0, sp.walk_bits));
+  walked_frac = groups * std::ldexp(1.0, R) / ((double)waves * std::ldexp(1.0, sp.walk_bits));
   return PERM_OK;
 }
 

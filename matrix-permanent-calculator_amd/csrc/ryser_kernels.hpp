@@ -51,6 +51,15 @@ struct SkipLaunch {
   uint64_t lanes, blocks;
   int n, walk_bits, high_bits, check_bits, mode;  // mode 0 FP64, 1 double-double products
   int lane_reverse = 0;  // 1: lane l fixes the high columns by the bit-reversal of l
+  // Without a work list: every wave of the plain split walks its range and,
+  // with budget > 0, stops after `budget` walked groups of 2^R steps,
+  // cont[wave] = the step it stopped at (0: finished).  With a work list
+  // (n_work entries of 4 words: wave, first step, log2 steps, 0; blocks =
+  // ceil(n_work / 4)): wave j walks entry j to its end.
+  const uint32_t* work = nullptr;
+  uint32_t n_work = 0;
+  uint32_t budget = 0;
+  uint32_t* cont = nullptr;
 };
 // Smallest n with a skipping kernel (below it the plain kernel takes microseconds).
 constexpr int kMinSkipN = 16;

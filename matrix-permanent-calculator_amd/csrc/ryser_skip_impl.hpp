@@ -61,13 +61,20 @@ __device__ __forceinline__ int step_block(uint32_t t) {
   return 2 * c + (int)((t >> (c + 1)) & 1u);
 }
 
+// The walk of one wave over steps [t_begin, t_begin + 2^t_bits) (t_begin a
+// multiple of 2^t_bits), with skips.  budget > 0: the wave stops once it has
+// walked `budget` groups of 2^R steps, at the start of its next check; stop_t
+// is then that step (0 when the range was finished) and the rest of the
+// range is left to later waves (a continuation starts from the closed form
+// of x at its first step).
 template <int N, int R, int MODE>
 __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0,
                                           const uint32_t* __restrict__ row_level,
                                           uint64_t pattern, bool valid, int n_walk_bits,
-                                          int n_high_bits, int check_bits, DD& s, double& a,
-                                          double& q, uint32_t& walked_groups) {
+                                          int n_high_bits, int check_bits, uint32_t t_begin,
+                                          int t_bits, uint32_t budget, DD& s, double& a,
+                                          double& q, uint32_t& walked_groups, uint32_t& stop_t) {
   const ColPtr cols = (ColPtr)colpm;
   // row levels, byte j of word j/4 = level of row j (scalar loads, SGPRs)
   uint32_t lev[(N + 3) / 4];
@@ -94,20 +101,32 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
     for (int j = 0; j < N; ++j) cv[j] = sel * c[j];
     add_column<N, MODE, N>(x, xl, cv);
   }
+  double cv[N];
+  if (t_begin > 0) {
+    // x at step t_begin - 1: the walk columns in gray(t_begin - 1), added
+    // (wave-uniform: the same for every lane)
+    const uint32_t gp = (t_begin - 1) ^ ((t_begin - 1) >> 1);
+    for (int b = 0; b < n_walk_bits; ++b)
+      if ((gp >> b) & 1u) {
+        load_col_n<N>(cv, cols, 2 * b);
+        add_column<N, MODE, N>(x, xl, cv);
+      }
+  }
 
   s = {0.0, 0.0};
   q = 0.0;
   walked_groups = 0;
+  stop_t = 0;
   LaneAcc acc;
-  const uint32_t steps = 1u << n_walk_bits;
-  const uint32_t groups = steps >> R;
+  const uint64_t t_end = (uint64_t)t_begin + (1ull << t_bits);
+  const uint32_t groups = (uint32_t)(t_end >> R);
   const uint32_t chunk_groups = 1u << (check_bits - R);
   Tails tl;
   double a_run = 0.0;
-  double cv[N];
-  load_col_n<N>(cv, cols, zero_blk);  // step 0 changes nothing
-  uint32_t t = 0;
-  while (t < steps) {
+  // the change of step t_begin (step 0 changes nothing)
+  load_col_n<N>(cv, cols, t_begin == 0 ? zero_blk : step_block(t_begin));
+  uint32_t t = t_begin;
+  while (t < t_end) {
     // x = x(gray(t-1)), cv = the change of step t: apply it, x = x(gray(t)).
     add_column<N, MODE, N>(x, xl, cv);
     // 1 + level of this lane's highest zero row (rows are in ascending
@@ -119,7 +138,7 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
       ml = x[j] == 0.0 ? lj : ml;
     }
     if (!valid) ml = 63;
-    const int align = t == 0 ? n_walk_bits : __builtin_ctz(t);
+    const int align = t == t_begin ? t_bits : __builtin_ctz(t);
     int m = (int)wave_min_u6(ml) - 1;
     m = m < align ? m : align;
     if (m >= check_bits) {
@@ -128,7 +147,7 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
       load_col_n<N>(cv, cols, 2 * (m - 1) + (int)(b & 1u));  // bit m-1 becomes !b_0
       add_column<N, MODE, N>(x, xl, cv);
       t += 1u << m;
-      load_col_n<N>(cv, cols, t < steps ? step_block(t) : zero_blk);
+      load_col_n<N>(cv, cols, t < t_end ? step_block(t) : zero_blk);
       continue;
     }
     // walk 2^check_bits steps; the first adds nothing (x is already at t)
@@ -153,6 +172,10 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
     q = __builtin_fma(d, d, q);
     a_run = acc.a;
     t += 1u << check_bits;
+    if (budget != 0 && walked_groups >= budget && t < t_end) {
+      stop_t = t;  // the rest of the range is a continuation's
+      break;
+    }
   }
   if constexpr (MODE == kFast) s = dd_add_d(s, acc.p);
   else s = dd_add(s, two_sum(acc.p, acc.plo));
@@ -163,7 +186,12 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
   }
 }
 
-// One skipping walk per lane over the plain split (pattern = lane id).
+// One skipping walk per lane.  Without a work list (work == nullptr): the
+// plain split, lane = global thread id, the whole walk [0, 2^walk_bits),
+// stopping after `budget` walked groups (cont[wave] = the step it stopped
+// at, 0 if it finished).  With a work list: wave j of the launch walks
+// entry j = (wave w of the plain split, first step, log2 steps) for the 64
+// lanes of wave w -- the continuations, to the end of their range.
 // partials: one record per workgroup; walked: groups of 2^R steps walked per
 // wave (the rest was skipped), one slot per wave, for the statistics.
 template <int N, int R, int MODE>
@@ -172,20 +200,38 @@ __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
                        const uint32_t* __restrict__ row_level, Partial* __restrict__ partials,
                        uint32_t* __restrict__ walked, uint64_t lanes, int n_walk_bits,
-                       int n_high_bits, int check_bits, int lane_reverse) {
+                       int n_high_bits, int check_bits, int lane_reverse,
+                       const uint32_t* __restrict__ work, uint32_t n_work, uint32_t budget,
+                       uint32_t* __restrict__ cont) {
   __shared__ Partial scratch[256 / kWave];
-  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = lane < lanes;
+  const uint32_t wave_g = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const uint32_t lane_w = threadIdx.x & (kWave - 1);
+  uint64_t lane;
+  uint32_t t_begin = 0;
+  int t_bits = n_walk_bits;
+  bool has = true;
+  if (work) {
+    has = wave_g < n_work;
+    const uint32_t e = has ? wave_g : 0u;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)work[4 * e]);
+    t_begin = (uint32_t)__builtin_amdgcn_readfirstlane((int)work[4 * e + 1]);
+    t_bits = __builtin_amdgcn_readfirstlane((int)work[4 * e + 2]);
+    lane = (uint64_t)w * kWave + lane_w;
+  } else {
+    lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  }
+  const bool valid = has && lane < lanes;
   // lanes of one wave differ in the low bits of the lane id: reversed, they
   // differ in the last high columns instead
   uint64_t pattern = valid ? lane : 0ull;
   if (lane_reverse && n_high_bits > 0)
     pattern = __builtin_bitreverse64(pattern) >> (64 - n_high_bits);
-  DD s;
-  double a, q;
-  uint32_t wg;
-  skip_walk<N, R, MODE>(colpm, x0, row_level, pattern, valid, n_walk_bits,
-                        n_high_bits, check_bits, s, a, q, wg);
+  DD s = {0.0, 0.0};
+  double a = 0.0, q = 0.0;
+  uint32_t wg = 0, stop_t = 0;
+  if (has)
+    skip_walk<N, R, MODE>(colpm, x0, row_level, pattern, valid, n_walk_bits, n_high_bits,
+                          check_bits, t_begin, t_bits, work ? 0u : budget, s, a, q, wg, stop_t);
   if (!valid) {
     s = {0.0, 0.0};
     a = 0.0;
@@ -193,8 +239,10 @@ void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restric
   }
   // integer matrices only: X is exact, no drift statistic (sq_run = 0)
   block_reduce_store(s, Moments{a, q, 0.0}, scratch, partials + blockIdx.x);
-  if ((threadIdx.x & (kWave - 1)) == 0)
-    walked[blockIdx.x * (256 / kWave) + threadIdx.x / kWave] = wg;
+  if (lane_w == 0) {
+    walked[wave_g] = wg;
+    if (cont) cont[wave_g] = stop_t;
+  }
 }
 
 template <int N>
@@ -205,11 +253,11 @@ hipError_t launch_skip_n(const SkipLaunch& L, hipStream_t stream) {
   if (L.mode == kFast)
     ryser_skip_kernel<N, R, kFast><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
         L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits, L.lane_reverse);
+        L.check_bits, L.lane_reverse, L.work, L.n_work, L.budget, L.cont);
   else if (L.mode == kDDProd)
     ryser_skip_kernel<N, R, kDDProd><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
         L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits, L.lane_reverse);
+        L.check_bits, L.lane_reverse, L.work, L.n_work, L.budget, L.cont);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
