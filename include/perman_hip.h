@@ -235,8 +235,9 @@ int perm_batch_add_dense_many(perm_batch* batch, int count, const int* dtypes,
 int perm_batch_size(const perm_batch* batch);
 /* One batch end to end: adds `count` dense matrices to an EMPTY batch and
  * runs it, pipelined -- the matrices are prepared on `threads` host threads
- * in decreasing order n, and the kernels of the largest orders (about 40 %
- * of the work) run while the rest is prepared and uploaded.  results: one
+ * in decreasing order n in three phases (cut at 15 % and 45 % of the work),
+ * and each phase's kernels run while the next phase is prepared and
+ * uploaded.  results: one
  * per matrix in the caller's order, the same values as perm_batch_add_dense_many
  * + perm_batch_run (a matrix's items depend only on its n); the batch is then
  * built, and later perm_batch_run calls replay it (results still in the

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -659,6 +661,16 @@ int perm_batch_run(perm_batch* B, perm_result* results, double* device_ms) {
 int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
                              const void* const* mats, const int* novs, int threads,
                              perm_result* results, double* device_ms) {
+  // PERMAN_BATCH_TRACE=1: the host time of each stage on stderr (diagnostics)
+  static const bool trace = perm::env_int("PERMAN_BATCH_TRACE", 0) != 0;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (trace)
+      std::fprintf(stderr, "[batch %.3f ms] %s\n",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                             t_start).count(),
+                   what);
+  };
   if (!B || count <= 0 || !dtypes || !mats || !novs || !results)
     return perm::fail(PERM_ERR_ARG, "null pointer or empty input");
   if (!B->ents.empty()) return perm::fail(PERM_ERR_ARG, "the batch must be empty");
@@ -670,8 +682,9 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
   HIP_TRYB(hipSetDevice(B->device));
   release_device(B);
   threads = std::max(1, std::min(threads, count));
-  // processing order: decreasing n; phase 0 = the largest orders until 40 %
-  // of the work (whole orders), phase 1 the rest
+  // processing order: decreasing n; three phases, cut at 15 % and 45 % of
+  // the work: the first is small, so the GPU starts early, and each later
+  // one is prepared while the earlier ones run
   std::vector<uint32_t> order((size_t)count);
   for (int i = 0; i < count; ++i) order[(size_t)i] = (uint32_t)i;
   std::stable_sort(order.begin(), order.end(),
@@ -679,10 +692,16 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
   auto work_of = [&](uint32_t i) { return std::ldexp((double)novs[i], novs[i] - 1); };
   double total = 0.0;
   for (int i = 0; i < count; ++i) total += work_of((uint32_t)i);
-  size_t split = 0;
-  for (double acc = 0.0; split < (size_t)count && acc < 0.4 * total; ++split)
-    acc += work_of(order[split]);
-  while (split < (size_t)count && novs[order[split]] == novs[order[split - 1]]) ++split;
+  size_t cut[2] = {0, 0};
+  {
+    const double frac[2] = {0.15, 0.45};
+    size_t k = 0;
+    double acc = 0.0;
+    for (int c = 0; c < 2; ++c) {
+      while (k < (size_t)count && acc < frac[c] * total) acc += work_of(order[k++]);
+      cut[c] = std::max<size_t>(k, 1);
+    }
+  }
   // device memory from the orders alone (upper bounds: column store, start
   // vector and padding per matrix; every matrix's items)
   size_t data_ub = 0, items_ub = 0;
@@ -711,8 +730,9 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
     B->groups.clear();
     return st;
   };
-  if (int st = alloc_device(B, data_ub, (size_t)count, items_ub, 2 * (size_t)count))
+  if (int st = alloc_device(B, data_ub, (size_t)count, items_ub, 3 * (size_t)count))
     return undo(st);
+  mark("allocated");
   hipStream_t s0 = B->s[0];
   const int ns = std::max(1, std::min(B->streams, kMaxStreams));
   // the variant kernels go to the streams other than s0 (s0 carries the
@@ -721,13 +741,13 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
   const int k0 = ns > 1 ? 1 : 0;
   std::vector<double> stream_work((size_t)ns, 0.0);
   HIP_TRYB(hipEventRecord(B->ev0, s0));
-  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)(2 * count), s0));
+  HIP_TRYB(perm::launch_batch_reset(B->d_counters, (uint32_t)(3 * count), s0));
   std::vector<perm::Prepared> prep((size_t)count);
   std::vector<int> status((size_t)count, PERM_OK);
   std::vector<std::string> msg((size_t)count);
   std::vector<BatchMatrix> dm((size_t)count);
-  const size_t phases[3] = {0, split, (size_t)count};
-  for (int ph = 0; ph < 2; ++ph) {
+  const size_t phases[4] = {0, cut[0], std::max(cut[0], cut[1]), (size_t)count};
+  for (int ph = 0; ph < 3; ++ph) {
     const size_t p0 = phases[ph], p1 = phases[ph + 1];
     if (p0 == p1) continue;
     // prepare the phase's matrices on `threads` host threads (the first
@@ -745,6 +765,7 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
         });
       for (auto& th : pool) th.join();
     }
+    mark(ph == 0 ? "phase 0 prepared" : ph == 1 ? "phase 1 prepared" : "phase 2 prepared");
     for (size_t k = p0; k < p1; ++k)
       if (status[k] != PERM_OK)
         return undo(perm::fail(status[k], "matrix " + std::to_string(order[k]) + ": " + msg[k]));
@@ -758,6 +779,7 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
     if (B->data.size() > data_ub) return undo(perm::fail(PERM_ERR_SIZE, "batch data bound"));
     plan_items(B, idx);
     for (size_t k = p0; k < p1; ++k) dm[k] = device_matrix(B, B->ents[k]);
+    mark("planned");
     // this phase's data, matrices and items, then its variant kernels
     HIP_TRYB(hipMemcpyAsync(B->d_data + d0, B->data.data() + d0,
                             (B->data.size() - d0) * sizeof(double), hipMemcpyHostToDevice, s0));
@@ -780,6 +802,7 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
       HIP_TRYB(perm::launch_batch_variant(G.vid, B->d_mats, B->d_items, G.begin, G.end,
                                           B->d_counters + gi, B->d_partials, grid, B->s[si]));
     }
+    mark("launched");
   }
   for (int i = 1; i < ns; ++i) {
     HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
@@ -790,13 +813,16 @@ int perm_batch_compute_dense(perm_batch* B, int count, const int* dtypes,
                           hipMemcpyDeviceToHost, s0));
   HIP_TRYB(hipEventRecord(B->ev1, s0));
   HIP_TRYB(hipStreamSynchronize(s0));
+  mark("device done");
   float ms = 0.f;
   HIP_TRYB(hipEventElapsedTime(&ms, B->ev0, B->ev1));
   // the batch is built: later runs replay it (a captured graph from the
   // second perm_batch_run on), results in the caller's order
   B->built = true;
   B->ran = true;
-  return finish_run(B, results, ms, device_ms);
+  const int st = finish_run(B, results, ms, device_ms);
+  mark("finished");
+  return st;
 }
 
 // Autotuner of the batch's work split: item size (Gray-code chunk length per
