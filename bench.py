@@ -327,7 +327,8 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
     autotuned on a sample of the same mix (another seed).  Capacity: every
     request submitted at once (closed loop).  Then an open loop of Poisson
     arrivals at `load` x capacity driven from C (perm_server_drive); p50/p99
-    of submit-to-result latency.  Plain: one perm_dense / perm_sparse call
+    of arrival-to-result latency (a submitter that falls behind its
+    arrival times counts against it).  Plain: one perm_dense / perm_sparse call
     per request.  Every side is timed around its C calls only (arguments
     prebuilt, results into C arrays)."""
     import ctypes
@@ -400,6 +401,7 @@ def served_leg(P, count: int, load: float, seed: int, decay: float = 4.0):
         "offered_load": load,
         "offered_req_per_s": rate,
         "achieved_req_per_s": count / t_open,
+        "latency_origin": "scheduled arrival time (Poisson), to the result",
         "latency_p50_ms": float(np.percentile(lat_ms, 50)),
         "latency_p99_ms": float(np.percentile(lat_ms, 99)),
         "latency_max_ms": float(lat_ms.max()),

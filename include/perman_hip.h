@@ -461,8 +461,10 @@ void perm_server_destroy(perm_server* server);
 /* Open-loop driver (load testing): submits request i (arguments as
  * perm_server_submit_many) at arrival_s[i] seconds after the call from a
  * thread of its own, and collects every result in order on the calling
- * thread: out[i], latency_ms[i] (submit to result; may be NULL), *wall_s
- * the time from the call until the last result (may be NULL). */
+ * thread: out[i], latency_ms[i] (from the arrival time to the result, so
+ * a submitter that falls behind shows; may be NULL), *wall_s the time from
+ * the call until the last result (may be NULL).  Requests already due when
+ * the thread gets to them are submitted in one call. */
 int perm_server_drive(perm_server* server, int count, const int* dtypes,
                       const void* const* mats, const int* const* cptrs, const int* const* rows,
                       const void* const* cvals, const int* novs, const double* arrival_s,

@@ -1085,7 +1085,7 @@ int make_request(int dtype, const void* mat, const int* cptrs, const int* rows,
 // Register requests (one lock) and wake the prep pool.  t0: when the
 // submission call began (statistics).
 int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* ids,
-            Clock::time_point t0) {
+            Clock::time_point t0, const Clock::time_point* arrival = nullptr) {
   {
     std::lock_guard<std::mutex> lk(S->mu);
     const char* why = S->fatal != PERM_OK ? "server failed: " : S->stopping.load() ? "server is shutting down" : nullptr;
@@ -1097,7 +1097,7 @@ int enqueue(perm_server* S, Request* const* rs, int count, unsigned long long* i
     const Clock::time_point now = Clock::now();
     for (int i = 0; i < count; ++i) {
       Request* r = rs[i];
-      r->t_submit = now;
+      r->t_submit = arrival ? arrival[i] : now;
       r->id = S->next_id++;
       S->table[r->id] = r;
       ids[i] = r->id;
@@ -1404,29 +1404,51 @@ int perm_server_drive(perm_server* S, int count, const int* dtypes, const void* 
   std::atomic<int> sub_status{PERM_OK};
   std::string sub_error;
   const Clock::time_point t0 = Clock::now();
+  auto due = [&](int i) {
+    return t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+  };
+  // Every request is handed over at (or after) its arrival time, and its
+  // latency counts from the arrival time: a submitter that falls behind
+  // shows in the latencies.  Requests already due are handed over in one
+  // call (up to kDriveBatch), as an ingress hands over simultaneous arrivals.
+  constexpr int kDriveBatch = 256;
   std::thread submitter([&] {
-    for (int i = 0; i < count; ++i) {
-      const Clock::time_point due =
-          t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+    std::vector<Request*> rs;
+    std::vector<Clock::time_point> arr;
+    for (int i = 0; i < count;) {
+      const Clock::time_point d = due(i);
       for (;;) {
         const Clock::time_point now = Clock::now();
-        if (now >= due) break;
-        if (due - now > std::chrono::microseconds(200))
-          std::this_thread::sleep_for(due - now - std::chrono::microseconds(100));
+        if (now >= d) break;
+        if (d - now > std::chrono::microseconds(200))
+          std::this_thread::sleep_for(d - now - std::chrono::microseconds(100));
       }
-      const bool sp = cptrs && cptrs[i];
       const Clock::time_point ts = Clock::now();
-      Request* r = nullptr;
-      int st = make_request(dtypes[i], mats[i], sp ? cptrs[i] : nullptr, sp ? rows[i] : nullptr,
-                            sp ? cvals[i] : nullptr, novs[i], r);
-      if (st == PERM_OK) st = enqueue(S, &r, 1, &ids[i], ts);
+      rs.clear();
+      arr.clear();
+      int st = PERM_OK;
+      for (int j = i; j < count && j - i < kDriveBatch && (j == i || due(j) <= ts); ++j) {
+        const bool sp = cptrs && cptrs[j];
+        Request* r = nullptr;
+        st = make_request(dtypes[j], mats[j], sp ? cptrs[j] : nullptr, sp ? rows[j] : nullptr,
+                          sp ? cvals[j] : nullptr, novs[j], r);
+        if (st != PERM_OK) break;
+        rs.push_back(r);
+        arr.push_back(due(j));
+      }
+      if (st != PERM_OK) {
+        for (Request* r : rs) delete r;
+      } else {
+        st = enqueue(S, rs.data(), (int)rs.size(), &ids[i], ts, arr.data());
+      }
       if (st != PERM_OK) {
         sub_error = perm::last_error();
         sub_status.store(st);
         submitted.store(-1);
         return;
       }
-      submitted.store(i + 1, std::memory_order_release);
+      i += (int)rs.size();
+      submitted.store(i, std::memory_order_release);
     }
   });
   int first_error = PERM_OK;

@@ -657,9 +657,11 @@ class Server:
         return ids
 
     def drive(self, batch: "RequestBatch", arrival_s):
-        """Open loop (perm_server_drive): request i submitted arrival_s[i]
-        seconds after the start; returns the ctypes arrays of results and
-        latencies and the wall time in seconds."""
+        """Open loop (perm_server_drive): request i arrives arrival_s[i]
+        seconds after the start (requests already due are submitted
+        together); returns the ctypes arrays of results and latencies (from
+        each request's arrival time to its result) and the wall time in
+        seconds."""
         k = batch.count
         arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
         res = (PermResultC * k)()

@@ -221,6 +221,30 @@ def test_server_submit_many_same_bits_and_errors_at_wait():
             srv.wait(tid.value)
 
 
+def test_server_open_loop_drive_same_bits():
+    """The open-loop driver (perm_server_drive) gives every request the bits
+    of a bulk submission, whether it arrives alone or in a burst of
+    simultaneous arrivals handed over in one call; latencies count from the
+    arrival time and the wall time covers the last arrival."""
+    mats = [(P.synthetic_matrix(12 + i % 15, 0.5, seed=2300 + i), i % 4 == 2) for i in range(600)]
+    reqs = [P.Request(m, sp) for m, sp in mats]
+    batch = P.RequestBatch(reqs)
+    # a burst of 300 at t = 0, then one every 0.2 ms
+    arrivals = np.concatenate([np.zeros(300), 0.001 + 0.0002 * np.arange(300)])
+    with P.Server() as srv:
+        ref = srv.wait_many(srv.submit_many(reqs))
+    with P.Server() as srv:
+        res, lat, wall = srv.drive(batch, arrivals)
+    assert wall >= arrivals[-1]
+    for (a, _), r, l in zip(ref, res, lat):
+        assert (a.value, a.sum_hi, a.sum_lo) == (r.value, r.sum_hi, r.sum_lo)
+        assert 0.0 < l <= wall * 1e3
+    lat = np.array(list(lat))
+    # the burst's requests wait for each other: its last result comes after
+    # its first, counted from the same arrival
+    assert lat[:300].max() > lat[:300].min()
+
+
 def test_server_sampled_error_measurement():
     """Non-integer requests of many items whose modelled error estimate
     misses 1e-10 have their error measured on a sample of items in the
