@@ -432,8 +432,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=cpu_share())
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 0/1, batched and served legs")
-    ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--serve-count", type=int, default=20000)
+    ap.add_argument("--batch-count", type=int, default=4096,
+                    help="matrices of the batched leg (0: no batched leg)")
+    ap.add_argument("--serve-count", type=int, default=20000,
+                    help="requests of each served leg (0: no served legs)")
+    ap.add_argument("--serve-mixes", choices=("both", "heavy", "small"), default="both",
+                    help="served legs: P(n) ~ 2^(-(n-16)/4) (heavy), ~ 2^-(n-16) (small)")
     ap.add_argument("--serve-load", type=float, default=0.7)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -530,11 +534,13 @@ def main():
     with affinity_kept():
         out["cpu_path_n20"] = cpu_leg(P, args.cpu_threads)
     if not args.no_mixed:
-        progress("batched leg")
-        out["batched"] = batched_leg(P, args.batch_count, 5)
-        if args.serve_count > 0:
+        if args.batch_count > 0:
+            progress("batched leg")
+            out["batched"] = batched_leg(P, args.batch_count, 5)
+        if args.serve_count > 0 and args.serve_mixes != "small":
             progress("served leg (decay 4)")
             out["served"] = served_leg(P, args.serve_count, args.serve_load, 6)
+        if args.serve_count > 0 and args.serve_mixes != "heavy":
             # the other reading of the configuration's size law, P(n) ~ 2^-(n-16):
             # mostly n = 16..18, per-request overhead dominates
             progress("served leg (decay 1)")

@@ -66,15 +66,21 @@ __device__ __forceinline__ int step_block(uint32_t t) {
 // walked `budget` groups of 2^R steps, at the start of its next check; stop_t
 // is then that step (0 when the range was finished) and the rest of the
 // range is left to later waves (a continuation starts from the closed form
-// of x at its first step).
-template <int N, int R, int MODE>
+// of x at its first step).  PIECES = false: the whole walk without a budget
+// (t_begin = 0, t_bits = n_walk_bits, budget = 0 fixed at compile time, so
+// the loop carries none of the range bookkeeping).  n_walk_bits < 32: every
+// step index fits 32 bits.
+template <int N, int R, int MODE, bool PIECES>
 __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
                                           const double* __restrict__ x0,
                                           const uint32_t* __restrict__ row_level,
                                           uint64_t pattern, bool valid, int n_walk_bits,
-                                          int n_high_bits, int check_bits, uint32_t t_begin,
-                                          int t_bits, uint32_t budget, DD& s, double& a,
+                                          int n_high_bits, int check_bits, uint32_t t_begin_arg,
+                                          int t_bits_arg, uint32_t budget_arg, DD& s, double& a,
                                           double& q, uint32_t& walked_groups, uint32_t& stop_t) {
+  const uint32_t t_begin = PIECES ? t_begin_arg : 0u;
+  const int t_bits = PIECES ? t_bits_arg : n_walk_bits;
+  const uint32_t budget = PIECES ? budget_arg : 0u;
   const ColPtr cols = (ColPtr)colpm;
   // row levels, byte j of word j/4 = level of row j (scalar loads, SGPRs)
   uint32_t lev[(N + 3) / 4];
@@ -118,8 +124,8 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
   walked_groups = 0;
   stop_t = 0;
   LaneAcc acc;
-  const uint64_t t_end = (uint64_t)t_begin + (1ull << t_bits);
-  const uint32_t groups = (uint32_t)(t_end >> R);
+  const uint32_t t_end = t_begin + (1u << t_bits);
+  const uint32_t groups = t_end >> R;
   const uint32_t chunk_groups = 1u << (check_bits - R);
   Tails tl;
   double a_run = 0.0;
@@ -172,7 +178,7 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
     q = __builtin_fma(d, d, q);
     a_run = acc.a;
     t += 1u << check_bits;
-    if (budget != 0 && walked_groups >= budget && t < t_end) {
+    if (PIECES && budget != 0 && walked_groups >= budget && t < t_end) {
       stop_t = t;  // the rest of the range is a continuation's
       break;
     }
@@ -186,7 +192,8 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
   }
 }
 
-// One skipping walk per lane.  Without a work list (work == nullptr): the
+// One skipping walk per lane.  PIECES = false: no work list and no budget
+// (the arguments are ignored).  Without a work list (work == nullptr): the
 // plain split, lane = global thread id, the whole walk [0, 2^walk_bits),
 // stopping after `budget` walked groups (cont[wave] = the step it stopped
 // at, 0 if it finished).  With a work list: wave j of the launch walks
@@ -194,7 +201,7 @@ __device__ __forceinline__ void skip_walk(const double* __restrict__ colpm,
 // lanes of wave w -- the continuations, to the end of their range.
 // partials: one record per workgroup; walked: groups of 2^R steps walked per
 // wave (the rest was skipped), one slot per wave, for the statistics.
-template <int N, int R, int MODE>
+template <int N, int R, int MODE, bool PIECES>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MODE == kFast ? kFastWavesPerEU : 1)))
 void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restrict__ x0,
@@ -210,7 +217,7 @@ void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restric
   uint32_t t_begin = 0;
   int t_bits = n_walk_bits;
   bool has = true;
-  if (work) {
+  if (PIECES && work) {
     has = wave_g < n_work;
     const uint32_t e = has ? wave_g : 0u;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)work[4 * e]);
@@ -230,8 +237,9 @@ void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restric
   double a = 0.0, q = 0.0;
   uint32_t wg = 0, stop_t = 0;
   if (has)
-    skip_walk<N, R, MODE>(colpm, x0, row_level, pattern, valid, n_walk_bits, n_high_bits,
-                          check_bits, t_begin, t_bits, work ? 0u : budget, s, a, q, wg, stop_t);
+    skip_walk<N, R, MODE, PIECES>(colpm, x0, row_level, pattern, valid, n_walk_bits, n_high_bits,
+                                  check_bits, t_begin, t_bits, work ? 0u : budget, s, a, q, wg,
+                                  stop_t);
   if (!valid) {
     s = {0.0, 0.0};
     a = 0.0;
@@ -241,7 +249,7 @@ void ryser_skip_kernel(const double* __restrict__ colpm, const double* __restric
   block_reduce_store(s, Moments{a, q, 0.0}, scratch, partials + blockIdx.x);
   if (lane_w == 0) {
     walked[wave_g] = wg;
-    if (cont) cont[wave_g] = stop_t;
+    if (PIECES && cont) cont[wave_g] = stop_t;
   }
 }
 
@@ -250,16 +258,22 @@ hipError_t launch_skip_n(const SkipLaunch& L, hipStream_t stream) {
   constexpr int R = unroll_bits_for<N>();
   if (L.walk_bits < R || L.check_bits < R || L.check_bits > L.walk_bits)
     return hipErrorInvalidValue;
-  if (L.mode == kFast)
-    ryser_skip_kernel<N, R, kFast><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
-        L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits, L.lane_reverse, L.work, L.n_work, L.budget, L.cont);
-  else if (L.mode == kDDProd)
-    ryser_skip_kernel<N, R, kDDProd><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>(
-        L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,
-        L.check_bits, L.lane_reverse, L.work, L.n_work, L.budget, L.cont);
-  else
+  if (L.walk_bits >= 32) return hipErrorInvalidValue;
+  const bool pieces = L.work != nullptr || L.budget != 0;
+#define PERM_SKIP_LAUNCH(MODE_, PIECES_)                                                   \
+  ryser_skip_kernel<N, R, MODE_, PIECES_><<<dim3((unsigned)L.blocks), dim3(256), 0, stream>>>( \
+      L.colpm, L.x0, L.row_level, L.partials, L.walked, L.lanes, L.walk_bits, L.high_bits,  \
+      L.check_bits, L.lane_reverse, L.work, L.n_work, L.budget, L.cont)
+  if (L.mode == kFast) {
+    if (pieces) PERM_SKIP_LAUNCH(kFast, true);
+    else PERM_SKIP_LAUNCH(kFast, false);
+  } else if (L.mode == kDDProd) {
+    if (pieces) PERM_SKIP_LAUNCH(kDDProd, true);
+    else PERM_SKIP_LAUNCH(kDDProd, false);
+  } else {
     return hipErrorInvalidValue;
+  }
+#undef PERM_SKIP_LAUNCH
   return hipGetLastError();
 }
 
