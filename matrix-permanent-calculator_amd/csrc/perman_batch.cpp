@@ -487,13 +487,21 @@ int finish_run(perm_batch* B, perm_result* out, float ms, double* device_ms) {
     }
     if (esc.empty() && sample.empty()) break;
     float pass_ms = 0.f;
+    // PERMAN_BATCH_TRACE=1: each round's passes on stderr (diagnostics)
+    static const bool trace = perm::env_int("PERMAN_BATCH_TRACE", 0) != 0;
     if (!sample.empty()) {
       if (int s = verify_batch(B, sample, results, pass_ms)) return s;
       total_ms += pass_ms;
+      if (trace)
+        std::fprintf(stderr, "[batch] main %.3f ms; round %d: %zu sampled, %.3f ms\n", ms, round,
+                     sample.size(), pass_ms);
     }
     if (!esc.empty()) {
       if (int s = escalate(B, esc, results, pass_ms)) return s;
       total_ms += pass_ms;
+      if (trace)
+        std::fprintf(stderr, "[batch] main %.3f ms; round %d: %zu escalated, %.3f ms\n", ms,
+                     round, esc.size(), pass_ms);
     }
   }
   if (device_ms) *device_ms = total_ms;

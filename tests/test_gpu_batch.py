@@ -235,3 +235,31 @@ def test_c4_mix_every_member_exact_zero_or_within_1e10():
         assert abs(r.value - s.value) <= 1e-10 * abs(s.value), (rec["i"], r, s)
     print(f"C4 mix: {len(members)} members checked, batch modes {modes}, device {ms:.1f} ms")
     assert all(r.certified for r in res)
+
+
+def test_replays_with_escalations_same_bits():
+    """The batched configuration's mix, whose hardest members escalate (a
+    sampled double-double measurement, then full double-double passes):
+    every graph replay gives the first run's bits, err_est, mode and
+    certificate for every member, and a fresh batch computed in one call
+    gives the same bits."""
+    mats = P.mixed_batch(4096, 12, 28, 0.2, 0.8, 5)
+    b = P.Batch(item_log2=16, streams=8, max_grid=512)
+    b.add_prepared(P.BatchInputs(mats), threads=8)
+    runs = [b.run() for _ in range(3)]
+    b.close()
+    first = runs[0][0]
+    escalated = sum(r.mode != 0 for r in first)
+    assert escalated >= 2, escalated  # the escalation rounds run on every replay
+    for res, ms in runs[1:]:
+        for i, (a, r) in enumerate(zip(first, res)):
+            assert (a.value, a.sum_hi, a.sum_lo, a.err_est, a.mode, a.certified) == (
+                r.value, r.sum_hi, r.sum_lo, r.err_est, r.mode, r.certified), i
+    one = P.Batch(item_log2=16, streams=8, max_grid=512)
+    res_c = (P.PermResultC * len(mats))()
+    one.compute_prepared(P.BatchInputs(mats), res_c, threads=8)
+    one.close()
+    for i, (a, r) in enumerate(zip(first, res_c)):
+        assert (a.value, a.sum_hi, a.sum_lo, a.mode) == (r.value, r.sum_hi, r.sum_lo, r.mode), i
+    print("escalated:", escalated, "device ms per run:", [round(ms, 2) for _, ms in runs])
+
