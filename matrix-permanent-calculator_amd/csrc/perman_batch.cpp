@@ -305,61 +305,29 @@ int build(perm_batch* B) {
   return perm::ok();
 }
 
-// One escalation round: for each (matrix, mode) a persistent kernel of that
-// mode's variant over the matrix's items (double-double X runs the plain
-// walk), spread over the streams; then the per-matrix reduction and the
-// results of those matrices refilled in the new mode.  pass_ms: device time.
-int escalate(perm_batch* B, const std::vector<std::pair<size_t, int>>& esc,
-             perm_result* results, float& pass_ms) {
-  hipStream_t s0 = B->s[0];
-  uint32_t* ctr = B->d_counters + B->groups.size();
-  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
-  HIP_TRYB(hipEventRecord(B->ev0, s0));
-  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)esc.size(), s0));
-  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
-  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
-  for (size_t i = 0; i < esc.size(); ++i) {
-    const Entry& e = B->ents[esc[i].first];
-    const int md = esc[i].second;
-    const perm::NestShape sh = md == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
-    const int vid = perm::batch_variant(e.n, sh, md);
-    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel of this mode for this n");
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
-    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
-                                        e.first_item + e.item_count, ctr + i, B->d_partials, grid,
-                                        B->s[i % ns]));
-  }
-  for (int i = 1; i < ns; ++i) {
-    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
-    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
-  }
-  const uint32_t nm = (uint32_t)B->ents.size();
-  HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
-  HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial), hipMemcpyDeviceToHost,
-                          s0));
-  HIP_TRYB(hipEventRecord(B->ev1, s0));
-  HIP_TRYB(hipStreamSynchronize(s0));
-  HIP_TRYB(hipEventElapsedTime(&pass_ms, B->ev0, B->ev1));
-  for (const auto& em : esc) {
-    const Entry& e = B->ents[em.first];
-    const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
-                           B->h_results[em.first], results[em.first]);
-  }
-  return PERM_OK;
-}
-
+// One round of the escalation (finish_run): the sampled error measurement
+// of the matrices `ms` and the passes in a more accurate mode of the
+// matrices `esc` (disjoint sets), launched together and finished with one
+// synchronisation.  pass_ms: device time.
+//
 // Sampled error measurement (perm::pps_sample) of the FP64 results of the
 // matrices `ms` (non-integer, at least kVerifyMinUnits items): for each, up
 // to kVerifyUnits of its items, drawn in proportion to their modelled
 // variance, are walked again in mode 2 (copies of the items in a separate
 // list, records in a separate array), and err_est becomes the estimate from
-// their measured errors (perm::sampled_error).  Values are unchanged.
-int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* results,
-                 float& pass_ms) {
+// their measured errors (perm::sampled_error); values are unchanged.
+// Escalation: for each (matrix, mode) a persistent kernel of that mode's
+// variant over the matrix's items (double-double X runs the plain walk),
+// spread over the streams; then the per-matrix reduction and the results of
+// those matrices refilled in the new mode.
+int run_round(perm_batch* B, const std::vector<size_t>& ms,
+              const std::vector<std::pair<size_t, int>>& esc, perm_result* results,
+              float& pass_ms) {
   pass_ms = 0.f;
   hipStream_t s0 = B->s[0];
-  // the FP64 item records of the sampled matrices only (not the batch's)
+  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  // the sample: the FP64 item records of the sampled matrices only (not the
+  // batch's), the picks, and copies of the picked items
   std::vector<std::vector<Partial>> fp(ms.size());
   for (size_t k = 0; k < ms.size(); ++k) {
     const Entry& e = B->ents[ms[k]];
@@ -367,7 +335,7 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     HIP_TRYB(hipMemcpyAsync(fp[k].data(), B->d_partials + e.first_item,
                             e.item_count * sizeof(Partial), hipMemcpyDeviceToHost, s0));
   }
-  HIP_TRYB(hipStreamSynchronize(s0));
+  if (!ms.empty()) HIP_TRYB(hipStreamSynchronize(s0));
   struct Pick {
     std::vector<uint32_t> ids;
     std::vector<double> weight;
@@ -386,24 +354,32 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     picks[k].begin = (uint32_t)vitems.size();
     for (uint32_t i : picks[k].ids) vitems.push_back(B->items[e.first_item + i]);
   }
-  if (vitems.empty()) return PERM_OK;
-  const size_t it_b = align256(vitems.size() * sizeof(BatchItem));
-  const size_t need = it_b + align256(vitems.size() * sizeof(Partial));
-  if (need > B->vbytes) {
-    if (B->vmem) HIP_TRYB(hipFree(B->vmem));
-    B->vmem = nullptr;
-    B->vbytes = 0;
-    HIP_TRYB(hipMalloc(&B->vmem, need));
-    B->vbytes = need;
+  BatchItem* d_vitems = nullptr;
+  Partial* d_vparts = nullptr;
+  if (!vitems.empty()) {
+    const size_t it_b = align256(vitems.size() * sizeof(BatchItem));
+    const size_t need = it_b + align256(vitems.size() * sizeof(Partial));
+    if (need > B->vbytes) {
+      if (B->vmem) HIP_TRYB(hipFree(B->vmem));
+      B->vmem = nullptr;
+      B->vbytes = 0;
+      HIP_TRYB(hipMalloc(&B->vmem, need));
+      B->vbytes = need;
+    }
+    d_vitems = (BatchItem*)B->vmem;
+    d_vparts = (Partial*)((char*)B->vmem + it_b);
   }
-  BatchItem* d_vitems = (BatchItem*)B->vmem;
-  Partial* d_vparts = (Partial*)((char*)B->vmem + it_b);
+  if (vitems.empty() && esc.empty()) return PERM_OK;
+  // one counter per launched kernel: the sample's first, then the passes'
+  // (the two sets are disjoint, so at most one counter per matrix)
   uint32_t* ctr = B->d_counters + B->groups.size();
-  const int ns = std::max(1, std::min(B->streams, kMaxStreams));
+  const size_t n_ctr = ms.size() + esc.size();
+  if (n_ctr > B->ents.size()) return perm::fail(PERM_ERR_ARG, "escalation round counters");
   HIP_TRYB(hipEventRecord(B->ev0, s0));
-  HIP_TRYB(hipMemcpyAsync(d_vitems, vitems.data(), vitems.size() * sizeof(BatchItem),
-                          hipMemcpyHostToDevice, s0));
-  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)ms.size(), s0));
+  if (!vitems.empty())
+    HIP_TRYB(hipMemcpyAsync(d_vitems, vitems.data(), vitems.size() * sizeof(BatchItem),
+                            hipMemcpyHostToDevice, s0));
+  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)n_ctr, s0));
   HIP_TRYB(hipEventRecord(B->ev_fork, s0));
   for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
   for (size_t k = 0; k < ms.size(); ++k) {
@@ -417,19 +393,39 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
                                         picks[k].begin + cnt, ctr + k, d_vparts, grid,
                                         B->s[k % ns]));
   }
+  for (size_t i = 0; i < esc.size(); ++i) {
+    const size_t j = ms.size() + i;
+    const Entry& e = B->ents[esc[i].first];
+    const int md = esc[i].second;
+    const perm::NestShape sh = md == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+    const int vid = perm::batch_variant(e.n, sh, md);
+    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel of this mode for this n");
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
+    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
+                                        e.first_item + e.item_count, ctr + j, B->d_partials, grid,
+                                        B->s[j % ns]));
+  }
   for (int i = 1; i < ns; ++i) {
     HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
     HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
   }
+  const uint32_t nm = (uint32_t)B->ents.size();
+  if (!esc.empty()) {
+    HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
+    HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial),
+                            hipMemcpyDeviceToHost, s0));
+  }
   std::vector<Partial> ref(vitems.size());
-  HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, vitems.size() * sizeof(Partial),
-                          hipMemcpyDeviceToHost, s0));
+  if (!vitems.empty())
+    HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, vitems.size() * sizeof(Partial),
+                            hipMemcpyDeviceToHost, s0));
   HIP_TRYB(hipEventRecord(B->ev1, s0));
   HIP_TRYB(hipStreamSynchronize(s0));
   HIP_TRYB(hipEventElapsedTime(&pass_ms, B->ev0, B->ev1));
   for (size_t k = 0; k < ms.size(); ++k) {
     const Entry& e = B->ents[ms[k]];
     const Pick& pk = picks[k];
+    if (pk.ids.empty()) continue;
     std::vector<double> err(pk.ids.size());
     for (size_t j = 0; j < pk.ids.size(); ++j) {
       const Partial& f = fp[k][pk.ids[j]];
@@ -439,6 +435,12 @@ int verify_batch(perm_batch* B, const std::vector<size_t>& ms, perm_result* resu
     perm_result& o = results[ms[k]];
     o.err_est = perm::sampled_error(err, pk.weight, pk.certain, e.n, o.value);
     o.certified = perm::is_certified(o) ? 1 : 0;
+  }
+  for (const auto& em : esc) {
+    const Entry& e = B->ents[em.first];
+    const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
+                           B->h_results[em.first], results[em.first]);
   }
   return PERM_OK;
 }
@@ -467,7 +469,7 @@ int finish_run(perm_batch* B, perm_result* out, float ms, double* device_ms) {
   // then the per-matrix reduction again.  At most three rounds: a sampled
   // measurement, then passes whose mode only increases.
   // A non-integer matrix of many items first has its FP64 error measured on
-  // a sample of its items (verify_batch); it gets the full pass only if that
+  // a sample of its items (run_round); it gets the full pass only if that
   // does not certify it.
   std::vector<char> sampled(nm, 0);
   for (int round = 0; round < 3; ++round) {
@@ -489,19 +491,18 @@ int finish_run(perm_batch* B, perm_result* out, float ms, double* device_ms) {
     float pass_ms = 0.f;
     // PERMAN_BATCH_TRACE=1: each round's passes on stderr (diagnostics)
     static const bool trace = perm::env_int("PERMAN_BATCH_TRACE", 0) != 0;
-    if (!sample.empty()) {
-      if (int s = verify_batch(B, sample, results, pass_ms)) return s;
-      total_ms += pass_ms;
-      if (trace)
-        std::fprintf(stderr, "[batch] main %.3f ms; round %d: %zu sampled, %.3f ms\n", ms, round,
-                     sample.size(), pass_ms);
-    }
-    if (!esc.empty()) {
-      if (int s = escalate(B, esc, results, pass_ms)) return s;
-      total_ms += pass_ms;
-      if (trace)
-        std::fprintf(stderr, "[batch] main %.3f ms; round %d: %zu escalated, %.3f ms\n", ms,
-                     round, esc.size(), pass_ms);
+    std::vector<double> est0;
+    if (trace)
+      for (size_t m : sample) est0.push_back(results[m].err_est / std::fabs(results[m].value));
+    if (int s = run_round(B, sample, esc, results, pass_ms)) return s;
+    total_ms += pass_ms;
+    if (trace) {
+      std::fprintf(stderr, "[batch] main %.3f ms; round %d: %zu sampled, %zu escalated, %.3f ms\n",
+                   ms, round, sample.size(), esc.size(), pass_ms);
+      for (size_t k = 0; k < sample.size(); ++k)
+        std::fprintf(stderr, "[batch]   sampled matrix %zu (n %d): err_est/|value| %.3g -> %.3g\n",
+                     sample[k], B->ents[sample[k]].n, est0[k],
+                     results[sample[k]].err_est / std::fabs(results[sample[k]].value));
     }
   }
   if (device_ms) *device_ms = total_ms;
