@@ -305,10 +305,24 @@ int build(perm_batch* B) {
   return perm::ok();
 }
 
+// Full double-double X passes (mode 2) of matrices of at most
+// kFineMaxItems items walk a 2^kFineBits times finer split: lanes of
+// 2^(walk_bits - kFineBits) steps, 2^kFineBits times the items.  A coarse
+// item lasts about a millisecond in that mode, so a pass of a few hundred
+// items was one item per SIMD with most of the GPU idle; the fine items
+// spread it over the whole GPU.  The result is the matrix's in a batch of
+// 2^kFineBits times smaller items, with the error model of that split, and
+// depends only on the matrix.  Fine passes are launched in chunks of at
+// most kFineChunkItems items (bounded scratch memory).
+constexpr int kFineBits = 4;
+constexpr uint32_t kFineMaxItems = 4096;
+constexpr size_t kFineChunkItems = size_t(1) << 20;
+
 // One round of the escalation (finish_run): the sampled error measurement
 // of the matrices `ms` and the passes in a more accurate mode of the
 // matrices `esc` (disjoint sets), launched together and finished with one
-// synchronisation.  pass_ms: device time.
+// synchronisation (fine passes beyond the first chunk in further launch
+// sets).  pass_ms: device time.
 //
 // Sampled error measurement (perm::pps_sample) of the FP64 results of the
 // matrices `ms` (non-integer, at least kVerifyMinUnits items): for each, up
@@ -317,9 +331,10 @@ int build(perm_batch* B) {
 // list, records in a separate array), and err_est becomes the estimate from
 // their measured errors (perm::sampled_error); values are unchanged.
 // Escalation: for each (matrix, mode) a persistent kernel of that mode's
-// variant over the matrix's items (double-double X runs the plain walk),
-// spread over the streams; then the per-matrix reduction and the results of
-// those matrices refilled in the new mode.
+// variant over the matrix's items (double-double X runs the plain walk,
+// finer for small matrices as above), spread over the streams; then the
+// per-matrix reduction and the results of those matrices refilled in the
+// new mode.
 int run_round(perm_batch* B, const std::vector<size_t>& ms,
               const std::vector<std::pair<size_t, int>>& esc, perm_result* results,
               float& pass_ms) {
@@ -354,11 +369,74 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     picks[k].begin = (uint32_t)vitems.size();
     for (uint32_t i : picks[k].ids) vitems.push_back(B->items[e.first_item + i]);
   }
-  BatchItem* d_vitems = nullptr;
-  Partial* d_vparts = nullptr;
-  if (!vitems.empty()) {
-    const size_t it_b = align256(vitems.size() * sizeof(BatchItem));
-    const size_t need = it_b + align256(vitems.size() * sizeof(Partial));
+  // the passes: coarse (the matrix's own items) or fine (its own split)
+  struct Fine {
+    size_t i;  // index in esc
+    int walk, high;
+    uint32_t count;
+  };
+  std::vector<size_t> coarse;
+  std::vector<Fine> fine;
+  for (size_t i = 0; i < esc.size(); ++i) {
+    const Entry& e = B->ents[esc[i].first];
+    const int w = std::max(std::min(perm::dense_unroll_bits(e.n), e.walk_bits),
+                           e.walk_bits - kFineBits);
+    if (esc[i].second != 2 || e.item_count > kFineMaxItems || w >= e.walk_bits) {
+      coarse.push_back(i);
+      continue;
+    }
+    const int h = e.walk_bits + e.high_bits - w;
+    const uint64_t cnt = ((1ull << h) + perm::kItemLanes - 1) / perm::kItemLanes;
+    fine.push_back({i, w, h, (uint32_t)cnt});
+  }
+  // launch sets: the first holds the sample, the coarse passes and the first
+  // chunk of fine passes; each further set one more chunk
+  std::vector<std::pair<size_t, size_t>> chunks;  // [begin, end) in fine
+  for (size_t f = 0; f < fine.size();) {
+    size_t g = f, items = 0;
+    while (g < fine.size() && (g == f || items + fine[g].count <= kFineChunkItems))
+      items += fine[g++].count;
+    chunks.push_back({f, g});
+    f = g;
+  }
+  const size_t n_sets = std::max<size_t>(1, chunks.size());
+  uint32_t* ctr = B->d_counters + B->groups.size();
+  const uint32_t nm = (uint32_t)B->ents.size();
+  std::vector<Partial> ref(vitems.size());
+  for (size_t set = 0; set < n_sets; ++set) {
+    const bool first = set == 0;
+    const size_t f0 = chunks.empty() ? 0 : chunks[set].first;
+    const size_t f1 = chunks.empty() ? 0 : chunks[set].second;
+    // this set's fine matrices (device copies of the entry with the fine
+    // split) and items
+    std::vector<BatchMatrix> fmats;
+    std::vector<BatchItem> fitems;
+    for (size_t f = f0; f < f1; ++f) {
+      const Entry& e = B->ents[esc[fine[f].i].first];
+      BatchMatrix bm = device_matrix(B, e);
+      bm.lanes = 1ull << fine[f].high;
+      bm.walk_bits = fine[f].walk;
+      bm.high_bits = fine[f].high;
+      bm.first_item = (uint32_t)fitems.size();
+      bm.item_count = fine[f].count;
+      const uint32_t fm = (uint32_t)fmats.size();
+      fmats.push_back(bm);
+      for (uint32_t k = 0; k < fine[f].count; ++k)
+        fitems.push_back({fm, 0u, (uint64_t)k * perm::kItemLanes});
+    }
+    const size_t nv = first ? vitems.size() : 0;
+    const size_t n_coarse = first ? coarse.size() : 0;
+    const size_t n_samp = first ? ms.size() : 0;
+    if (nv == 0 && n_coarse == 0 && fmats.empty()) continue;
+    // scratch: sample items and records, fine matrices, items, records and
+    // results
+    const size_t vi_b = align256(nv * sizeof(BatchItem));
+    const size_t vp_b = align256(nv * sizeof(Partial));
+    const size_t fm_b = align256(fmats.size() * sizeof(BatchMatrix));
+    const size_t fi_b = align256(fitems.size() * sizeof(BatchItem));
+    const size_t fp_b = align256(fitems.size() * sizeof(Partial));
+    const size_t fr_b = align256(fmats.size() * sizeof(Partial));
+    const size_t need = vi_b + vp_b + fm_b + fi_b + fp_b + fr_b;
     if (need > B->vbytes) {
       if (B->vmem) HIP_TRYB(hipFree(B->vmem));
       B->vmem = nullptr;
@@ -366,62 +444,99 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
       HIP_TRYB(hipMalloc(&B->vmem, need));
       B->vbytes = need;
     }
-    d_vitems = (BatchItem*)B->vmem;
-    d_vparts = (Partial*)((char*)B->vmem + it_b);
+    char* vm = (char*)B->vmem;
+    BatchItem* d_vitems = (BatchItem*)vm;
+    Partial* d_vparts = (Partial*)(vm + vi_b);
+    BatchMatrix* d_fmats = (BatchMatrix*)(vm + vi_b + vp_b);
+    BatchItem* d_fitems = (BatchItem*)(vm + vi_b + vp_b + fm_b);
+    Partial* d_fparts = (Partial*)(vm + vi_b + vp_b + fm_b + fi_b);
+    Partial* d_fres = (Partial*)(vm + vi_b + vp_b + fm_b + fi_b + fp_b);
+    // one counter per launched kernel (every matrix is in at most one list)
+    const size_t n_ctr = n_samp + n_coarse + fmats.size();
+    if (n_ctr > nm) return perm::fail(PERM_ERR_ARG, "escalation round counters");
+    HIP_TRYB(hipEventRecord(B->ev0, s0));
+    if (nv)
+      HIP_TRYB(hipMemcpyAsync(d_vitems, vitems.data(), nv * sizeof(BatchItem),
+                              hipMemcpyHostToDevice, s0));
+    if (!fmats.empty()) {
+      HIP_TRYB(hipMemcpyAsync(d_fmats, fmats.data(), fmats.size() * sizeof(BatchMatrix),
+                              hipMemcpyHostToDevice, s0));
+      HIP_TRYB(hipMemcpyAsync(d_fitems, fitems.data(), fitems.size() * sizeof(BatchItem),
+                              hipMemcpyHostToDevice, s0));
+    }
+    HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)n_ctr, s0));
+    HIP_TRYB(hipEventRecord(B->ev_fork, s0));
+    for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
+    size_t j = 0;  // kernel index: counter and stream
+    for (size_t k = 0; k < n_samp; ++k, ++j) {
+      const Entry& e = B->ents[ms[k]];
+      const uint32_t cnt = (uint32_t)picks[k].ids.size();
+      if (cnt == 0) continue;
+      const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
+      if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
+      const uint32_t grid = std::max(1u, std::min<uint32_t>((cnt + 3) / 4, B->max_grid));
+      HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, d_vitems, picks[k].begin,
+                                          picks[k].begin + cnt, ctr + j, d_vparts, grid,
+                                          B->s[j % ns]));
+    }
+    for (size_t c = 0; c < n_coarse; ++c, ++j) {
+      const Entry& e = B->ents[esc[coarse[c]].first];
+      const int md = esc[coarse[c]].second;
+      const perm::NestShape sh = md == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+      const int vid = perm::batch_variant(e.n, sh, md);
+      if (vid < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel of this mode for this n");
+      const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
+      HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
+                                          e.first_item + e.item_count, ctr + j, B->d_partials,
+                                          grid, B->s[j % ns]));
+    }
+    for (size_t fm = 0; fm < fmats.size(); ++fm, ++j) {
+      const Entry& e = B->ents[esc[fine[f0 + fm].i].first];
+      const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
+      if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
+      const uint32_t cnt = fmats[fm].item_count;
+      const uint32_t grid = std::max(1u, std::min<uint32_t>((cnt + 3) / 4, B->max_grid));
+      HIP_TRYB(perm::launch_batch_variant(vid, d_fmats, d_fitems, fmats[fm].first_item,
+                                          fmats[fm].first_item + cnt, ctr + j, d_fparts, grid,
+                                          B->s[j % ns]));
+    }
+    for (int i = 1; i < ns; ++i) {
+      HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
+      HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
+    }
+    if (n_coarse) {
+      HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
+      HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial),
+                              hipMemcpyDeviceToHost, s0));
+    }
+    std::vector<Partial> fres(fmats.size());
+    if (!fmats.empty()) {
+      HIP_TRYB(perm::launch_batch_reduce(d_fmats, d_fparts, d_fres, (uint32_t)fmats.size(), s0));
+      HIP_TRYB(hipMemcpyAsync(fres.data(), d_fres, fmats.size() * sizeof(Partial),
+                              hipMemcpyDeviceToHost, s0));
+    }
+    if (nv)
+      HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, nv * sizeof(Partial), hipMemcpyDeviceToHost,
+                              s0));
+    HIP_TRYB(hipEventRecord(B->ev1, s0));
+    HIP_TRYB(hipStreamSynchronize(s0));
+    float set_ms = 0.f;
+    HIP_TRYB(hipEventElapsedTime(&set_ms, B->ev0, B->ev1));
+    pass_ms += set_ms;
+    for (size_t c = 0; c < n_coarse; ++c) {
+      const auto& em = esc[coarse[c]];
+      const Entry& e = B->ents[em.first];
+      const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
+      perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
+                             B->h_results[em.first], results[em.first]);
+    }
+    for (size_t fm = 0; fm < fmats.size(); ++fm) {
+      const Fine& f = fine[f0 + fm];
+      const Entry& e = B->ents[esc[f.i].first];
+      perm::fill_item_result(e.n, f.walk, e.integral, e.product_bits,
+                             perm::NestShape{e.n, e.n, e.n}, 2, fres[fm], results[esc[f.i].first]);
+    }
   }
-  if (vitems.empty() && esc.empty()) return PERM_OK;
-  // one counter per launched kernel: the sample's first, then the passes'
-  // (the two sets are disjoint, so at most one counter per matrix)
-  uint32_t* ctr = B->d_counters + B->groups.size();
-  const size_t n_ctr = ms.size() + esc.size();
-  if (n_ctr > B->ents.size()) return perm::fail(PERM_ERR_ARG, "escalation round counters");
-  HIP_TRYB(hipEventRecord(B->ev0, s0));
-  if (!vitems.empty())
-    HIP_TRYB(hipMemcpyAsync(d_vitems, vitems.data(), vitems.size() * sizeof(BatchItem),
-                            hipMemcpyHostToDevice, s0));
-  HIP_TRYB(perm::launch_batch_reset(ctr, (uint32_t)n_ctr, s0));
-  HIP_TRYB(hipEventRecord(B->ev_fork, s0));
-  for (int i = 1; i < ns; ++i) HIP_TRYB(hipStreamWaitEvent(B->s[i], B->ev_fork, 0));
-  for (size_t k = 0; k < ms.size(); ++k) {
-    const Entry& e = B->ents[ms[k]];
-    const uint32_t cnt = (uint32_t)picks[k].ids.size();
-    if (cnt == 0) continue;
-    const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
-    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((cnt + 3) / 4, B->max_grid));
-    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, d_vitems, picks[k].begin,
-                                        picks[k].begin + cnt, ctr + k, d_vparts, grid,
-                                        B->s[k % ns]));
-  }
-  for (size_t i = 0; i < esc.size(); ++i) {
-    const size_t j = ms.size() + i;
-    const Entry& e = B->ents[esc[i].first];
-    const int md = esc[i].second;
-    const perm::NestShape sh = md == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
-    const int vid = perm::batch_variant(e.n, sh, md);
-    if (vid < 0) return perm::fail(PERM_ERR_ARG, "no batch kernel of this mode for this n");
-    const uint32_t grid = std::max(1u, std::min<uint32_t>((e.item_count + 3) / 4, B->max_grid));
-    HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, B->d_items, e.first_item,
-                                        e.first_item + e.item_count, ctr + j, B->d_partials, grid,
-                                        B->s[j % ns]));
-  }
-  for (int i = 1; i < ns; ++i) {
-    HIP_TRYB(hipEventRecord(B->ev_join[i], B->s[i]));
-    HIP_TRYB(hipStreamWaitEvent(s0, B->ev_join[i], 0));
-  }
-  const uint32_t nm = (uint32_t)B->ents.size();
-  if (!esc.empty()) {
-    HIP_TRYB(perm::launch_batch_reduce(B->d_mats, B->d_partials, B->d_results, nm, s0));
-    HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial),
-                            hipMemcpyDeviceToHost, s0));
-  }
-  std::vector<Partial> ref(vitems.size());
-  if (!vitems.empty())
-    HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, vitems.size() * sizeof(Partial),
-                            hipMemcpyDeviceToHost, s0));
-  HIP_TRYB(hipEventRecord(B->ev1, s0));
-  HIP_TRYB(hipStreamSynchronize(s0));
-  HIP_TRYB(hipEventElapsedTime(&pass_ms, B->ev0, B->ev1));
   for (size_t k = 0; k < ms.size(); ++k) {
     const Entry& e = B->ents[ms[k]];
     const Pick& pk = picks[k];
@@ -435,12 +550,6 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     perm_result& o = results[ms[k]];
     o.err_est = perm::sampled_error(err, pk.weight, pk.certain, e.n, o.value);
     o.certified = perm::is_certified(o) ? 1 : 0;
-  }
-  for (const auto& em : esc) {
-    const Entry& e = B->ents[em.first];
-    const perm::NestShape sh = em.second == 2 ? perm::NestShape{e.n, e.n, e.n} : e.shape;
-    perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
-                           B->h_results[em.first], results[em.first]);
   }
   return PERM_OK;
 }
