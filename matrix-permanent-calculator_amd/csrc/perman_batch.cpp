@@ -351,13 +351,21 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
                             e.item_count * sizeof(Partial), hipMemcpyDeviceToHost, s0));
   }
   if (!ms.empty()) HIP_TRYB(hipStreamSynchronize(s0));
+  // A picked item is walked on the fine split when its matrix has one (the
+  // item's 64 lanes of 2^w steps are fine items 16 i .. 16 i + 15, lanes of
+  // 2^(w-4) steps): its double-double value is then the sum of those 16
+  // records in order.
   struct Pick {
     std::vector<uint32_t> ids;
     std::vector<double> weight;
     uint32_t certain = 0, begin = 0;
+    int fine_walk = 0;  // > 0: walked on the fine split of this many walk bits
   };
+  constexpr uint32_t kSub = 1u << kFineBits;
   std::vector<Pick> picks(ms.size());
-  std::vector<BatchItem> vitems;
+  std::vector<BatchItem> vitems;  // coarse picks
+  std::vector<BatchMatrix> sfmats;  // fine picks: one fine matrix per sampled matrix
+  std::vector<BatchItem> sfitems;
   for (size_t k = 0; k < ms.size(); ++k) {
     const Entry& e = B->ents[ms[k]];
     std::vector<double> z(e.item_count);
@@ -366,9 +374,29 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
       z[i] = perm::unit_weight(p.sq_sum, p.sq_run, e.walk_bits, e.high_bits);
     }
     perm::pps_sample(z, perm::kVerifyUnits, picks[k].ids, picks[k].weight, picks[k].certain);
-    picks[k].begin = (uint32_t)vitems.size();
-    for (uint32_t i : picks[k].ids) vitems.push_back(B->items[e.first_item + i]);
+    const int w = std::max(std::min(perm::dense_unroll_bits(e.n), e.walk_bits),
+                           e.walk_bits - kFineBits);
+    if (w == e.walk_bits - kFineBits && e.high_bits >= 6) {
+      // (high_bits >= 6: every coarse item is 64 whole lanes)
+      picks[k].fine_walk = w;
+      picks[k].begin = (uint32_t)sfitems.size();
+      BatchMatrix bm = device_matrix(B, e);
+      bm.lanes = 1ull << (e.high_bits + kFineBits);
+      bm.walk_bits = w;
+      bm.high_bits = e.high_bits + kFineBits;
+      bm.first_item = picks[k].begin;
+      bm.item_count = (uint32_t)picks[k].ids.size() * kSub;
+      const uint32_t fm = (uint32_t)sfmats.size();
+      sfmats.push_back(bm);
+      for (uint32_t i : picks[k].ids)
+        for (uint32_t t = 0; t < kSub; ++t)
+          sfitems.push_back({fm, 0u, ((uint64_t)i * kSub + t) * perm::kItemLanes});
+    } else {
+      picks[k].begin = (uint32_t)vitems.size();
+      for (uint32_t i : picks[k].ids) vitems.push_back(B->items[e.first_item + i]);
+    }
   }
+  const size_t n_sfi = sfitems.size();
   // the passes: coarse (the matrix's own items) or fine (its own split)
   struct Fine {
     size_t i;  // index in esc
@@ -402,7 +430,7 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
   const size_t n_sets = std::max<size_t>(1, chunks.size());
   uint32_t* ctr = B->d_counters + B->groups.size();
   const uint32_t nm = (uint32_t)B->ents.size();
-  std::vector<Partial> ref(vitems.size());
+  std::vector<Partial> ref(vitems.size()), sfref(n_sfi);
   for (size_t set = 0; set < n_sets; ++set) {
     const bool first = set == 0;
     const size_t f0 = chunks.empty() ? 0 : chunks[set].first;
@@ -411,6 +439,11 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     // split) and items
     std::vector<BatchMatrix> fmats;
     std::vector<BatchItem> fitems;
+    if (first) {  // the fine picks first: their fine matrices, then items [0, n_sfi)
+      fmats = sfmats;
+      fitems = sfitems;
+    }
+    const size_t fe0 = fmats.size();  // this set's fine passes: fine matrices [fe0, ...)
     for (size_t f = f0; f < f1; ++f) {
       const Entry& e = B->ents[esc[fine[f].i].first];
       BatchMatrix bm = device_matrix(B, e);
@@ -427,6 +460,7 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     const size_t nv = first ? vitems.size() : 0;
     const size_t n_coarse = first ? coarse.size() : 0;
     const size_t n_samp = first ? ms.size() : 0;
+    const size_t n_pass = fmats.size() - fe0;
     if (nv == 0 && n_coarse == 0 && fmats.empty()) continue;
     // scratch: sample items and records, fine matrices, items, records and
     // results
@@ -435,7 +469,7 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     const size_t fm_b = align256(fmats.size() * sizeof(BatchMatrix));
     const size_t fi_b = align256(fitems.size() * sizeof(BatchItem));
     const size_t fp_b = align256(fitems.size() * sizeof(Partial));
-    const size_t fr_b = align256(fmats.size() * sizeof(Partial));
+    const size_t fr_b = align256(n_pass * sizeof(Partial));
     const size_t need = vi_b + vp_b + fm_b + fi_b + fp_b + fr_b;
     if (need > B->vbytes) {
       if (B->vmem) HIP_TRYB(hipFree(B->vmem));
@@ -452,7 +486,7 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     Partial* d_fparts = (Partial*)(vm + vi_b + vp_b + fm_b + fi_b);
     Partial* d_fres = (Partial*)(vm + vi_b + vp_b + fm_b + fi_b + fp_b);
     // one counter per launched kernel (every matrix is in at most one list)
-    const size_t n_ctr = n_samp + n_coarse + fmats.size();
+    const size_t n_ctr = n_samp + n_coarse + n_pass;
     if (n_ctr > nm) return perm::fail(PERM_ERR_ARG, "escalation round counters");
     HIP_TRYB(hipEventRecord(B->ev0, s0));
     if (nv)
@@ -470,14 +504,20 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     size_t j = 0;  // kernel index: counter and stream
     for (size_t k = 0; k < n_samp; ++k, ++j) {
       const Entry& e = B->ents[ms[k]];
-      const uint32_t cnt = (uint32_t)picks[k].ids.size();
+      const bool fine_k = picks[k].fine_walk > 0;
+      const uint32_t cnt = (uint32_t)picks[k].ids.size() * (fine_k ? kSub : 1u);
       if (cnt == 0) continue;
       const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
       if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
       const uint32_t grid = std::max(1u, std::min<uint32_t>((cnt + 3) / 4, B->max_grid));
-      HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, d_vitems, picks[k].begin,
-                                          picks[k].begin + cnt, ctr + j, d_vparts, grid,
-                                          B->s[j % ns]));
+      if (fine_k)
+        HIP_TRYB(perm::launch_batch_variant(vid, d_fmats, d_fitems, picks[k].begin,
+                                            picks[k].begin + cnt, ctr + j, d_fparts, grid,
+                                            B->s[j % ns]));
+      else
+        HIP_TRYB(perm::launch_batch_variant(vid, B->d_mats, d_vitems, picks[k].begin,
+                                            picks[k].begin + cnt, ctr + j, d_vparts, grid,
+                                            B->s[j % ns]));
     }
     for (size_t c = 0; c < n_coarse; ++c, ++j) {
       const Entry& e = B->ents[esc[coarse[c]].first];
@@ -490,8 +530,8 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
                                           e.first_item + e.item_count, ctr + j, B->d_partials,
                                           grid, B->s[j % ns]));
     }
-    for (size_t fm = 0; fm < fmats.size(); ++fm, ++j) {
-      const Entry& e = B->ents[esc[fine[f0 + fm].i].first];
+    for (size_t fm = fe0; fm < fmats.size(); ++fm, ++j) {
+      const Entry& e = B->ents[esc[fine[f0 + fm - fe0].i].first];
       const int vid = perm::batch_variant(e.n, perm::NestShape{e.n, e.n, e.n}, 2);
       if (vid < 0) return perm::fail(PERM_ERR_ARG, "no double-double X batch kernel for this n");
       const uint32_t cnt = fmats[fm].item_count;
@@ -509,12 +549,15 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
       HIP_TRYB(hipMemcpyAsync(B->h_results, B->d_results, nm * sizeof(Partial),
                               hipMemcpyDeviceToHost, s0));
     }
-    std::vector<Partial> fres(fmats.size());
-    if (!fmats.empty()) {
-      HIP_TRYB(perm::launch_batch_reduce(d_fmats, d_fparts, d_fres, (uint32_t)fmats.size(), s0));
-      HIP_TRYB(hipMemcpyAsync(fres.data(), d_fres, fmats.size() * sizeof(Partial),
+    std::vector<Partial> fres(n_pass);
+    if (n_pass) {
+      HIP_TRYB(perm::launch_batch_reduce(d_fmats + fe0, d_fparts, d_fres, (uint32_t)n_pass, s0));
+      HIP_TRYB(hipMemcpyAsync(fres.data(), d_fres, n_pass * sizeof(Partial),
                               hipMemcpyDeviceToHost, s0));
     }
+    if (first && n_sfi)
+      HIP_TRYB(hipMemcpyAsync(sfref.data(), d_fparts, n_sfi * sizeof(Partial),
+                              hipMemcpyDeviceToHost, s0));
     if (nv)
       HIP_TRYB(hipMemcpyAsync(ref.data(), d_vparts, nv * sizeof(Partial), hipMemcpyDeviceToHost,
                               s0));
@@ -530,7 +573,7 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
       perm::fill_item_result(e.n, e.walk_bits, e.integral, e.product_bits, sh, em.second,
                              B->h_results[em.first], results[em.first]);
     }
-    for (size_t fm = 0; fm < fmats.size(); ++fm) {
+    for (size_t fm = 0; fm < n_pass; ++fm) {
       const Fine& f = fine[f0 + fm];
       const Entry& e = B->ents[esc[f.i].first];
       perm::fill_item_result(e.n, f.walk, e.integral, e.product_bits,
@@ -544,7 +587,15 @@ int run_round(perm_batch* B, const std::vector<size_t>& ms,
     std::vector<double> err(pk.ids.size());
     for (size_t j = 0; j < pk.ids.size(); ++j) {
       const Partial& f = fp[k][pk.ids[j]];
-      const Partial& r = ref[pk.begin + j];
+      perm::HostDD r{0.0, 0.0};
+      if (pk.fine_walk > 0) {
+        for (uint32_t t = 0; t < kSub; ++t) {
+          const Partial& q = sfref[pk.begin + j * kSub + t];
+          r = perm::host_dd_add(r, {q.hi, q.lo});
+        }
+      } else {
+        r = {ref[pk.begin + j].hi, ref[pk.begin + j].lo};
+      }
       err[j] = (f.hi - r.hi) + (f.lo - r.lo);
     }
     perm_result& o = results[ms[k]];
