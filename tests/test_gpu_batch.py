@@ -241,8 +241,8 @@ def test_replays_with_escalations_same_bits():
     """The batched configuration's mix, whose hardest members escalate (a
     sampled double-double measurement, then full double-double passes):
     every graph replay gives the first run's bits, err_est, mode and
-    certificate for every member, and a fresh batch computed in one call
-    gives the same bits."""
+    certificate for every member, a fresh batch computed in one call gives
+    the same bits, and so do the escalated members in a batch of their own."""
     mats = P.mixed_batch(4096, 12, 28, 0.2, 0.8, 5)
     b = P.Batch(item_log2=16, streams=8, max_grid=512)
     b.add_prepared(P.BatchInputs(mats), threads=8)
@@ -261,5 +261,17 @@ def test_replays_with_escalations_same_bits():
     one.close()
     for i, (a, r) in enumerate(zip(first, res_c)):
         assert (a.value, a.sum_hi, a.sum_lo, a.mode) == (r.value, r.sum_hi, r.sum_lo, r.mode), i
+    # the escalated members alone: every decision (sampling, the split of the
+    # double-double passes, the rounds) depends on the matrix only
+    esc = [i for i, r in enumerate(first) if r.mode != 0]
+    alone = P.Batch(item_log2=16, streams=8, max_grid=512)
+    for i in esc:
+        alone.add(mats[i])
+    res_alone, _ = alone.run()
+    alone.close()
+    for i, r in zip(esc, res_alone):
+        a = first[i]
+        assert (a.value, a.sum_hi, a.sum_lo, a.err_est, a.mode, a.certified) == (
+            r.value, r.sum_hi, r.sum_lo, r.err_est, r.mode, r.certified), i
     print("escalated:", escalated, "device ms per run:", [round(ms, 2) for _, ms in runs])
 
