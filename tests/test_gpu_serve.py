@@ -262,6 +262,30 @@ def test_server_sampled_error_measurement():
     print("dd passes:", st["dd_passes"], [(r.mode, r.err_est / abs(r.value)) for r, _ in res])
 
 
+def test_server_escalated_request_same_bits_alone_and_in_mix():
+    """A request whose FP64 pass does not meet the target (its error
+    measured on a sample of items in the double-double X mode, and a full
+    pass if that does not certify) gives the same bits, err_est, mode and
+    certificate alone and among 120 concurrent requests."""
+    targets = [_sparse_ill_conditioned(28, 28020), _sparse_ill_conditioned(30, 30020)]
+    with P.Server() as srv:
+        alone = [srv.wait(srv.submit(t))[0] for t in targets]
+    others = P.served_workload(120, seed=9, n_lo=12, n_hi=28)
+    with P.Server(grid=256) as srv:
+        ids = [srv.submit(m, sparse_path=sp) for m, sp in others[:60]]
+        tids = [srv.submit(t) for t in targets]
+        ids += [srv.submit(m, sparse_path=sp) for m, sp in others[60:]]
+        mixed = [srv.wait(t)[0] for t in tids]
+        srv.wait_many(ids)
+        st = srv.stats()
+    assert st["dd_passes"] >= 2
+    for a, r in zip(alone, mixed):
+        assert r.mode != 0 or r.err_est > 0.0
+        assert (a.value, a.sum_hi, a.sum_lo, a.err_est, a.mode, a.certified) == (
+            r.value, r.sum_hi, r.sum_lo, r.err_est, r.mode, r.certified)
+        assert r.certified
+
+
 def test_server_relaunch_after_idle_and_arena_wrap():
     """The persistent kernel leaves after idle_us without work and the next
     submission launches it again; with a 16 MiB arena the block ring wraps
