@@ -217,9 +217,9 @@ void perm_plan_destroy(perm_plan* plan);
  * whose FP64 error could reach 1/2 then get a double-double pass over the
  * same items (the batch kernels' double-double variant).  A matrix is cut into 64-lane
  * items of about 2^item_log2 subsets that depend only on its own n, so its
- * result is identical alone and in any mix.  item_log2 (default 18),
- * streams (default 4) and max_grid (workgroups per variant kernel, default
- * 2048) are the work-split parameters (<= 0 keeps a value). */
+ * result is identical alone and in any mix.  item_log2 (default 17),
+ * streams (default 8) and max_grid (workgroups per variant kernel, default
+ * 1024) are the work-split parameters (<= 0 keeps a value). */
 typedef struct perm_batch perm_batch;
 int perm_batch_create(perm_batch** batch);
 int perm_batch_set(perm_batch* batch, int item_log2, int streams, int max_grid);

@@ -52,9 +52,9 @@ struct Entry {
 
 struct perm_batch {
   int device = 0;
-  int item_log2 = 18;   // subsets per item (64 lanes x 2^(item_log2-6) steps)
-  int streams = 4;
-  int max_grid = 2048;  // workgroups per variant kernel
+  int item_log2 = 17;   // subsets per item (64 lanes x 2^(item_log2-6) steps)
+  int streams = 8;
+  int max_grid = 1024;  // workgroups per variant kernel
   std::vector<double> data;
   std::vector<Entry> ents;
   std::vector<BatchItem> items;
