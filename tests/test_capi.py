@@ -359,3 +359,22 @@ def test_lib_connect_exports_the_reference_connector_names():
     lib.read_calculate_return.restype = ctypes.c_double
     for algo in (0, 1, 2, 3):
         assert math.isnan(lib.read_calculate_return(b"/nonexistent", algo, 1, 1, 1, 1))
+
+
+def test_bench_contract_one_gpu_and_leg_options():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    so (before touching a device); its options select the legs."""
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2, (r.returncode, r.stderr)
+    assert "measures one GPU" in r.stderr
+    assert r.stdout == ""
+    h = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
+                       capture_output=True, text=True, timeout=120)
+    assert h.returncode == 0
+    for opt in ("--steps", "--warmup", "--no-cpu", "--no-mixed", "--batch-count",
+                "--serve-count", "--serve-mixes", "--serve-load"):
+        assert opt in h.stdout, opt
+
